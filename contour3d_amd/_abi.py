@@ -1,0 +1,65 @@
+"""ctypes mirrors of the public C structs in include/contour3d.h.
+
+The layouts follow the reference interface member for member
+(include/contour3d.h:10-56); tests/test_abi.py checks sizes and offsets
+against the C compiler's view of the same header.
+"""
+import ctypes as C
+
+
+class Vector(C.Structure):
+    """contour3d_vector_t, include/contour3d.h:10-14."""
+
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
+
+    def __iter__(self):
+        return iter((self.x, self.y, self.z))
+
+
+class Color(C.Structure):
+    """contour3d_color_t, include/contour3d.h:16-20."""
+
+    _fields_ = [("r", C.c_uint8), ("g", C.c_uint8), ("b", C.c_uint8)]
+
+
+# contour3d_vector_t (*)(const contour3d_vector_t)
+Converter = C.CFUNCTYPE(Vector, Vector)
+
+
+class ContourObj(C.Structure):
+    """contour3d_contour_obj_t, include/contour3d.h:22-40."""
+
+    _fields_ = [
+        ("pencil", C.c_int),
+        ("glsizes", C.c_size_t * 3),
+        ("grids", C.POINTER(C.c_double) * 3),
+        ("converter", C.c_void_p),
+        ("threshold", C.c_double),
+        ("color", Color),
+        ("array", C.c_void_p),
+    ]
+
+
+class LineObj(C.Structure):
+    """contour3d_line_obj_t, include/contour3d.h:42-56."""
+
+    _fields_ = [
+        ("edges", Vector * 2),
+        ("nitems", C.c_size_t),
+        ("converter", C.c_void_p),
+        ("color", Color),
+        ("width", C.c_double),
+    ]
+
+
+def vec(v):
+    return Vector(float(v[0]), float(v[1]), float(v[2]))
+
+
+def color(c):
+    return Color(int(c[0]), int(c[1]), int(c[2]))
+
+
+def dptr(addr):
+    """A raw address as a double* for the grids[] members."""
+    return C.cast(C.c_void_p(addr), C.POINTER(C.c_double))
