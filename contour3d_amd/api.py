@@ -1,0 +1,271 @@
+"""ctypes front end of libcontour3d.so.
+
+`execute` mirrors the reference's single public entry point
+contour3d_execute (include/contour3d.h:58-86): same arguments, same
+0/1-plus-stderr error behaviour (raised here as RuntimeError), same PPM
+output.  `execute_rgb`, `Mesh` and the profiling calls wrap the extensions
+of include/contour3d_amd.h.  Fields may be numpy arrays (host) or torch
+device tensors (used in place, no copy).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcontour3d.so")
+
+FP64 = 0
+FP32 = 1
+STAGES = ("count", "scan", "emit", "smooth", "raster", "resolve")
+
+# every entry point declared in include/contour3d.h and include/contour3d_amd.h
+EXPORTS = (
+    "contour3d_execute",
+    "contour3d_amd_init",
+    "contour3d_amd_finalize",
+    "contour3d_amd_set_precision",
+    "contour3d_amd_identity",
+    "contour3d_amd_execute_rgb",
+    "contour3d_amd_mesh_build",
+    "contour3d_amd_mesh_count",
+    "contour3d_amd_mesh_copy",
+    "contour3d_amd_mesh_render",
+    "contour3d_amd_mesh_free",
+    "contour3d_amd_profile_enable",
+    "contour3d_amd_profile_read",
+    "contour3d_amd_profile_reset",
+)
+
+_lib = None
+
+
+def lib():
+    """Load the native library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"native library missing: {LIB_PATH} (run python contour3d_amd/build.py)")
+        L = C.CDLL(LIB_PATH)
+        V = C.POINTER(_abi.Vector)
+        L.contour3d_execute.restype = C.c_int
+        L.contour3d_execute.argtypes = [C.c_void_p, V, V, V, C.POINTER(C.c_size_t), V, V,
+                                        C.POINTER(_abi.Color), C.c_size_t, C.POINTER(_abi.ContourObj),
+                                        C.c_size_t, C.POINTER(_abi.LineObj), C.c_char_p]
+        L.contour3d_amd_execute_rgb.restype = C.c_int
+        L.contour3d_amd_execute_rgb.argtypes = [V, V, C.POINTER(C.c_size_t), V, V, C.POINTER(_abi.Color),
+                                                C.c_size_t, C.POINTER(_abi.ContourObj), C.c_size_t,
+                                                C.POINTER(_abi.LineObj), C.c_void_p, C.c_void_p, C.c_int]
+        L.contour3d_amd_init.argtypes = [C.c_int]
+        L.contour3d_amd_set_precision.argtypes = [C.c_int]
+        L.contour3d_amd_mesh_build.argtypes = [C.POINTER(_abi.ContourObj), C.c_size_t, C.c_int,
+                                               C.POINTER(C.c_void_p)]
+        L.contour3d_amd_mesh_count.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        L.contour3d_amd_mesh_copy.argtypes = [C.c_void_p, C.c_size_t] + [C.c_void_p] * 5
+        L.contour3d_amd_mesh_render.argtypes = [C.c_void_p, V, V, C.POINTER(C.c_size_t), V, V,
+                                                C.POINTER(_abi.Color), C.c_size_t, C.POINTER(_abi.LineObj),
+                                                C.c_void_p, C.c_void_p]
+        L.contour3d_amd_mesh_free.argtypes = [C.c_void_p]
+        L.contour3d_amd_profile_enable.argtypes = [C.c_int]
+        L.contour3d_amd_profile_read.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def identity_address():
+    return C.cast(lib().contour3d_amd_identity, C.c_void_p).value
+
+
+def _addr(a):
+    """Data address of a numpy array or a torch tensor."""
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+def _shape(a):
+    return tuple(int(x) for x in a.shape)
+
+
+class _Keep:
+    """Holds the buffers that ctypes structs point into."""
+
+    def __init__(self):
+        self.items = []
+
+
+def contour_objs(contours, keep):
+    """ctypes array of contour3d_contour_obj_t from dicts with keys field
+    (nz, ny, nx), threshold, color and optional grids / converter."""
+    ident = identity_address()
+    objs = (_abi.ContourObj * max(1, len(contours)))()
+    for n, c in enumerate(contours):
+        f = c["field"]
+        if isinstance(f, np.ndarray):
+            f = np.ascontiguousarray(f, dtype=np.float64)
+        elif not f.is_contiguous():
+            raise ValueError("device fields must be contiguous")
+        nz, ny, nx = _shape(f)
+        grids = c.get("grids")
+        if grids is None:
+            grids = tuple(-0.5 + (np.arange(m) + 0.5) / m for m in (nx, ny, nz))
+        grids = [np.ascontiguousarray(g, dtype=np.float64) for g in grids]
+        keep.items += [f] + grids
+        conv = c.get("converter", ident)
+        if not isinstance(conv, int):
+            keep.items.append(conv)
+            conv = C.cast(conv, C.c_void_p).value
+        o = objs[n]
+        o.pencil = 0
+        o.glsizes[:] = [nx, ny, nz]
+        for d in range(3):
+            o.grids[d] = _abi.dptr(grids[d].ctypes.data)
+        o.converter = conv
+        o.threshold = float(c["threshold"])
+        o.color = _abi.color(c["color"])
+        o.array = _addr(f)
+    return objs
+
+
+def line_objs(lines, keep):
+    ident = identity_address()
+    objs = (_abi.LineObj * max(1, len(lines)))()
+    for n, ln in enumerate(lines):
+        conv = ln.get("converter", ident)
+        if not isinstance(conv, int):
+            keep.items.append(conv)
+            conv = C.cast(conv, C.c_void_p).value
+        o = objs[n]
+        o.edges[0] = _abi.vec(ln["edges"][0])
+        o.edges[1] = _abi.vec(ln["edges"][1])
+        o.nitems = int(ln["nitems"])
+        o.converter = conv
+        o.color = _abi.color(ln["color"])
+        o.width = float(ln["width"])
+    return objs
+
+
+def _view_args(camera, light, size, center, local, bg):
+    sizes = (C.c_size_t * 2)(int(size[0]), int(size[1]))
+    loc = (_abi.Vector * 2)(_abi.vec(local[0]), _abi.vec(local[1]))
+    return (C.byref(_abi.vec(camera)), C.byref(_abi.vec(light)), sizes, C.byref(_abi.vec(center)), loc,
+            C.byref(_abi.color(bg)))
+
+
+def execute(camera, look_at, light, size, center, local, bg, contours, lines=(), fname="output.ppm"):
+    """contour3d_execute: render and write a binary PPM to fname."""
+    keep = _Keep()
+    objs = contour_objs(contours, keep)
+    lobjs = line_objs(lines, keep)
+    sizes = (C.c_size_t * 2)(int(size[0]), int(size[1]))
+    loc = (_abi.Vector * 2)(_abi.vec(local[0]), _abi.vec(local[1]))
+    rc = lib().contour3d_execute(None, C.byref(_abi.vec(camera)), C.byref(_abi.vec(look_at)),
+                                 C.byref(_abi.vec(light)), sizes, C.byref(_abi.vec(center)), loc,
+                                 C.byref(_abi.color(bg)), len(contours), objs, len(lines), lobjs,
+                                 str(fname).encode())
+    if rc != 0:
+        raise RuntimeError("contour3d_execute failed")
+
+
+def execute_rgb(camera, light, size, center, local, bg, contours, lines=(), precision=FP64,
+                rgb=None, depth=None, want_depth=False):
+    """Render into rgb (numpy (H, W, 3) uint8 or a device tensor) and
+    optionally depth ((H, W) float64).  Returns (rgb, depth)."""
+    W, H = int(size[0]), int(size[1])
+    keep = _Keep()
+    objs = contour_objs(contours, keep)
+    lobjs = line_objs(lines, keep)
+    if rgb is None:
+        rgb = np.zeros((H, W, 3), np.uint8)
+    if depth is None and want_depth:
+        depth = np.zeros((H, W), np.float64)
+    rc = lib().contour3d_amd_execute_rgb(*_view_args(camera, light, size, center, local, bg),
+                                         len(contours), objs, len(lines), lobjs, _addr(rgb),
+                                         None if depth is None else _addr(depth), int(precision))
+    if rc != 0:
+        raise RuntimeError("contour3d_amd_execute_rgb failed")
+    return rgb, depth
+
+
+class Mesh:
+    """Tessellated and smoothed contour objects held on the device."""
+
+    def __init__(self, contours, precision=FP64):
+        self._keep = _Keep()
+        objs = contour_objs(contours, self._keep)
+        h = C.c_void_p()
+        if lib().contour3d_amd_mesh_build(objs, len(contours), int(precision), C.byref(h)) != 0:
+            raise RuntimeError("contour3d_amd_mesh_build failed")
+        self.handle = h
+        self.n = len(contours)
+
+    def count(self, obj):
+        n = C.c_size_t()
+        if lib().contour3d_amd_mesh_count(self.handle, obj, C.byref(n)) != 0:
+            raise RuntimeError("mesh_count failed")
+        return n.value
+
+    def copy(self, obj):
+        T = self.count(obj)
+        out = dict(pos=np.zeros((T, 3, 3)), wfn=np.zeros((T, 3)), vn=np.zeros((T, 3, 3)),
+                   cell=np.zeros(T, np.uint32), edges=np.zeros(T, np.uint32))
+        if T and lib().contour3d_amd_mesh_copy(self.handle, obj, *(out[k].ctypes.data for k in
+                                                                   ("pos", "wfn", "vn", "cell", "edges"))) != 0:
+            raise RuntimeError("mesh_copy failed")
+        e = out["edges"]
+        out["edge"] = np.stack([e & 31, (e >> 5) & 31, (e >> 10) & 31], axis=1).astype(np.int32)
+        out["interior"] = ((e >> 15) & 1).astype(bool)
+        return out
+
+    def render(self, camera, light, size, center, local, bg, lines=(), rgb=None, depth=None, want_depth=False):
+        W, H = int(size[0]), int(size[1])
+        keep = _Keep()
+        lobjs = line_objs(lines, keep)
+        if rgb is None:
+            rgb = np.zeros((H, W, 3), np.uint8)
+        if depth is None and want_depth:
+            depth = np.zeros((H, W), np.float64)
+        rc = lib().contour3d_amd_mesh_render(self.handle, *_view_args(camera, light, size, center, local, bg),
+                                             len(lines), lobjs, _addr(rgb),
+                                             None if depth is None else _addr(depth))
+        if rc != 0:
+            raise RuntimeError("mesh_render failed")
+        return rgb, depth
+
+    def free(self):
+        if self.handle:
+            lib().contour3d_amd_mesh_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def profile_enable(on=True):
+    lib().contour3d_amd_profile_enable(1 if on else 0)
+
+
+def profile_reset():
+    lib().contour3d_amd_profile_reset()
+
+
+def profile_read():
+    ms = np.zeros(len(STAGES), np.float64)
+    calls = np.zeros(len(STAGES), np.uint64)
+    nt = C.c_uint64()
+    lib().contour3d_amd_profile_read(ms.ctypes.data, calls.ctypes.data, C.byref(nt))
+    return dict(zip(STAGES, ms)), dict(zip(STAGES, calls)), nt.value
+
+
+def set_precision(p):
+    if lib().contour3d_amd_set_precision(int(p)) != 0:
+        raise RuntimeError("bad precision")
+
+
+def finalize():
+    lib().contour3d_amd_finalize()

@@ -1,0 +1,1087 @@
+// C ABI of the MI355X Contour3D library: contour3d_execute (contour3d.h) and
+// the extensions of contour3d_amd.h.  Host orchestration only; the compute
+// runs in the kernels of c3d_kernels.hip on one library stream.
+#include "contour3d.h"
+#include "contour3d_amd.h"
+#include "c3d_internal.h"
+
+#include <cerrno>
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+namespace c3d {
+namespace {
+
+// "[CONTOUR3D ERROR] " + message on stderr, the reference's convention
+// (src/contour3d/logger.c:5-18)
+void log_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  std::fprintf(stderr, "[CONTOUR3D ERROR] ");
+  std::vfprintf(stderr, fmt, ap);
+  std::fprintf(stderr, "\n");
+  std::fflush(stderr);
+  va_end(ap);
+}
+
+#define C3D_HIP(expr)                                                                  \
+  do {                                                                                 \
+    const hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess) {                                                            \
+      log_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return 1;                                                                        \
+    }                                                                                  \
+  } while (0)
+
+// grow-only device buffer
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() = default;
+  hipError_t ensure(size_t want) {
+    if (want <= bytes) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      bytes = 0;
+      if (e != hipSuccess) return e;
+    }
+    size_t n = want + want / 8 + 256;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    bytes = n;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// pinned host staging buffer
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t want) {
+    if (want <= bytes) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    bytes = want;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// lattice coordinates of one (converter, grids) geometry
+struct Geom {
+  void* conv = nullptr;
+  const double* g[3] = {nullptr, nullptr, nullptr};
+  size_t n[3] = {0, 0, 0};
+  uint64_t hash = 0;
+  bool identity = true;
+  DevBuf grids;  // nx + ny + nz doubles
+  DevBuf table;  // 3 * npts doubles when not the identity
+  uint64_t stamp = 0;
+};
+
+struct Frags {
+  std::vector<uint32_t> pix;
+  std::vector<double> depth;
+  std::vector<uint8_t> rgb;
+};
+
+struct Prof {
+  bool on = false;
+  double ms[CONTOUR3D_AMD_NSTAGES] = {};
+  uint64_t calls[CONTOUR3D_AMD_NSTAGES] = {};
+  uint64_t ntri = 0;
+  struct Pend {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Pend> pending;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t cur_a = nullptr;
+  int cur_stage = -1;
+};
+
+}  // namespace
+}  // namespace c3d
+
+struct contour3d_amd_mesh {
+  int precision = 0;
+  c3d::DevBuf pos, wfn, vn, cell, edges;
+  uint64_t cap = 0;
+  uint64_t total = 0;
+  std::vector<uint64_t> base, count;
+  std::vector<contour3d_color_t> color;
+  void release() {
+    pos.release();
+    wfn.release();
+    vn.release();
+    cell.release();
+    edges.release();
+    cap = 0;
+  }
+};
+
+namespace c3d {
+namespace {
+
+struct Lib {
+  std::mutex mu;
+  bool ready = false;
+  int device = 0;
+  hipStream_t st = nullptr;
+  int precision = CONTOUR3D_AMD_FP64;
+  NbrTable nbr{};
+  uint64_t stamp = 0;
+  std::vector<std::unique_ptr<Geom>> geoms;
+  std::vector<std::unique_ptr<DevBuf>> uploads;
+  DevBuf cnt, start, tiles, gidx, gout;
+  DevBuf key, id, rgb, depth, fpix, fdep, frgb, obase, orgb;
+  HostBuf hgout, hrgb;
+  // host sources of asynchronous uploads stay alive until the next call
+  std::vector<uint32_t> h_boff;
+  std::vector<uint64_t> h_obase;
+  std::vector<uint8_t> h_orgb;
+  Frags h_frags;
+  contour3d_amd_mesh frame_mesh;  // reused by contour3d_execute
+  Prof prof;
+};
+
+Lib& lib() {
+  static Lib L;
+  return L;
+}
+
+int ensure_init(Lib& L, int device) {
+  if (L.ready) return 0;
+  int ndev = 0;
+  C3D_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) {
+    log_error("device %d not available (%d devices)", device, ndev);
+    return 1;
+  }
+  C3D_HIP(hipSetDevice(device));
+  C3D_HIP(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
+  C3D_HIP(upload_tables());
+  L.nbr = build_nbr_table();
+  L.device = device;
+  L.ready = true;
+  return 0;
+}
+
+// ----------------------------------------------------------------------
+// profiling
+
+int prof_begin(Lib& L, int stage) {
+  Prof& P = L.prof;
+  if (!P.on) return 0;
+  if (P.pool.size() < 2) {
+    for (int q = 0; q < 8; q++) {
+      hipEvent_t e;
+      C3D_HIP(hipEventCreate(&e));
+      P.pool.push_back(e);
+    }
+  }
+  P.cur_a = P.pool.back();
+  P.pool.pop_back();
+  P.cur_stage = stage;
+  C3D_HIP(hipEventRecord(P.cur_a, L.st));
+  return 0;
+}
+
+int prof_end(Lib& L) {
+  Prof& P = L.prof;
+  if (!P.on || P.cur_stage < 0) return 0;
+  hipEvent_t b = P.pool.back();
+  P.pool.pop_back();
+  C3D_HIP(hipEventRecord(b, L.st));
+  P.pending.push_back({P.cur_stage, P.cur_a, b});
+  P.cur_stage = -1;
+  return 0;
+}
+
+// after a stream synchronisation
+int prof_collect(Lib& L) {
+  Prof& P = L.prof;
+  for (auto& q : P.pending) {
+    float ms = 0;
+    C3D_HIP(hipEventElapsedTime(&ms, q.a, q.b));
+    P.ms[q.stage] += ms;
+    P.calls[q.stage] += 1;
+    P.pool.push_back(q.a);
+    P.pool.push_back(q.b);
+  }
+  P.pending.clear();
+  return 0;
+}
+
+// ----------------------------------------------------------------------
+// pointers, fields and geometry
+
+bool is_device_ptr(const void* p) {
+  hipPointerAttribute_t a;
+  const hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+uint64_t fnv(const void* data, size_t bytes, uint64_t h) {
+  const unsigned char* c = static_cast<const unsigned char*>(data);
+  for (size_t q = 0; q < bytes; q++) {
+    h ^= c[q];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+using ConvFn = contour3d_vector_t (*)(const contour3d_vector_t);
+
+// Device coordinates of an object's lattice.  The converter maps each
+// lattice point (grids[0][i], grids[1][j], grids[2][k]) to Cartesian
+// (triangulate.c:165-170); it is evaluated once per point and the result
+// cached for as long as the converter, grid pointers and grid values stay
+// the same.  A map that returns every point unchanged needs no table.
+int get_geometry(Lib& L, const contour3d_contour_obj_t& o, Geom** out) {
+  const size_t nx = o.glsizes[0], ny = o.glsizes[1], nz = o.glsizes[2];
+  std::vector<double> hg[3];
+  const double* gp[3];
+  for (int d = 0; d < 3; d++) {
+    if (is_device_ptr(o.grids[d])) {
+      hg[d].resize(o.glsizes[d]);
+      C3D_HIP(hipMemcpy(hg[d].data(), o.grids[d], o.glsizes[d] * sizeof(double), hipMemcpyDeviceToHost));
+      gp[d] = hg[d].data();
+    } else {
+      gp[d] = o.grids[d];
+    }
+  }
+  uint64_t h = 1469598103934665603ull;
+  for (int d = 0; d < 3; d++) h = fnv(gp[d], o.glsizes[d] * sizeof(double), h);
+  for (auto& g : L.geoms) {
+    if (g->conv == (void*)o.converter && g->hash == h && g->n[0] == nx && g->n[1] == ny && g->n[2] == nz &&
+        g->g[0] == o.grids[0] && g->g[1] == o.grids[1] && g->g[2] == o.grids[2]) {
+      g->stamp = ++L.stamp;
+      *out = g.get();
+      return 0;
+    }
+  }
+  auto g = std::make_unique<Geom>();
+  g->conv = (void*)o.converter;
+  for (int d = 0; d < 3; d++) {
+    g->g[d] = o.grids[d];
+    g->n[d] = o.glsizes[d];
+  }
+  g->hash = h;
+  C3D_HIP(g->grids.ensure((nx + ny + nz) * sizeof(double)));
+  C3D_HIP(hipMemcpy(g->grids.as<double>(), gp[0], nx * sizeof(double), hipMemcpyHostToDevice));
+  C3D_HIP(hipMemcpy(g->grids.as<double>() + nx, gp[1], ny * sizeof(double), hipMemcpyHostToDevice));
+  C3D_HIP(hipMemcpy(g->grids.as<double>() + nx + ny, gp[2], nz * sizeof(double), hipMemcpyHostToDevice));
+  g->identity = true;
+  if (o.converter != contour3d_amd_identity) {
+    const ConvFn conv = o.converter;
+    const size_t npts = nx * ny * nz;
+    std::vector<double> tab;
+    size_t p = 0;
+    for (size_t k = 0; k < nz; k++) {
+      for (size_t j = 0; j < ny; j++) {
+        for (size_t i = 0; i < nx; i++, p++) {
+          const contour3d_vector_t in = {gp[0][i], gp[1][j], gp[2][k]};
+          const contour3d_vector_t r = conv(in);
+          if (g->identity && std::memcmp(&r, &in, sizeof(r)) != 0) {
+            g->identity = false;
+            tab.resize(3 * npts);
+            size_t q = 0;
+            for (size_t kk = 0; kk <= k; kk++)
+              for (size_t jj = 0; jj < ny; jj++)
+                for (size_t ii = 0; ii < nx && q < p; ii++, q++) {
+                  tab[q] = gp[0][ii];
+                  tab[npts + q] = gp[1][jj];
+                  tab[2 * npts + q] = gp[2][kk];
+                }
+          }
+          if (!g->identity) {
+            tab[p] = r.x;
+            tab[npts + p] = r.y;
+            tab[2 * npts + p] = r.z;
+          }
+        }
+      }
+    }
+    if (!g->identity) {
+      C3D_HIP(g->table.ensure(3 * npts * sizeof(double)));
+      C3D_HIP(hipMemcpy(g->table.p, tab.data(), 3 * npts * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
+  g->stamp = ++L.stamp;
+  // keep a handful of geometries; evict the least recently used
+  if (L.geoms.size() >= 16) {
+    size_t victim = 0;
+    for (size_t q = 1; q < L.geoms.size(); q++)
+      if (L.geoms[q]->stamp < L.geoms[victim]->stamp) victim = q;
+    L.geoms[victim]->grids.release();
+    L.geoms[victim]->table.release();
+    L.geoms.erase(L.geoms.begin() + victim);
+  }
+  *out = g.get();
+  L.geoms.push_back(std::move(g));
+  return 0;
+}
+
+int validate_obj(const contour3d_contour_obj_t& o, size_t n) {
+  for (int d = 0; d < 3; d++) {
+    if (o.glsizes[d] < 2) {
+      log_error("contour object %zu: glsizes[%d] = %zu, need at least 2 points per axis", n, d, o.glsizes[d]);
+      return 1;
+    }
+    if (!o.grids[d]) {
+      log_error("contour object %zu: grids[%d] is NULL", n, d);
+      return 1;
+    }
+  }
+  if (!o.array || !o.converter) {
+    log_error("contour object %zu: array and converter must not be NULL", n);
+    return 1;
+  }
+  const uint64_t nc = uint64_t(o.glsizes[0] - 1) * uint64_t(o.glsizes[1] - 1) * uint64_t(o.glsizes[2] - 1);
+  if (nc >= (uint64_t(1) << 32) - kCellsPerBlock || o.glsizes[0] > (1u << 30) || o.glsizes[1] > (1u << 30) ||
+      o.glsizes[2] > (1u << 30)) {
+    log_error("contour object %zu: %zu x %zu x %zu lattice exceeds 2^32 cells", n, o.glsizes[0], o.glsizes[1],
+              o.glsizes[2]);
+    return 1;
+  }
+  return 0;
+}
+
+template <typename R>
+MeshView<R> view_of(const contour3d_amd_mesh& M) {
+  MeshView<R> m;
+  m.pos = M.pos.as<R>();
+  m.wfn = M.wfn.as<R>();
+  m.vn = M.vn.as<R>();
+  m.cell = M.cell.as<uint32_t>();
+  m.edges = M.edges.as<uint32_t>();
+  m.cap = M.cap;
+  return m;
+}
+
+// ----------------------------------------------------------------------
+// mesh build: count -> scan -> emit -> smooth
+
+template <typename R>
+int build_mesh_t(Lib& L, contour3d_amd_mesh& M, const contour3d_contour_obj_t* objs, size_t n) {
+  std::vector<FieldDev> fd(n);
+  std::vector<int> group(n, -1);
+  std::vector<uint32_t>& boff = L.h_boff;
+  boff.assign(n + 1, 0);
+  std::vector<const double*> upl_src;
+  std::vector<void*> upl_dst;
+  // resolve fields and geometry
+  for (size_t o = 0; o < n; o++) {
+    const contour3d_contour_obj_t& ob = objs[o];
+    if (validate_obj(ob, o)) return 1;
+    FieldDev& f = fd[o];
+    f.nx = uint32_t(ob.glsizes[0]);
+    f.ny = uint32_t(ob.glsizes[1]);
+    f.nz = uint32_t(ob.glsizes[2]);
+    f.ncx = f.nx - 1;
+    f.ncy = f.ny - 1;
+    f.ncz = f.nz - 1;
+    f.ncells = f.ncx * f.ncy * f.ncz;
+    f.nblocks = (f.ncells + kCellsPerBlock - 1) / kCellsPerBlock;
+    f.div_ncx = make_fastdiv(f.ncx);
+    f.div_ncy = make_fastdiv(f.ncy);
+    if (is_device_ptr(ob.array)) {
+      f.f = ob.array;
+    } else {
+      const size_t bytes = size_t(f.nx) * f.ny * f.nz * sizeof(double);
+      void* dst = nullptr;
+      for (size_t q = 0; q < upl_src.size(); q++)
+        if (upl_src[q] == ob.array) dst = upl_dst[q];
+      if (!dst) {
+        if (L.uploads.size() <= upl_src.size()) L.uploads.push_back(std::make_unique<DevBuf>());
+        DevBuf& b = *L.uploads[upl_src.size()];
+        C3D_HIP(b.ensure(bytes));
+        C3D_HIP(hipMemcpyAsync(b.p, ob.array, bytes, hipMemcpyHostToDevice, L.st));
+        upl_src.push_back(ob.array);
+        upl_dst.push_back(b.p);
+        dst = b.p;
+      }
+      f.f = static_cast<const double*>(dst);
+    }
+    Geom* g = nullptr;
+    if (get_geometry(L, ob, &g)) return 1;
+    f.gx = g->grids.as<double>();
+    f.gy = f.gx + f.nx;
+    f.gz = f.gy + f.ny;
+    if (g->identity) {
+      f.cx = f.cy = f.cz = nullptr;
+    } else {
+      const size_t npts = size_t(f.nx) * f.ny * f.nz;
+      f.cx = g->table.as<double>();
+      f.cy = f.cx + npts;
+      f.cz = f.cy + npts;
+    }
+    boff[o + 1] = boff[o] + f.nblocks;
+    for (size_t q = 0; q < o && group[o] < 0; q++) {
+      if (fd[q].f == f.f && fd[q].nx == f.nx && fd[q].ny == f.ny && fd[q].nz == f.nz && fd[q].gx == f.gx &&
+          fd[q].cx == f.cx)
+        group[o] = group[q];
+    }
+    if (group[o] < 0) group[o] = int(o);
+  }
+  const uint64_t nbt = boff[n];
+  M.precision = sizeof(R) == 8 ? CONTOUR3D_AMD_FP64 : CONTOUR3D_AMD_FP32;
+  M.base.assign(n, 0);
+  M.count.assign(n, 0);
+  M.color.resize(n);
+  for (size_t o = 0; o < n; o++) M.color[o] = objs[o].color;
+  M.total = 0;
+  if (n == 0) return 0;
+
+  const uint64_t ntile = (nbt + 1023) / 1024;
+  C3D_HIP(L.cnt.ensure(nbt * sizeof(uint32_t)));
+  C3D_HIP(L.start.ensure((nbt + 1) * sizeof(uint64_t)));
+  C3D_HIP(L.tiles.ensure((ntile + 1) * sizeof(uint64_t)));
+  C3D_HIP(L.gidx.ensure((n + 1) * sizeof(uint32_t)));
+  C3D_HIP(L.gout.ensure((n + 1) * sizeof(uint64_t)));
+  C3D_HIP(L.hgout.ensure((n + 1) * sizeof(uint64_t)));
+
+  // thresholds of objects sharing a field are classified in one read
+  auto for_each_thrset = [&](auto&& fn) -> int {
+    for (size_t o = 0; o < n; o++) {
+      if (group[o] != int(o)) continue;
+      ThrSet ts{};
+      ts.n = 0;
+      for (size_t q = o; q < n; q++) {
+        if (group[q] != int(o)) continue;
+        ts.thr[ts.n] = objs[q].threshold;
+        ts.boff[ts.n] = boff[q];
+        ts.n++;
+        if (ts.n == kMaxThr) {
+          if (fn(fd[o], ts)) return 1;
+          ts.n = 0;
+        }
+      }
+      if (ts.n && fn(fd[o], ts)) return 1;
+    }
+    return 0;
+  };
+
+  if (prof_begin(L, CONTOUR3D_AMD_STAGE_COUNT)) return 1;
+  if (for_each_thrset([&](const FieldDev& f, const ThrSet& ts) -> int {
+        C3D_HIP(launch_count(f, ts, L.cnt.as<uint32_t>(), L.st));
+        return 0;
+      }))
+    return 1;
+  if (prof_end(L)) return 1;
+
+  if (prof_begin(L, CONTOUR3D_AMD_STAGE_SCAN)) return 1;
+  C3D_HIP(launch_scan(L.cnt.as<uint32_t>(), L.start.as<uint64_t>(), nbt, L.tiles.as<uint64_t>(), L.st));
+  C3D_HIP(hipMemcpyAsync(L.gidx.p, boff.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, L.st));
+  C3D_HIP(launch_gather_u64(L.start.as<uint64_t>(), L.gidx.as<uint32_t>(), L.gout.as<uint64_t>(), int(n + 1),
+                            L.st));
+  C3D_HIP(hipMemcpyAsync(L.hgout.p, L.gout.p, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, L.st));
+  if (prof_end(L)) return 1;
+  C3D_HIP(hipStreamSynchronize(L.st));
+  const uint64_t* hb = static_cast<const uint64_t*>(L.hgout.p);
+  for (size_t o = 0; o < n; o++) {
+    M.base[o] = hb[o];
+    M.count[o] = hb[o + 1] - hb[o];
+  }
+  M.total = hb[n];
+  L.prof.ntri += M.total;
+  if (M.total >= (uint64_t(1) << 32) - (uint64_t(1) << 24)) {
+    log_error("%llu triangles exceed the 2^32 triangle ids of one frame", (unsigned long long)M.total);
+    return 1;
+  }
+  if (M.cap < M.total || M.cap == 0) {
+    const uint64_t cap = M.total + M.total / 4 + 1024;
+    M.pos.release();
+    M.wfn.release();
+    M.vn.release();
+    M.cell.release();
+    M.edges.release();
+    M.cap = 0;
+    C3D_HIP(M.pos.ensure(9 * cap * sizeof(R)));
+    C3D_HIP(M.wfn.ensure(3 * cap * sizeof(R)));
+    C3D_HIP(M.vn.ensure(9 * cap * sizeof(R)));
+    C3D_HIP(M.cell.ensure(cap * sizeof(uint32_t)));
+    C3D_HIP(M.edges.ensure(cap * sizeof(uint32_t)));
+    M.cap = cap;
+  }
+  const MeshView<R> mv = view_of<R>(M);
+
+  if (prof_begin(L, CONTOUR3D_AMD_STAGE_EMIT)) return 1;
+  if (for_each_thrset([&](const FieldDev& f, const ThrSet& ts) -> int {
+        C3D_HIP(launch_emit<R>(f, ts, L.start.as<uint64_t>(), mv, L.st));
+        return 0;
+      }))
+    return 1;
+  if (prof_end(L)) return 1;
+
+  if (prof_begin(L, CONTOUR3D_AMD_STAGE_SMOOTH)) return 1;
+  for (size_t o = 0; o < n; o++) {
+    SmoothObj so;
+    so.base = M.base[o];
+    so.count = M.count[o];
+    so.bstart = L.start.as<uint64_t>() + boff[o];
+    so.ncx = fd[o].ncx;
+    so.ncy = fd[o].ncy;
+    so.ncz = fd[o].ncz;
+    so.div_ncx = fd[o].div_ncx;
+    so.div_ncy = fd[o].div_ncy;
+    C3D_HIP(launch_smooth<R>(so, L.nbr, mv, L.st));
+  }
+  if (prof_end(L)) return 1;
+  return 0;
+}
+
+int build_mesh(Lib& L, contour3d_amd_mesh& M, const contour3d_contour_obj_t* objs, size_t n, int precision) {
+  if (n && !objs) {
+    log_error("contour objects pointer is NULL");
+    return 1;
+  }
+  if (precision == CONTOUR3D_AMD_FP64) return build_mesh_t<double>(L, M, objs, n);
+  if (precision == CONTOUR3D_AMD_FP32) return build_mesh_t<float>(L, M, objs, n);
+  log_error("unknown precision %d", precision);
+  return 1;
+}
+
+// ----------------------------------------------------------------------
+// view and lines (host FP64, exactly as the reference forms them)
+
+using V = contour3d_vector_t;
+V hsub(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+V hadd(V a, V b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+V hscale(double w, V a) { return {w * a.x, w * a.y, w * a.z}; }
+double hdot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+V hcross(V a, V b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+V hunit(V a) {
+  const double s = 1. / std::sqrt(hdot(a, a));
+  return {a.x * s, a.y * s, a.z * s};
+}
+
+struct HostView {
+  V cam, ctr, lx, ly, nrm, light;
+  double nc, d, lxx, lyy;
+  size_t W, H;
+  contour3d_color_t bg;
+};
+
+// src/contour3d/main.c:25-63 and the per-call constants of project.c:34-36
+HostView make_view(const V* cam, const V* light, const size_t sizes[2], const V* ctr, const V local[2],
+                   const contour3d_color_t* bg) {
+  HostView v;
+  v.cam = *cam;
+  v.ctr = *ctr;
+  v.lx = local[0];
+  v.ly = local[1];
+  v.light = hunit(*light);
+  v.nrm = hunit(hcross(local[0], local[1]));
+  v.nc = hdot(v.nrm, v.cam);
+  v.d = hdot(v.nrm, v.ctr);
+  v.lxx = hdot(v.lx, v.lx);
+  v.lyy = hdot(v.ly, v.ly);
+  v.W = sizes[0];
+  v.H = sizes[1];
+  v.bg = *bg;
+  return v;
+}
+
+template <typename R>
+ViewDev<R> dev_view(const HostView& h) {
+  ViewDev<R> v;
+  const V* src[6] = {&h.cam, &h.ctr, &h.lx, &h.ly, &h.nrm, &h.light};
+  R* dst[6] = {v.cam, v.ctr, v.lx, v.ly, v.nrm, v.light};
+  for (int q = 0; q < 6; q++) {
+    dst[q][0] = R(src[q]->x);
+    dst[q][1] = R(src[q]->y);
+    dst[q][2] = R(src[q]->z);
+  }
+  v.nc = R(h.nc);
+  v.d = R(h.d);
+  v.lxx = R(h.lxx);
+  v.lyy = R(h.lyy);
+  v.fw = R(double(h.W));
+  v.fh = R(double(h.H));
+  v.W = int32_t(h.W);
+  v.H = int32_t(h.H);
+  return v;
+}
+
+// project.c:9-67 on the host
+bool hproject(const HostView& v, V p, V& q) {
+  const V ray = hsub(p, v.cam);
+  const double nr = hdot(v.nrm, ray);
+  if (-1.e-8 < nr && nr < 1.e-8) return false;
+  const double t = (v.d - v.nc) / nr;
+  if (t <= 0. || 1. <= t) return false;
+  const V delta = {v.cam.x + t * ray.x - v.ctr.x, v.cam.y + t * ray.y - v.ctr.y, v.cam.z + t * ray.z - v.ctr.z};
+  q.x = hdot(delta, v.lx) / v.lxx;
+  q.y = hdot(delta, v.ly) / v.lyy;
+  q.z = nr;
+  return true;
+}
+
+// Line objects (line.c:26-170) become a list of (pixel, depth) fragments in
+// the reference's drawing order; the z-test against the contours happens on
+// the device.  Coverage uses libm pow as the reference does, whose rounding
+// differs from a plain square in about 1 of 1200 arguments.
+int line_fragments(const HostView& v, size_t nlines, const contour3d_line_obj_t* lines, Frags& F) {
+  const long W = long(v.W), H = long(v.H);
+  for (size_t ln = 0; ln < nlines; ln++) {
+    const contour3d_line_obj_t& L = lines[ln];
+    if (L.nitems < 2) {
+      log_error("line object %zu: give more than two points to draw a line, received %zu", ln, L.nitems);
+      return 1;
+    }
+    if (!L.converter) {
+      log_error("line object %zu: converter is NULL", ln);
+      return 1;
+    }
+    const V head = L.edges[0], tail = L.edges[1];
+    const V delta = hscale(1. / double(L.nitems - 1), hsub(tail, head));
+    for (size_t s = 0; s + 1 < L.nitems; s++) {
+      const V a0 = L.converter(hadd(head, hscale(double(s + 0), delta)));
+      const V b0 = L.converter(hadd(head, hscale(double(s + 1), delta)));
+      V a, b;
+      if (!hproject(v, a0, a) || !hproject(v, b0, b)) continue;
+      a.x = (0.5 + a.x) * double(v.W);
+      a.y = (0.5 + a.y) * double(v.H);
+      b.x = (0.5 + b.x) * double(v.W);
+      b.y = (0.5 + b.y) * double(v.H);
+      const long xmin = long(std::fmin(a.x, b.x) - 1), xmax = long(std::fmax(a.x, b.x) + 1);
+      const long ymin = long(std::fmin(a.y, b.y) - 1), ymax = long(std::fmax(a.y, b.y) + 1);
+      auto clampi = [](long lo, long x, long hi) { return x < lo ? lo : (x > hi ? hi : x); };
+      const long i0 = clampi(0, xmin, W - 1), i1 = clampi(0, xmax, W - 1);
+      const long j0 = clampi(0, ymin, H - 1), j1 = clampi(0, ymax, H - 1);
+      const double la = b.y - a.y;
+      const double lb = -b.x + a.x;
+      const double lc = (b.x - a.x) * a.y - (b.y - a.y) * a.x;
+      for (long j = j0; j <= j1; j++) {
+        for (long i = i0; i <= i1; i++) {
+          const double di = double(i), dj = double(j);
+          const double lim = std::pow(0.5 * L.width, 2.);
+          const double dist = std::pow(la * di + lb * dj + lc, 2.) / (la * la + lb * lb);
+          if (lim < dist) continue;
+          const double prm = -1. * ((b.x - a.x) * (a.x - di) + (b.y - a.y) * (a.y - dj)) /
+                             (std::pow(b.x - a.x, 2.) + std::pow(b.y - a.y, 2.));
+          const double dep = 1. / (prm / b.z + (1. - prm) / a.z);
+          F.pix.push_back(uint32_t(j * W + i));
+          F.depth.push_back(dep);
+          F.rgb.push_back(L.color.r);
+          F.rgb.push_back(L.color.g);
+          F.rgb.push_back(L.color.b);
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+// ----------------------------------------------------------------------
+// render: z-buffer passes, lines, resolve
+
+template <typename R>
+int render_t(Lib& L, const contour3d_amd_mesh& M, const HostView& hv, const Frags& F, uint8_t* rgb_dev,
+             double* depth_dev) {
+  const uint64_t npx = uint64_t(hv.W) * uint64_t(hv.H);
+  const bool fp64 = sizeof(R) == 8;
+  C3D_HIP(L.key.ensure(npx * sizeof(uint64_t)));
+  if (fp64) C3D_HIP(L.id.ensure(npx * sizeof(uint32_t)));
+  uint32_t* id = fp64 ? L.id.as<uint32_t>() : nullptr;
+  const uint32_t nf = uint32_t(F.pix.size());
+  if (nf) {
+    C3D_HIP(L.fpix.ensure(nf * sizeof(uint32_t)));
+    C3D_HIP(L.fdep.ensure(nf * sizeof(double)));
+    C3D_HIP(L.frgb.ensure(3 * size_t(nf)));
+    C3D_HIP(hipMemcpyAsync(L.fpix.p, F.pix.data(), nf * sizeof(uint32_t), hipMemcpyHostToDevice, L.st));
+    C3D_HIP(hipMemcpyAsync(L.fdep.p, F.depth.data(), nf * sizeof(double), hipMemcpyHostToDevice, L.st));
+    C3D_HIP(hipMemcpyAsync(L.frgb.p, F.rgb.data(), 3 * size_t(nf), hipMemcpyHostToDevice, L.st));
+  }
+  const size_t nobj = M.base.size();
+  std::vector<uint64_t>& ob = L.h_obase;
+  std::vector<uint8_t>& orgb = L.h_orgb;
+  ob.assign(nobj ? nobj : 1, 0);
+  orgb.assign(3 * (nobj ? nobj : 1), 0);
+  for (size_t o = 0; o < nobj; o++) {
+    ob[o] = M.base[o];
+    orgb[3 * o + 0] = M.color[o].r;
+    orgb[3 * o + 1] = M.color[o].g;
+    orgb[3 * o + 2] = M.color[o].b;
+  }
+  C3D_HIP(L.obase.ensure(ob.size() * sizeof(uint64_t)));
+  C3D_HIP(L.orgb.ensure(orgb.size()));
+  C3D_HIP(hipMemcpyAsync(L.obase.p, ob.data(), ob.size() * sizeof(uint64_t), hipMemcpyHostToDevice, L.st));
+  C3D_HIP(hipMemcpyAsync(L.orgb.p, orgb.data(), orgb.size(), hipMemcpyHostToDevice, L.st));
+
+  const ViewDev<R> v = dev_view<R>(hv);
+  const MeshView<R> mv = view_of<R>(M);
+  const uint32_t id_base = uint32_t(M.total + 1);
+  if (prof_begin(L, CONTOUR3D_AMD_STAGE_RASTER)) return 1;
+  C3D_HIP(launch_zinit(L.key.as<uint64_t>(), id, npx, fp64 ? 0 : 1, L.st));
+  for (int pass = 0; pass < (fp64 ? 2 : 1); pass++) {
+    C3D_HIP(launch_raster<R>(mv, M.total, v, pass, L.key.as<uint64_t>(), id, L.st));
+    C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, id_base, pass, fp64 ? 0 : 1,
+                              L.key.as<uint64_t>(), id, L.st));
+  }
+  if (prof_end(L)) return 1;
+  if (prof_begin(L, CONTOUR3D_AMD_STAGE_RESOLVE)) return 1;
+  ObjColors oc{L.obase.as<uint64_t>(), L.orgb.as<uint8_t>(), int(nobj ? nobj : 1)};
+  const uint8_t bg[3] = {hv.bg.r, hv.bg.g, hv.bg.b};
+  C3D_HIP(launch_resolve<R>(mv, M.total, v, oc, L.frgb.as<uint8_t>(), bg, L.key.as<uint64_t>(), id, rgb_dev,
+                            depth_dev, L.st));
+  if (prof_end(L)) return 1;
+  return 0;
+}
+
+int render(Lib& L, const contour3d_amd_mesh& M, const HostView& hv, size_t nlines,
+           const contour3d_line_obj_t* lines, uint8_t* rgb, double* depth, bool rgb_host_only) {
+  if (hv.W == 0 || hv.H == 0 || hv.W > (1u << 16) || hv.H > (1u << 16) ||
+      uint64_t(hv.W) * uint64_t(hv.H) >= (uint64_t(1) << 31)) {
+    log_error("screen size %zu x %zu out of range", hv.W, hv.H);
+    return 1;
+  }
+  if (nlines && !lines) {
+    log_error("line objects pointer is NULL");
+    return 1;
+  }
+  Frags& F = L.h_frags;
+  F.pix.clear();
+  F.depth.clear();
+  F.rgb.clear();
+  if (line_fragments(hv, nlines, lines, F)) return 1;
+  const uint64_t npx = uint64_t(hv.W) * uint64_t(hv.H);
+  const bool rgb_dev = !rgb_host_only && is_device_ptr(rgb);
+  const bool dep_dev = depth && is_device_ptr(depth);
+  uint8_t* rgb_target = rgb;
+  if (!rgb_dev) {
+    C3D_HIP(L.rgb.ensure(npx * 3));
+    rgb_target = L.rgb.as<uint8_t>();
+  }
+  double* dep_target = depth;
+  if (depth && !dep_dev) {
+    C3D_HIP(L.depth.ensure(npx * sizeof(double)));
+    dep_target = L.depth.as<double>();
+  }
+  int rc = M.precision == CONTOUR3D_AMD_FP64 ? render_t<double>(L, M, hv, F, rgb_target, dep_target)
+                                             : render_t<float>(L, M, hv, F, rgb_target, dep_target);
+  if (rc) return rc;
+  if (!rgb_dev) C3D_HIP(hipMemcpyAsync(rgb, rgb_target, npx * 3, hipMemcpyDeviceToHost, L.st));
+  if (depth && !dep_dev)
+    C3D_HIP(hipMemcpyAsync(depth, dep_target, npx * sizeof(double), hipMemcpyDeviceToHost, L.st));
+  C3D_HIP(hipStreamSynchronize(L.st));
+  return prof_collect(L);
+}
+
+// output.c:125-140
+int write_ppm(const char* fname, size_t W, size_t H, const uint8_t* rgb) {
+  errno = 0;
+  FILE* fp = std::fopen(fname, "w");
+  if (!fp) {
+    log_error("%s: %s", fname, std::strerror(errno));
+    return 1;
+  }
+  errno = 0;
+  std::fprintf(fp, "P6\n%zu %zu\n255\n", W, H);
+  if (errno != 0) {
+    log_error("%s: %s", fname, std::strerror(errno));
+    std::fclose(fp);
+    return 1;
+  }
+  const size_t n = std::fwrite(rgb, 3, W * H, fp);
+  if (n != W * H) {
+    log_error("fwrite failed (%zu expected, %zu returned)", W * H, n);
+    std::fclose(fp);
+    return 1;
+  }
+  std::fclose(fp);
+  return 0;
+}
+
+bool args_ok(const void* cam, const void* light, const size_t* sizes, const void* ctr, const void* local,
+             const void* bg) {
+  if (!cam || !light || !sizes || !ctr || !local || !bg) {
+    log_error("camera, light, screen and background arguments must not be NULL");
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+}  // namespace c3d
+
+using c3d::lib;
+
+extern "C" {
+
+contour3d_vector_t contour3d_amd_identity(const contour3d_vector_t orthogonal) { return orthogonal; }
+
+int contour3d_amd_init(int device) {
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  return c3d::ensure_init(L, device);
+}
+
+int contour3d_amd_finalize(void) {
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  if (!L.ready) return 0;
+  (void)hipStreamSynchronize(L.st);
+  for (auto& gm : L.geoms) {
+    gm->grids.release();
+    gm->table.release();
+  }
+  L.geoms.clear();
+  for (auto& u : L.uploads) u->release();
+  L.uploads.clear();
+  c3d::DevBuf* bufs[] = {&L.cnt, &L.start, &L.tiles, &L.gidx, &L.gout, &L.key, &L.id, &L.rgb,
+                         &L.depth, &L.fpix, &L.fdep, &L.frgb, &L.obase, &L.orgb};
+  for (auto* b : bufs) b->release();
+  L.frame_mesh.release();
+  L.hgout.release();
+  L.hrgb.release();
+  for (auto e : L.prof.pool) (void)hipEventDestroy(e);
+  L.prof.pool.clear();
+  L.prof.pending.clear();
+  (void)hipStreamDestroy(L.st);
+  L.st = nullptr;
+  L.ready = false;
+  return 0;
+}
+
+int contour3d_amd_set_precision(int precision) {
+  if (precision != CONTOUR3D_AMD_FP64 && precision != CONTOUR3D_AMD_FP32) {
+    c3d::log_error("unknown precision %d", precision);
+    return 1;
+  }
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  L.precision = precision;
+  return 0;
+}
+
+int contour3d_amd_execute_rgb(const contour3d_vector_t* camera_position, const contour3d_vector_t* light_direction,
+                              const size_t screen_sizes[2], const contour3d_vector_t* screen_center,
+                              const contour3d_vector_t screen_local[2], const contour3d_color_t* bg_color,
+                              size_t num_contours, const contour3d_contour_obj_t* contour_objs, size_t num_lines,
+                              const contour3d_line_obj_t* line_objs, uint8_t* rgb, double* depth, int precision) {
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  if (c3d::ensure_init(L, 0)) return 1;
+  if (!c3d::args_ok(camera_position, light_direction, screen_sizes, screen_center, screen_local, bg_color))
+    return 1;
+  if (!rgb) {
+    c3d::log_error("rgb output pointer is NULL");
+    return 1;
+  }
+  const auto hv = c3d::make_view(camera_position, light_direction, screen_sizes, screen_center, screen_local,
+                                 bg_color);
+  if (c3d::build_mesh(L, L.frame_mesh, contour_objs, num_contours, precision)) return 1;
+  return c3d::render(L, L.frame_mesh, hv, num_lines, line_objs, rgb, depth, false);
+}
+
+int contour3d_execute(const sdecomp_info_t* const sdecomp_info, const contour3d_vector_t* const camera_position,
+                      const contour3d_vector_t* const camera_look_at,
+                      const contour3d_vector_t* const light_direction, const size_t screen_sizes[2],
+                      const contour3d_vector_t* const screen_center, const contour3d_vector_t screen_local[2],
+                      const contour3d_color_t* const bg_color, const size_t num_contours,
+                      const contour3d_contour_obj_t* const contour3d_contour_objs, const size_t num_lines,
+                      const contour3d_line_obj_t* const contour3d_line_objs, const char fname[]) {
+  (void)sdecomp_info;    // one GPU holds the whole field
+  (void)camera_look_at;  // unused by the reference too (struct.h:53-58)
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  if (c3d::ensure_init(L, 0)) return 1;
+  if (!c3d::args_ok(camera_position, light_direction, screen_sizes, screen_center, screen_local, bg_color))
+    return 1;
+  if (!fname) {
+    c3d::log_error("output file name is NULL");
+    return 1;
+  }
+  const auto hv = c3d::make_view(camera_position, light_direction, screen_sizes, screen_center, screen_local,
+                                 bg_color);
+  if (hv.W == 0 || hv.H == 0) {
+    c3d::log_error("screen size %zu x %zu out of range", hv.W, hv.H);
+    return 1;
+  }
+  if (c3d::build_mesh(L, L.frame_mesh, contour3d_contour_objs, num_contours, L.precision)) {
+    c3d::log_error("contour processing failed");
+    return 1;
+  }
+  const size_t bytes = hv.W * hv.H * 3;
+  if (L.hrgb.ensure(bytes) != hipSuccess) {
+    c3d::log_error("failed to allocate image buffer");
+    return 1;
+  }
+  uint8_t* host = static_cast<uint8_t*>(L.hrgb.p);
+  if (c3d::render(L, L.frame_mesh, hv, num_lines, contour3d_line_objs, host, nullptr, true)) {
+    c3d::log_error("image output failed");
+    return 1;
+  }
+  return c3d::write_ppm(fname, hv.W, hv.H, host);
+}
+
+int contour3d_amd_mesh_build(const contour3d_contour_obj_t* contour_objs, size_t num_contours, int precision,
+                             contour3d_amd_mesh_t** mesh) {
+  if (!mesh) {
+    c3d::log_error("mesh output pointer is NULL");
+    return 1;
+  }
+  *mesh = nullptr;
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  if (c3d::ensure_init(L, 0)) return 1;
+  auto M = std::make_unique<contour3d_amd_mesh>();
+  if (c3d::build_mesh(L, *M, contour_objs, num_contours, precision)) return 1;
+  if (hipStreamSynchronize(L.st) != hipSuccess) {
+    c3d::log_error("mesh build failed on the device");
+    return 1;
+  }
+  if (c3d::prof_collect(L)) return 1;
+  *mesh = M.release();
+  return 0;
+}
+
+int contour3d_amd_mesh_count(const contour3d_amd_mesh_t* mesh, size_t obj, size_t* ntri) {
+  if (!mesh || !ntri || obj >= mesh->count.size()) {
+    c3d::log_error("mesh_count: bad arguments");
+    return 1;
+  }
+  *ntri = size_t(mesh->count[obj]);
+  return 0;
+}
+
+int contour3d_amd_mesh_copy(const contour3d_amd_mesh_t* mesh, size_t obj, double* pos, double* wfn, double* vn,
+                            uint32_t* cell, uint32_t* edges) {
+  if (!mesh || obj >= mesh->count.size()) {
+    c3d::log_error("mesh_copy: bad arguments");
+    return 1;
+  }
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  const uint64_t n = mesh->count[obj], b = mesh->base[obj], cap = mesh->cap;
+  if (n == 0) return 0;
+  const size_t rs = mesh->precision == CONTOUR3D_AMD_FP64 ? 8 : 4;
+  std::vector<unsigned char> tmp(n * rs);
+  auto fetch = [&](const c3d::DevBuf& buf, int ncomp, int comp, double* dst, int stride) -> int {
+    const unsigned char* src = static_cast<const unsigned char*>(buf.p) + (uint64_t(comp) * cap + b) * rs;
+    if (hipMemcpy(tmp.data(), src, n * rs, hipMemcpyDeviceToHost) != hipSuccess) {
+      c3d::log_error("mesh_copy: device copy failed");
+      return 1;
+    }
+    (void)ncomp;
+    for (uint64_t t = 0; t < n; t++) {
+      double x;
+      if (rs == 8) {
+        std::memcpy(&x, tmp.data() + t * 8, 8);
+      } else {
+        float y;
+        std::memcpy(&y, tmp.data() + t * 4, 4);
+        x = y;
+      }
+      dst[t * stride + comp] = x;
+    }
+    return 0;
+  };
+  for (int c = 0; pos && c < 9; c++)
+    if (fetch(mesh->pos, 9, c, pos, 9)) return 1;
+  for (int c = 0; wfn && c < 3; c++)
+    if (fetch(mesh->wfn, 3, c, wfn, 3)) return 1;
+  for (int c = 0; vn && c < 9; c++)
+    if (fetch(mesh->vn, 9, c, vn, 9)) return 1;
+  if (cell && hipMemcpy(cell, mesh->cell.as<uint32_t>() + b, n * 4, hipMemcpyDeviceToHost) != hipSuccess) return 1;
+  if (edges && hipMemcpy(edges, mesh->edges.as<uint32_t>() + b, n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return 1;
+  return 0;
+}
+
+int contour3d_amd_mesh_render(const contour3d_amd_mesh_t* mesh, const contour3d_vector_t* camera_position,
+                              const contour3d_vector_t* light_direction, const size_t screen_sizes[2],
+                              const contour3d_vector_t* screen_center, const contour3d_vector_t screen_local[2],
+                              const contour3d_color_t* bg_color, size_t num_lines,
+                              const contour3d_line_obj_t* line_objs, uint8_t* rgb, double* depth) {
+  if (!mesh || !rgb) {
+    c3d::log_error("mesh_render: mesh and rgb must not be NULL");
+    return 1;
+  }
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  if (c3d::ensure_init(L, 0)) return 1;
+  if (!c3d::args_ok(camera_position, light_direction, screen_sizes, screen_center, screen_local, bg_color))
+    return 1;
+  const auto hv = c3d::make_view(camera_position, light_direction, screen_sizes, screen_center, screen_local,
+                                 bg_color);
+  return c3d::render(L, *mesh, hv, num_lines, line_objs, rgb, depth, false);
+}
+
+int contour3d_amd_mesh_free(contour3d_amd_mesh_t* mesh) {
+  if (!mesh) return 0;
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  if (L.ready) (void)hipStreamSynchronize(L.st);
+  mesh->release();
+  delete mesh;
+  return 0;
+}
+
+int contour3d_amd_profile_enable(int enable) {
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  L.prof.on = enable != 0;
+  return 0;
+}
+
+int contour3d_amd_profile_read(double ms[CONTOUR3D_AMD_NSTAGES], uint64_t calls[CONTOUR3D_AMD_NSTAGES],
+                               uint64_t* ntri_total) {
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  for (int s = 0; s < CONTOUR3D_AMD_NSTAGES; s++) {
+    if (ms) ms[s] = L.prof.ms[s];
+    if (calls) calls[s] = L.prof.calls[s];
+  }
+  if (ntri_total) *ntri_total = L.prof.ntri;
+  return 0;
+}
+
+int contour3d_amd_profile_reset(void) {
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  for (int s = 0; s < CONTOUR3D_AMD_NSTAGES; s++) {
+    L.prof.ms[s] = 0;
+    L.prof.calls[s] = 0;
+  }
+  L.prof.ntri = 0;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,874 @@
+// Device kernels of the tessellate -> smooth -> render path for gfx950.
+//
+// FP64 form: every floating-point step is the reference's, in the same
+// order, built with -ffp-contract=off so no multiply-add is fused; the
+// results are bit-identical to the reference's FP64 arithmetic.
+// FP32 form (R = float): the corner classification and the 1e-8 edge test
+// stay in FP64 (topology is exact); interpolation, normals, projection,
+// depth and shading run in FP32.
+#include "c3d_internal.h"
+
+#include <cfloat>
+
+namespace c3d {
+
+// ----------------------------------------------------------------------
+// small helpers
+
+FastDiv make_fastdiv(uint32_t d) {
+  FastDiv r;
+  r.d = d;
+  uint32_t s = 0;
+  while ((uint64_t(1) << s) < d) s++;
+  r.s = s;
+  r.m = uint32_t(((uint64_t(1) << 32) * ((uint64_t(1) << s) - d)) / d + 1);
+  return r;
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
+  return uint32_t((uint64_t(__umulhi(n, f.m)) + n) >> f.s);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so neighbouring z-planes meet in its own L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t ntiles) {
+  const uint32_t per = (ntiles + 7) / 8;
+  return (blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+
+template <typename R>
+struct V3 {
+  R x, y, z;
+};
+
+template <typename R>
+__device__ __forceinline__ V3<R> vadd(V3<R> a, V3<R> b) {
+  return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+template <typename R>
+__device__ __forceinline__ V3<R> vsub(V3<R> a, V3<R> b) {
+  return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+template <typename R>
+__device__ __forceinline__ V3<R> vscale(R w, V3<R> a) {
+  return {w * a.x, w * a.y, w * a.z};
+}
+template <typename R>
+__device__ __forceinline__ R vdot(V3<R> a, V3<R> b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <typename R>
+__device__ __forceinline__ V3<R> vcross(V3<R> a, V3<R> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// reciprocal of the norm, then one multiply per component (vector.c:66-76)
+template <typename R>
+__device__ __forceinline__ V3<R> vunit(V3<R> a) {
+  const R s = R(1) / sqrt(vdot(a, a));
+  return {a.x * s, a.y * s, a.z * s};
+}
+
+// ----------------------------------------------------------------------
+// tessellation tables
+//
+// Cube corner c = i + 2j + 4k.  The six tetrahedra share corners 0 and 7;
+// their other two corners, in tetra-vertex order 2 and 3, are (3,1) (1,5)
+// (5,4) (4,6) (6,2) (2,3) (triangulate.c:145-152), packed 6 bits per tetra.
+constexpr uint64_t kTetOther = (3ull | 1ull << 3) | (1ull | 5ull << 3) << 6 |
+                               (5ull | 4ull << 3) << 12 | (4ull | 6ull << 3) << 18 |
+                               (6ull | 2ull << 3) << 24 | (2ull | 3ull << 3) << 30;
+
+__device__ __forceinline__ uint32_t tet_corner(int t, int v) {
+  if (v == 0) return 0;
+  if (v == 1) return 7;
+  return uint32_t(kTetOther >> (6 * t + 3 * (v - 2))) & 7u;
+}
+
+// 4-bit mask of tetra t from the cell's 8-bit corner code (bit n = tetra
+// vertex n above the threshold, triangulate.c:176-179)
+__device__ __forceinline__ uint32_t tet_mask(uint32_t code, int t) {
+  const uint32_t a = uint32_t(kTetOther >> (6 * t)) & 7u;
+  const uint32_t b = uint32_t(kTetOther >> (6 * t + 3)) & 7u;
+  return (code & 1u) | ((code >> 7) & 1u) << 1 | ((code >> a) & 1u) << 2 | ((code >> b) & 1u) << 3;
+}
+
+// 0 or 4 vertices above: none; 1 or 3: one triangle; 2: two (a quad)
+__device__ __forceinline__ uint32_t tet_ntri(uint32_t m) {
+  const uint32_t pc = __popc(m);
+  return pc == 2 ? 2u : (pc & 1u);
+}
+
+__device__ __forceinline__ uint32_t cell_ntri(uint32_t code) {
+  uint32_t n = 0;
+#pragma unroll
+  for (int t = 0; t < 6; t++) n += tet_ntri(tet_mask(code, t));
+  return n;
+}
+
+// (tetra, reduced mask 1..7, triangle 0/1) -> three (tail, head) cube-corner
+// pairs, 6 bits per vertex; filled from the case table of
+// triangulate.c:204-270 by upload_tables().
+__constant__ uint32_t c_tri[6 * 8 * 2];
+
+// cell-local id of the lattice edge between cube corners a and b
+// (numbering of triangulate.c:23-32)
+__device__ __forceinline__ uint32_t edge_id(uint32_t a, uint32_t b) {
+  const uint32_t lo = a < b ? a : b;
+  switch (a ^ b) {
+    case 1: return lo >> 1;
+    case 2: return 4 + (lo & 1) + 2 * (lo >> 2);
+    case 4: return 8 + (lo & 3);
+    case 6: return 12 + (lo & 1);
+    case 5: return 14 + ((lo >> 1) & 1);
+    case 3: return 16 + (lo >> 2);
+    default: return 18;
+  }
+}
+
+// ----------------------------------------------------------------------
+// block-wide exclusive scan of one value per thread (256 threads)
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave, uint32_t& total) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) s_wave[wave] = x;
+  __syncthreads();
+  uint32_t off = 0;
+  uint32_t tot = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; w++) {
+    const uint32_t s = s_wave[w];
+    off += w < wave ? s : 0u;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return off + x - v;
+}
+
+// ----------------------------------------------------------------------
+// field access
+
+struct CellLoc {
+  uint32_t i, j, k;
+  uint64_t p0;  // lattice index of corner 0
+};
+
+__device__ __forceinline__ CellLoc locate(const FieldDev& f, uint32_t c) {
+  CellLoc L;
+  const uint32_t jk = fdiv(c, f.div_ncx);
+  L.i = c - jk * f.ncx;
+  L.k = fdiv(jk, f.div_ncy);
+  L.j = jk - L.k * f.ncy;
+  L.p0 = (uint64_t(L.k) * f.ny + L.j) * f.nx + L.i;
+  return L;
+}
+
+__device__ __forceinline__ uint64_t corner_offset(const FieldDev& f, uint32_t c) {
+  return uint64_t(c & 1u) + uint64_t((c >> 1) & 1u) * f.nx + uint64_t(c >> 2) * f.nx * f.ny;
+}
+
+template <typename R>
+__device__ __forceinline__ V3<R> corner_pos(const FieldDev& f, const CellLoc& L, uint32_t c) {
+  if (f.cx) {
+    const uint64_t p = L.p0 + corner_offset(f, c);
+    return {R(f.cx[p]), R(f.cy[p]), R(f.cz[p])};
+  }
+  return {R(f.gx[L.i + (c & 1u)]), R(f.gy[L.j + ((c >> 1) & 1u)]), R(f.gz[L.k + (c >> 2)])};
+}
+
+// ----------------------------------------------------------------------
+// tessellation: count pass
+
+__global__ __launch_bounds__(kThreads) void k_count(FieldDev f, ThrSet ts, uint32_t* __restrict__ cnt) {
+  const uint32_t tile = xcd_tile(f.nblocks);
+  if (tile >= f.nblocks) return;
+  __shared__ uint32_t s_wave[kThreads / 64];
+  uint32_t acc[kMaxThr];
+#pragma unroll
+  for (int q = 0; q < kMaxThr; q++) acc[q] = 0;
+  for (int chunk = 0; chunk < kCellsPerBlock / kThreads; chunk++) {
+    const uint32_t c = tile * kCellsPerBlock + chunk * kThreads + threadIdx.x;
+    if (c < f.ncells) {
+      const CellLoc L = locate(f, c);
+      double v[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) v[q] = f.f[L.p0 + corner_offset(f, q)];
+#pragma unroll
+      for (int q = 0; q < kMaxThr; q++) {
+        if (q < ts.n) {
+          uint32_t code = 0;
+#pragma unroll
+          for (int n = 0; n < 8; n++) code |= uint32_t(ts.thr[q] < v[n]) << n;
+          acc[q] += cell_ntri(code);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kMaxThr; q++) {
+    if (q < ts.n) {
+      uint32_t tot;
+      block_excl_scan(acc[q], s_wave, tot);
+      if (threadIdx.x == 0) cnt[ts.boff[q] + tile] = tot;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------
+// scan of tile counts into triangle starts (u64, exclusive, start[n] = total)
+
+constexpr int kScanTile = 1024;
+
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(const uint32_t* __restrict__ cnt, uint64_t n,
+                                                          uint64_t* __restrict__ tile_sum) {
+  __shared__ uint64_t s[kThreads];
+  const uint64_t base = uint64_t(blockIdx.x) * kScanTile;
+  uint64_t acc = 0;
+  for (int q = 0; q < kScanTile / kThreads; q++) {
+    const uint64_t i = base + q * kThreads + threadIdx.x;
+    if (i < n) acc += cnt[i];
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = s[0];
+}
+
+// single workgroup: exclusive scan of the tile sums in place
+__global__ __launch_bounds__(kThreads) void k_scan_tiles(uint64_t* __restrict__ tile_sum, uint64_t ntile) {
+  __shared__ uint64_t s[kThreads];
+  uint64_t carry = 0;
+  for (uint64_t b = 0; b < ntile; b += kThreads) {
+    const uint64_t i = b + threadIdx.x;
+    const uint64_t v = i < ntile ? tile_sum[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < kThreads; d <<= 1) {
+      const uint64_t y = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
+      __syncthreads();
+      s[threadIdx.x] += y;
+      __syncthreads();
+    }
+    if (i < ntile) tile_sum[i] = carry + s[threadIdx.x] - v;
+    const uint64_t tot = s[kThreads - 1];
+    __syncthreads();
+    carry += tot;
+  }
+  if (threadIdx.x == 0) tile_sum[ntile] = carry;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_apply(const uint32_t* __restrict__ cnt, uint64_t n,
+                                                         const uint64_t* __restrict__ tile_off,
+                                                         uint64_t* __restrict__ start) {
+  __shared__ uint32_t s_wave[kThreads / 64];
+  const uint64_t base = uint64_t(blockIdx.x) * kScanTile;
+  uint64_t carry = tile_off[blockIdx.x];
+  for (int q = 0; q < kScanTile / kThreads; q++) {
+    const uint64_t i = base + q * kThreads + threadIdx.x;
+    const uint32_t v = i < n ? cnt[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan(v, s_wave, tot);
+    if (i < n) start[i] = carry + ex;
+    carry += tot;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[n] = tile_off[gridDim.x];
+}
+
+__global__ void k_gather_u64(const uint64_t* __restrict__ src, const uint32_t* __restrict__ idx,
+                             uint64_t* __restrict__ dst, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+// ----------------------------------------------------------------------
+// tessellation: emit pass
+
+// linear interpolation along a tetra edge (triangulate.c:40-73); the 1e-8
+// test is on the FP64 corner values in both forms
+template <typename R>
+__device__ __forceinline__ V3<R> interp(double thr, V3<R> p0, double v0, V3<R> p1, double v1) {
+  if (fabs(v1 - v0) < 1.e-8) return p0;
+  R fct;
+  if constexpr (sizeof(R) == 8) {
+    fct = (thr - v0) / (v1 - v0);
+  } else {
+    fct = R(thr - v0) / R(v1 - v0);
+  }
+  return vadd(p0, vscale(fct, vsub(p1, p0)));
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_emit(FieldDev f, ThrSet ts, const uint64_t* __restrict__ start,
+                                                   MeshView<R> m) {
+  const uint32_t tile = xcd_tile(f.nblocks);
+  if (tile >= f.nblocks) return;
+  __shared__ double s_val[8][kThreads];
+  __shared__ uint8_t s_code[kMaxThr][kThreads];
+  __shared__ uint16_t s_q[kThreads * 12];
+  __shared__ uint32_t s_wave[kThreads / 64];
+  uint64_t base[kMaxThr];
+#pragma unroll
+  for (int q = 0; q < kMaxThr; q++) base[q] = q < ts.n ? start[ts.boff[q] + tile] : 0;
+
+  for (int chunk = 0; chunk < kCellsPerBlock / kThreads; chunk++) {
+    const uint32_t c0 = tile * kCellsPerBlock + chunk * kThreads;
+    if (c0 >= f.ncells) break;
+    const uint32_t c = c0 + threadIdx.x;
+    const bool valid = c < f.ncells;
+    double v[8];
+    if (valid) {
+      const CellLoc L = locate(f, c);
+#pragma unroll
+      for (int q = 0; q < 8; q++) v[q] = f.f[L.p0 + corner_offset(f, q)];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; q++) v[q] = 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) s_val[q][threadIdx.x] = v[q];
+
+    for (int th = 0; th < ts.n; th++) {
+      const double thr = ts.thr[th];
+      uint32_t code = 0;
+#pragma unroll
+      for (int n = 0; n < 8; n++) code |= uint32_t(thr < v[n]) << n;
+      const uint32_t nt = valid ? cell_ntri(code) : 0u;
+      uint32_t total;
+      uint32_t o = block_excl_scan(nt, s_wave, total);
+      s_code[th][threadIdx.x] = uint8_t(code);
+      if (nt) {
+        for (int t = 0; t < 6; t++) {
+          const uint32_t k = tet_ntri(tet_mask(code, t));
+          for (uint32_t s = 0; s < k; s++) s_q[o++] = uint16_t(threadIdx.x | t << 8 | s << 11);
+        }
+      }
+      __syncthreads();
+      for (uint32_t it = threadIdx.x; it < total; it += kThreads) {
+        const uint32_t item = s_q[it];
+        const uint32_t l = item & 255u;
+        const int t = int(item >> 8) & 7;
+        const uint32_t s = item >> 11;
+        const uint32_t cc = c0 + l;
+        const CellLoc L = locate(f, cc);
+        const uint32_t cd = s_code[th][l];
+        const uint32_t mt = tet_mask(cd, t);
+        const bool rev = mt >= 8;
+        const uint32_t mm = rev ? 15u - mt : mt;
+        const uint32_t ent = c_tri[(t * 8 + mm) * 2 + s];
+        V3<R> P[3];
+        uint32_t E = 0;
+#pragma unroll
+        for (int vv = 0; vv < 3; vv++) {
+          const int q = rev ? 2 - vv : vv;   // triangulate.c:108-112
+          const uint32_t tail = (ent >> (6 * q)) & 7u;
+          const uint32_t head = (ent >> (6 * q + 3)) & 7u;
+          P[vv] = interp<R>(thr, corner_pos<R>(f, L, tail), s_val[tail][l], corner_pos<R>(f, L, head),
+                            s_val[head][l]);
+          E |= edge_id(tail, head) << (5 * vv);
+        }
+        // face normal and area (triangulate.c:75-93); the smoothing weight
+        // area * unit normal is formed as normal.c:39-40 forms it
+        const V3<R> n = vcross(vsub(P[1], P[0]), vsub(P[2], P[0]));
+        const R area = sqrt(vdot(n, n));
+        const V3<R> fn = vscale(R(1) / area, n);
+        const V3<R> w = vscale(area, fn);
+        const bool in = L.i >= 1 && L.i + 2 <= f.ncx && L.j >= 1 && L.j + 2 <= f.ncy && L.k >= 1 &&
+                        L.k + 2 <= f.ncz;
+        const uint64_t g = base[th] + it;
+        const uint64_t cap = m.cap;
+#pragma unroll
+        for (int vv = 0; vv < 3; vv++) {
+          m.pos[(3 * vv + 0) * cap + g] = P[vv].x;
+          m.pos[(3 * vv + 1) * cap + g] = P[vv].y;
+          m.pos[(3 * vv + 2) * cap + g] = P[vv].z;
+        }
+        m.wfn[0 * cap + g] = w.x;
+        m.wfn[1 * cap + g] = w.y;
+        m.wfn[2 * cap + g] = w.z;
+        m.cell[g] = cc;
+        m.edges[g] = E | uint32_t(in) << 15;
+      }
+      base[th] += total;
+      __syncthreads();
+    }
+  }
+}
+
+// ----------------------------------------------------------------------
+// vertex-normal smoothing: gather over the cells sharing each vertex's
+// lattice edge, in increasing cell order and slot order (normal.c:6-165)
+
+__device__ __forceinline__ uint64_t lower_bound_cell(const uint32_t* __restrict__ cell, uint64_t lo,
+                                                     uint64_t hi, uint32_t key) {
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (cell[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_smooth(SmoothObj o, NbrTable nt, MeshView<R> m) {
+  const uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (t >= o.count) return;
+  const uint64_t g = o.base + t;
+  const uint32_t ed = m.edges[g];
+  if (!((ed >> 15) & 1u)) return;
+  const uint32_t c = m.cell[g];
+  const uint64_t cap = m.cap;
+  const int64_t sy = o.ncx;
+  const int64_t sz = int64_t(o.ncx) * o.ncy;
+#pragma unroll 1
+  for (int vv = 0; vv < 3; vv++) {
+    const uint32_t e = (ed >> (5 * vv)) & 31u;
+    V3<R> sum = {R(0), R(0), R(0)};
+    for (int q = 0; q < nt.n[e]; q++) {
+      const int dk = nt.v[e][q][0], dj = nt.v[e][q][1], di = nt.v[e][q][2];
+      const uint32_t e2 = uint32_t(nt.v[e][q][3]);
+      const uint32_t c2 = uint32_t(int64_t(c) + dk * sz + dj * sy + di);
+      const uint32_t tl = c2 / kCellsPerBlock;
+      const uint64_t lo = o.base + o.bstart[tl] - o.bstart[0];
+      const uint64_t hi = o.base + o.bstart[tl + 1] - o.bstart[0];
+      for (uint64_t u = lower_bound_cell(m.cell, lo, hi, c2); u < hi && m.cell[u] == c2; u++) {
+        const uint32_t eu = m.edges[u];
+        if ((eu & 31u) == e2 || ((eu >> 5) & 31u) == e2 || ((eu >> 10) & 31u) == e2) {
+          const V3<R> w = {m.wfn[u], m.wfn[cap + u], m.wfn[2 * cap + u]};
+          sum = vadd(sum, w);
+        }
+      }
+    }
+    const V3<R> n = vunit(sum);
+    m.vn[(3 * vv + 0) * cap + g] = n.x;
+    m.vn[(3 * vv + 1) * cap + g] = n.y;
+    m.vn[(3 * vv + 2) * cap + g] = n.z;
+  }
+}
+
+// ----------------------------------------------------------------------
+// rasterisation
+
+// order-preserving integer images of the depths (larger depth = nearer)
+__device__ __forceinline__ uint64_t ord64(double x) {
+  if (x == 0.0) x = 0.0;
+  const uint64_t u = __double_as_longlong(x);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double unord64(uint64_t k) {
+  const uint64_t u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double(u);
+}
+__device__ __forceinline__ uint32_t ord32(float x) {
+  if (x == 0.0f) x = 0.0f;
+  const uint32_t u = __float_as_uint(x);
+  return (u >> 31) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unord32(uint32_t k) {
+  const uint32_t u = (k >> 31) ? (k & 0x7fffffffu) : ~k;
+  return __uint_as_float(u);
+}
+
+// perspective projection (project.c:9-67); false when rejected
+template <typename R>
+__device__ __forceinline__ bool project(const ViewDev<R>& v, V3<R> p, V3<R>& q) {
+  const V3<R> cam = {v.cam[0], v.cam[1], v.cam[2]};
+  const V3<R> nrm = {v.nrm[0], v.nrm[1], v.nrm[2]};
+  const V3<R> ray = vsub(p, cam);
+  const R nr = vdot(nrm, ray);
+  if (R(-1.e-8) < nr && nr < R(1.e-8)) return false;
+  const R t = (v.d - v.nc) / nr;
+  if (t <= R(0) || R(1) <= t) return false;
+  const V3<R> delta = {v.cam[0] + t * ray.x - v.ctr[0], v.cam[1] + t * ray.y - v.ctr[1],
+                       v.cam[2] + t * ray.z - v.ctr[2]};
+  const V3<R> lx = {v.lx[0], v.lx[1], v.lx[2]};
+  const V3<R> ly = {v.ly[0], v.ly[1], v.ly[2]};
+  q.x = vdot(delta, lx) / v.lxx;
+  q.y = vdot(delta, ly) / v.lyy;
+  q.z = nr;
+  return true;
+}
+
+// 2-D edge function (render.c:38-46)
+template <typename R>
+__device__ __forceinline__ R edge_fn(const V3<R>& a, const V3<R>& b, R cx, R cy) {
+  return (cx - a.x) * (b.y - a.y) - (cy - a.y) * (b.x - a.x);
+}
+
+// truncation toward zero of a bbox bound; the bound only matters after
+// clamping to [0, W-1], so it is first clamped into int range
+template <typename R>
+__device__ __forceinline__ int32_t trunc_i(R x) {
+  x = fmin(fmax(x, R(-1.e9)), R(1.e9));
+  return int32_t(x);
+}
+
+template <typename R>
+struct Projected {
+  V3<R> q[3];
+  R ainv;
+  int32_t i0, i1, j0, j1;
+};
+
+template <typename R>
+__device__ __forceinline__ bool setup_tri(const MeshView<R>& m, uint64_t t, const ViewDev<R>& v,
+                                          Projected<R>& P) {
+  const uint64_t cap = m.cap;
+#pragma unroll
+  for (int n = 0; n < 3; n++) {
+    const V3<R> p = {m.pos[(3 * n + 0) * cap + t], m.pos[(3 * n + 1) * cap + t],
+                     m.pos[(3 * n + 2) * cap + t]};
+    if (!project(v, p, P.q[n])) return false;
+  }
+  const V3<R>* q = P.q;
+  const int32_t xmin = trunc_i<R>((R(0.5) + fmin(q[0].x, fmin(q[1].x, q[2].x))) * v.fw - R(1));
+  const int32_t xmax = trunc_i<R>((R(0.5) + fmax(q[0].x, fmax(q[1].x, q[2].x))) * v.fw + R(1));
+  const int32_t ymin = trunc_i<R>((R(0.5) + fmin(q[0].y, fmin(q[1].y, q[2].y))) * v.fh - R(1));
+  const int32_t ymax = trunc_i<R>((R(0.5) + fmax(q[0].y, fmax(q[1].y, q[2].y))) * v.fh + R(1));
+  P.i0 = min(v.W - 1, max(0, xmin));
+  P.i1 = min(v.W - 1, max(0, xmax));
+  P.j0 = min(v.H - 1, max(0, ymin));
+  P.j1 = min(v.H - 1, max(0, ymax));
+  const R area = edge_fn(q[0], q[1], q[2].x, q[2].y);
+  P.ainv = R(1) / (area < R(0) ? fmin(area, R(-1.e-16)) : fmax(area, R(1.e-16)));
+  return true;
+}
+
+// barycentric weights at pixel (i, j); false when outside (render.c:91-115)
+template <typename R>
+__device__ __forceinline__ bool pixel_weights(const ViewDev<R>& v, const Projected<R>& P, int32_t i,
+                                              int32_t j, R& w0, R& w1, R& w2) {
+  const R cx = R(1) * (R(i) + R(0.5)) / v.fw - R(0.5);
+  const R cy = R(1) * (R(j) + R(0.5)) / v.fh - R(0.5);
+  w0 = P.ainv * edge_fn(P.q[1], P.q[2], cx, cy);
+  w1 = P.ainv * edge_fn(P.q[2], P.q[0], cx, cy);
+  w2 = P.ainv * edge_fn(P.q[0], P.q[1], cx, cy);
+  return !(w0 < R(-1.e-8) || w1 < R(-1.e-8) || w2 < R(-1.e-8));
+}
+
+// harmonic depth (render.c:120-124)
+template <typename R>
+__device__ __forceinline__ R pixel_depth(const Projected<R>& P, R w0, R w1, R w2) {
+  return R(1) / (w0 / P.q[0].z + w1 / P.q[1].z + w2 / P.q[2].z);
+}
+
+// FP64: pass 0 keeps the largest depth per pixel, pass 1 the latest serial
+// id among the fragments at that depth.  FP32: one pass over packed
+// (depth << 32 | id) keys.
+template <typename R, int PASS>
+__global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, uint64_t ntri, ViewDev<R> v,
+                                                     uint64_t* __restrict__ key, uint32_t* __restrict__ id) {
+  const uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (t >= ntri) return;
+  if (!((m.edges[t] >> 15) & 1u)) return;
+  Projected<R> P;
+  if (!setup_tri(m, t, v, P)) return;
+  for (int32_t j = P.j0; j <= P.j1; j++) {
+    for (int32_t i = P.i0; i <= P.i1; i++) {
+      R w0, w1, w2;
+      if (!pixel_weights(v, P, i, j, w0, w1, w2)) continue;
+      const R dep = pixel_depth(P, w0, w1, w2);
+      const uint64_t px = uint64_t(j) * uint64_t(v.W) + uint64_t(i);
+      if constexpr (sizeof(R) == 8) {
+        const uint64_t k = ord64(dep);
+        if (PASS == 0) {
+          atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+        } else if (k == __hip_atomic_load(key + px, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          atomicMax(id + px, uint32_t(t + 1));
+        }
+      } else {
+        const uint64_t k = uint64_t(ord32(dep)) << 32 | uint64_t(t + 1);
+        atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+      }
+    }
+  }
+}
+
+__global__ void k_zinit(uint64_t* __restrict__ key, uint32_t* __restrict__ id, uint64_t npx, uint64_t k0) {
+  const uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= npx) return;
+  key[p] = k0;
+  if (id) id[p] = 0;
+}
+
+__global__ void k_line_frags(const uint32_t* __restrict__ pix, const double* __restrict__ depth, uint32_t nfrag,
+                             uint32_t id_base, int pass, int fp32, uint64_t* __restrict__ key,
+                             uint32_t* __restrict__ id) {
+  const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nfrag) return;
+  const uint32_t px = pix[f];
+  if (fp32) {
+    const uint64_t k = uint64_t(ord32(float(depth[f]))) << 32 | uint64_t(id_base + f);
+    atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+    return;
+  }
+  const uint64_t k = ord64(depth[f]);
+  if (pass == 0) {
+    atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+  } else if (k == __hip_atomic_load(key + px, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+    atomicMax(id + px, id_base + f);
+  }
+}
+
+// final colour of every pixel from the winning fragment; rgb rows are
+// written top-down (y index H-1 first) as the PPM writer packs them
+// (output.c:113-123)
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_resolve(MeshView<R> m, uint64_t ntri, ViewDev<R> v, ObjColors oc,
+                                                      const uint8_t* __restrict__ frag_rgb, uchar4 bg,
+                                                      const uint64_t* __restrict__ key,
+                                                      const uint32_t* __restrict__ idbuf, uint8_t* __restrict__ rgb,
+                                                      double* __restrict__ depth) {
+  const uint64_t p = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const uint64_t npx = uint64_t(v.W) * uint64_t(v.H);
+  if (p >= npx) return;
+  const int32_t j = int32_t(p / uint64_t(v.W));
+  const int32_t i = int32_t(p - uint64_t(j) * uint64_t(v.W));
+  const uint64_t k = key[p];
+  uint32_t id;
+  double dep;
+  if constexpr (sizeof(R) == 8) {
+    id = idbuf[p];
+    dep = unord64(k);
+  } else {
+    id = uint32_t(k);
+    dep = id ? double(unord32(uint32_t(k >> 32))) : -DBL_MAX;
+  }
+  uint8_t r = bg.x, gg = bg.y, b = bg.z;
+  if (id != 0 && id <= ntri) {
+    const uint64_t t = id - 1;
+    Projected<R> P;
+    setup_tri(m, t, v, P);
+    R w0, w1, w2;
+    pixel_weights(v, P, i, j, w0, w1, w2);
+    const uint64_t cap = m.cap;
+    V3<R> nv[3];
+#pragma unroll
+    for (int n = 0; n < 3; n++) {
+      nv[n] = {m.vn[(3 * n + 0) * cap + t], m.vn[(3 * n + 1) * cap + t], m.vn[(3 * n + 2) * cap + t]};
+    }
+    const V3<R> n = vunit(vadd(vscale(w0, nv[0]), vadd(vscale(w1, nv[1]), vscale(w2, nv[2]))));
+    const V3<R> light = {v.light[0], v.light[1], v.light[2]};
+    const R fct = fmax(R(0.15), fabs(vdot(n, light)));
+    int lo = 0, hi = oc.n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (oc.base[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    r = uint8_t(fct * R(oc.rgb[3 * lo + 0]));
+    gg = uint8_t(fct * R(oc.rgb[3 * lo + 1]));
+    b = uint8_t(fct * R(oc.rgb[3 * lo + 2]));
+  } else if (id > ntri) {
+    const uint64_t f = id - ntri - 1;
+    r = frag_rgb[3 * f + 0];
+    gg = frag_rgb[3 * f + 1];
+    b = frag_rgb[3 * f + 2];
+  }
+  const uint64_t o = (uint64_t(v.H - 1 - j) * uint64_t(v.W) + uint64_t(i)) * 3;
+  rgb[o + 0] = r;
+  rgb[o + 1] = gg;
+  rgb[o + 2] = b;
+  if (depth) depth[p] = dep;
+}
+
+// ----------------------------------------------------------------------
+// host launchers
+
+hipError_t upload_tables() {
+  // case table of triangulate.c:204-270: reduced mask -> triangles as
+  // (tail, head) tetra-vertex pairs
+  static const int ntri[8] = {0, 1, 1, 2, 1, 2, 2, 1};
+  static const int pairs[8][2][3][2] = {
+      {{{0, 0}, {0, 0}, {0, 0}}, {{0, 0}, {0, 0}, {0, 0}}},
+      {{{0, 1}, {0, 3}, {0, 2}}, {{0, 0}, {0, 0}, {0, 0}}},
+      {{{1, 0}, {1, 2}, {1, 3}}, {{0, 0}, {0, 0}, {0, 0}}},
+      {{{0, 2}, {1, 3}, {0, 3}}, {{0, 2}, {1, 2}, {1, 3}}},
+      {{{2, 0}, {2, 3}, {2, 1}}, {{0, 0}, {0, 0}, {0, 0}}},
+      {{{2, 1}, {0, 3}, {2, 3}}, {{2, 1}, {0, 1}, {0, 3}}},
+      {{{0, 2}, {2, 3}, {1, 3}}, {{0, 2}, {1, 3}, {0, 1}}},
+      {{{3, 0}, {3, 2}, {3, 1}}, {{0, 0}, {0, 0}, {0, 0}}},
+  };
+  static const int tet[6][4] = {{0, 7, 3, 1}, {0, 7, 1, 5}, {0, 7, 5, 4},
+                                {0, 7, 4, 6}, {0, 7, 6, 2}, {0, 7, 2, 3}};
+  uint32_t h[6 * 8 * 2] = {};
+  for (int t = 0; t < 6; t++) {
+    for (int mm = 1; mm < 8; mm++) {
+      for (int s = 0; s < ntri[mm]; s++) {
+        uint32_t e = 0;
+        for (int q = 0; q < 3; q++) {
+          const uint32_t tail = uint32_t(tet[t][pairs[mm][s][q][0]]);
+          const uint32_t head = uint32_t(tet[t][pairs[mm][s][q][1]]);
+          e |= (tail | head << 3) << (6 * q);
+        }
+        h[(t * 8 + mm) * 2 + s] = e;
+      }
+    }
+  }
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_tri), h, sizeof(h));
+}
+
+NbrTable build_nbr_table() {
+  // every cell within one step holding both endpoints of edge e as a
+  // tetra edge, in increasing (dk, dj, di) = increasing linear order
+  static const int tet[6][4] = {{0, 7, 3, 1}, {0, 7, 1, 5}, {0, 7, 5, 4},
+                                {0, 7, 4, 6}, {0, 7, 6, 2}, {0, 7, 2, 3}};
+  auto is_tet_edge = [&](int a, int b) {
+    if (a == b) return false;
+    for (int t = 0; t < 6; t++) {
+      bool ha = false, hb = false;
+      for (int n = 0; n < 4; n++) {
+        ha |= tet[t][n] == a;
+        hb |= tet[t][n] == b;
+      }
+      if (ha && hb) return true;
+    }
+    return false;
+  };
+  auto eid = [](int a, int b) {
+    const int lo = a < b ? a : b;
+    switch (a ^ b) {
+      case 1: return lo >> 1;
+      case 2: return 4 + (lo & 1) + 2 * (lo >> 2);
+      case 4: return 8 + (lo & 3);
+      case 6: return 12 + (lo & 1);
+      case 5: return 14 + ((lo >> 1) & 1);
+      case 3: return 16 + (lo >> 2);
+      default: return 18;
+    }
+  };
+  NbrTable nt = {};
+  for (int a = 0; a < 8; a++) {
+    for (int b = a + 1; b < 8; b++) {
+      if (!is_tet_edge(a, b)) continue;
+      const int e = eid(a, b);
+      const int A[3] = {a & 1, (a >> 1) & 1, a >> 2};
+      const int B[3] = {b & 1, (b >> 1) & 1, b >> 2};
+      for (int dk = -1; dk <= 1; dk++)
+        for (int dj = -1; dj <= 1; dj++)
+          for (int di = -1; di <= 1; di++) {
+            const int d[3] = {di, dj, dk};
+            int la[3], lb[3];
+            bool ok = true;
+            for (int x = 0; x < 3; x++) {
+              la[x] = A[x] - d[x];
+              lb[x] = B[x] - d[x];
+              ok = ok && la[x] >= 0 && la[x] <= 1 && lb[x] >= 0 && lb[x] <= 1;
+            }
+            if (!ok) continue;
+            const int ca = la[0] + 2 * la[1] + 4 * la[2];
+            const int cb = lb[0] + 2 * lb[1] + 4 * lb[2];
+            if (!is_tet_edge(ca, cb)) continue;
+            const int q = nt.n[e]++;
+            nt.v[e][q][0] = int8_t(dk);
+            nt.v[e][q][1] = int8_t(dj);
+            nt.v[e][q][2] = int8_t(di);
+            nt.v[e][q][3] = int8_t(eid(ca, cb));
+          }
+    }
+  }
+  return nt;
+}
+
+static dim3 tile_grid(uint32_t nblocks) { return dim3(8 * ((nblocks + 7) / 8)); }
+
+hipError_t launch_count(const FieldDev& f, const ThrSet& ts, uint32_t* cnt, hipStream_t st) {
+  hipLaunchKernelGGL(k_count, tile_grid(f.nblocks), dim3(kThreads), 0, st, f, ts, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(const uint32_t* cnt, uint64_t* start, uint64_t n, uint64_t* tile_tmp, hipStream_t st) {
+  const uint64_t ntile = (n + kScanTile - 1) / kScanTile;
+  if (ntile == 0) return hipMemsetAsync(start, 0, sizeof(uint64_t), st);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(uint32_t(ntile)), dim3(kThreads), 0, st, cnt, n, tile_tmp);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kThreads), 0, st, tile_tmp, ntile);
+  hipLaunchKernelGGL(k_scan_apply, dim3(uint32_t(ntile)), dim3(kThreads), 0, st, cnt, n,
+                     (const uint64_t*)tile_tmp, start);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_u64(const uint64_t* src, const uint32_t* idx, uint64_t* dst, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_u64, dim3((n + 255) / 256), dim3(256), 0, st, src, idx, dst, n);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_emit(const FieldDev& f, const ThrSet& ts, const uint64_t* start, const MeshView<R>& m,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_emit<R>, tile_grid(f.nblocks), dim3(kThreads), 0, st, f, ts, start, m);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_smooth(const SmoothObj& o, const NbrTable& nt, const MeshView<R>& m, hipStream_t st) {
+  if (o.count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_smooth<R>, dim3(uint32_t((o.count + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, o,
+                     nt, m);
+  return hipGetLastError();
+}
+
+hipError_t launch_zinit(uint64_t* key, uint32_t* id, uint64_t npx, int precision, hipStream_t st) {
+  const uint64_t k0 = precision == 0 ? 0x0010000000000000ull /* ord64(-DBL_MAX) */ : 0ull;
+  hipLaunchKernelGGL(k_zinit, dim3(uint32_t((npx + 255) / 256)), dim3(256), 0, st, key, id, npx, k0);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_raster(const MeshView<R>& m, uint64_t ntri, const ViewDev<R>& v, int pass, uint64_t* key,
+                         uint32_t* id, hipStream_t st) {
+  if (ntri == 0) return hipSuccess;
+  const dim3 grid(uint32_t((ntri + kThreads - 1) / kThreads));
+  if (pass == 0)
+    hipLaunchKernelGGL((k_raster<R, 0>), grid, dim3(kThreads), 0, st, m, ntri, v, key, id);
+  else
+    hipLaunchKernelGGL((k_raster<R, 1>), grid, dim3(kThreads), 0, st, m, ntri, v, key, id);
+  return hipGetLastError();
+}
+
+hipError_t launch_line_frags(const uint32_t* pix, const double* depth, uint32_t nfrag, uint32_t id_base, int pass,
+                             int precision, uint64_t* key, uint32_t* id, hipStream_t st) {
+  if (nfrag == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_line_frags, dim3((nfrag + 255) / 256), dim3(256), 0, st, pix, depth, nfrag, id_base, pass,
+                     precision, key, id);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_resolve(const MeshView<R>& m, uint64_t ntri, const ViewDev<R>& v, const ObjColors& oc,
+                          const uint8_t* frag_rgb, const uint8_t bg[3], const uint64_t* key, const uint32_t* id,
+                          uint8_t* rgb, double* depth, hipStream_t st) {
+  const uint64_t npx = uint64_t(v.W) * uint64_t(v.H);
+  const uchar4 b = make_uchar4(bg[0], bg[1], bg[2], 0);
+  hipLaunchKernelGGL(k_resolve<R>, dim3(uint32_t((npx + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, m, ntri,
+                     v, oc, frag_rgb, b, key, id, rgb, depth);
+  return hipGetLastError();
+}
+
+template hipError_t launch_emit<double>(const FieldDev&, const ThrSet&, const uint64_t*, const MeshView<double>&,
+                                        hipStream_t);
+template hipError_t launch_emit<float>(const FieldDev&, const ThrSet&, const uint64_t*, const MeshView<float>&,
+                                       hipStream_t);
+template hipError_t launch_smooth<double>(const SmoothObj&, const NbrTable&, const MeshView<double>&, hipStream_t);
+template hipError_t launch_smooth<float>(const SmoothObj&, const NbrTable&, const MeshView<float>&, hipStream_t);
+template hipError_t launch_raster<double>(const MeshView<double>&, uint64_t, const ViewDev<double>&, int, uint64_t*,
+                                          uint32_t*, hipStream_t);
+template hipError_t launch_raster<float>(const MeshView<float>&, uint64_t, const ViewDev<float>&, int, uint64_t*,
+                                         uint32_t*, hipStream_t);
+template hipError_t launch_resolve<double>(const MeshView<double>&, uint64_t, const ViewDev<double>&,
+                                           const ObjColors&, const uint8_t*, const uint8_t*, const uint64_t*,
+                                           const uint32_t*, uint8_t*, double*, hipStream_t);
+template hipError_t launch_resolve<float>(const MeshView<float>&, uint64_t, const ViewDev<float>&, const ObjColors&,
+                                          const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*,
+                                          uint8_t*, double*, hipStream_t);
+
+}  // namespace c3d

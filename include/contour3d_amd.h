@@ -1,0 +1,152 @@
+/*
+ * contour3d_amd.h -- extensions of the MI355X Contour3D library.
+ *
+ * contour3d.h stays the drop-in interface.  This header adds what an
+ * embedding solver or a serving process needs beyond it: device-resident
+ * images, an FP32-geometry form, stage-level access to the triangle mesh,
+ * profiling, and explicit device/memory control.  None of these entry
+ * points exists in the reference; each says which reference behaviour it
+ * exposes.
+ *
+ * Conventions shared with contour3d_execute: 0 = success, 1 = failure with
+ * a "[CONTOUR3D ERROR] " line on stderr.  Pointers documented "host or
+ * device" are classified with hipPointerGetAttributes; device pointers are
+ * read or written in place.  Calls are serialised by a library mutex.
+ */
+#if !defined(CONTOUR3D_AMD_H)
+#define CONTOUR3D_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "contour3d.h"
+
+#if defined(__cplusplus)
+extern "C" {
+#endif
+
+/* Arithmetic forms. */
+#define CONTOUR3D_AMD_FP64 0  /* every step in FP64: bit-identical to the reference */
+#define CONTOUR3D_AMD_FP32 1  /* FP64 classification and edge tests, FP32 geometry,
+                                 depth and shading */
+
+/* Select the device and create the library's stream.  Optional: every
+ * entry point initialises device 0 on first use. */
+extern int contour3d_amd_init(int device);
+
+/* Free every device buffer, cached geometry and the stream.  The library
+ * keeps device buffers between calls (grow-only) so that repeated frames do
+ * not allocate; this returns them. */
+extern int contour3d_amd_finalize(void);
+
+/* Arithmetic form used by contour3d_execute (default CONTOUR3D_AMD_FP64). */
+extern int contour3d_amd_set_precision(int precision);
+
+/* The identity coordinate map.  Passing it as `converter` skips the
+ * one-time lattice check that every other converter gets (the reference
+ * calls the converter per tetra vertex, triangulate.c:170; this library
+ * evaluates it once per lattice point and caches the result per geometry). */
+extern contour3d_vector_t contour3d_amd_identity(const contour3d_vector_t orthogonal);
+
+/*
+ * contour3d_execute with the image returned instead of written to a file.
+ *   rgb    W*H*3 bytes in PPM order (top row first, output.c:113-123);
+ *          host or device pointer
+ *   depth  optional (may be NULL), W*H doubles, row j = screen y index j:
+ *          the final z-buffer, -DBL_MAX where nothing was drawn; host or device
+ *   precision  CONTOUR3D_AMD_FP64 or CONTOUR3D_AMD_FP32
+ */
+extern int contour3d_amd_execute_rgb(
+    const contour3d_vector_t * camera_position,
+    const contour3d_vector_t * light_direction,
+    const size_t screen_sizes[2],
+    const contour3d_vector_t * screen_center,
+    const contour3d_vector_t screen_local[2],
+    const contour3d_color_t * bg_color,
+    size_t num_contours,
+    const contour3d_contour_obj_t * contour_objs,
+    size_t num_lines,
+    const contour3d_line_obj_t * line_objs,
+    uint8_t * rgb,
+    double * depth,
+    int precision
+);
+
+/* ---- stage-level mesh access (tessellation + smoothing) --------------- */
+
+typedef struct contour3d_amd_mesh contour3d_amd_mesh_t;
+
+/* Tessellate and smooth every contour object (objects that share a field,
+ * grids and converter share one pass over the field).  Triangles of object
+ * n follow those of object n-1; within an object they are in the
+ * reference's serial order (cell index k, j, i, then the slot order of
+ * triangulate.c:325-338). */
+extern int contour3d_amd_mesh_build(
+    const contour3d_contour_obj_t * contour_objs,
+    size_t num_contours,
+    int precision,
+    contour3d_amd_mesh_t ** mesh
+);
+
+/* Number of triangles of object `obj` (all cells, rendered or not). */
+extern int contour3d_amd_mesh_count(const contour3d_amd_mesh_t * mesh, size_t obj, size_t * ntri);
+
+/*
+ * Copy object `obj`'s triangles to host arrays (any may be NULL), as FP64:
+ *   pos   ntri*9: vertex v, axis d at [9t + 3v + d]
+ *   wfn   ntri*3: area * unit face normal (triangulate.c:75-93, normal.c:39-40)
+ *   vn    ntri*9: unit vertex normals; only triangles of interior cells
+ *         (index 1..n-3 per axis, contour/main.c:92-93) have them, other
+ *         rows are unspecified
+ *   cell  ntri: linear cell index (k*ncy + j)*ncx + i
+ *   edges ntri: three 5-bit cell-local lattice edge ids (triangulate.c:23-32)
+ *         in bits 0-14, bit 15 set for triangles of interior cells
+ */
+extern int contour3d_amd_mesh_copy(const contour3d_amd_mesh_t * mesh, size_t obj, double * pos,
+                                   double * wfn, double * vn, uint32_t * cell, uint32_t * edges);
+
+/* Render a built mesh (all its objects, in order) and then the lines, as
+ * contour3d_amd_execute_rgb does. */
+extern int contour3d_amd_mesh_render(
+    const contour3d_amd_mesh_t * mesh,
+    const contour3d_vector_t * camera_position,
+    const contour3d_vector_t * light_direction,
+    const size_t screen_sizes[2],
+    const contour3d_vector_t * screen_center,
+    const contour3d_vector_t screen_local[2],
+    const contour3d_color_t * bg_color,
+    size_t num_lines,
+    const contour3d_line_obj_t * line_objs,
+    uint8_t * rgb,
+    double * depth
+);
+
+extern int contour3d_amd_mesh_free(contour3d_amd_mesh_t * mesh);
+
+/* ---- profiling --------------------------------------------------------- */
+
+/* Stages timed with HIP events on the library stream when profiling is on. */
+enum {
+  CONTOUR3D_AMD_STAGE_COUNT = 0,   /* classify + per-tile triangle counts */
+  CONTOUR3D_AMD_STAGE_SCAN = 1,    /* tile scan + totals */
+  CONTOUR3D_AMD_STAGE_EMIT = 2,    /* triangle emission */
+  CONTOUR3D_AMD_STAGE_SMOOTH = 3,  /* vertex normals */
+  CONTOUR3D_AMD_STAGE_RASTER = 4,  /* z-buffer passes incl. lines */
+  CONTOUR3D_AMD_STAGE_RESOLVE = 5, /* colour resolve + y-flip pack */
+  CONTOUR3D_AMD_NSTAGES = 6
+};
+
+extern int contour3d_amd_profile_enable(int enable);
+
+/* Accumulated milliseconds and launch counts per stage since the last
+ * reset; ntri_total receives the triangles emitted since the last reset. */
+extern int contour3d_amd_profile_read(double ms[CONTOUR3D_AMD_NSTAGES],
+                                      uint64_t calls[CONTOUR3D_AMD_NSTAGES],
+                                      uint64_t * ntri_total);
+
+extern int contour3d_amd_profile_reset(void);
+
+#if defined(__cplusplus)
+}
+#endif
+
+#endif /* CONTOUR3D_AMD_H */

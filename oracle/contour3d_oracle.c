@@ -803,3 +803,26 @@ void oracle_neighbour_table(int32_t *out, int32_t *n_out) {
     }
   }
 }
+
+/* ---------------------------------------------------------------------- */
+/* user-style coordinate maps for tests of non-identity converters          */
+
+/* cylindrical (r, theta, z) -> Cartesian */
+contour3d_vector_t oracle_conv_cylinder(const contour3d_vector_t p) {
+  const contour3d_vector_t r = { p.x * cos(p.y), p.x * sin(p.y), p.z };
+  return r;
+}
+
+/* a shear that moves every lattice point */
+contour3d_vector_t oracle_conv_shear(const contour3d_vector_t p) {
+  const contour3d_vector_t r = { p.x + 0.25 * p.y, p.y, p.z - 0.125 * p.x };
+  return r;
+}
+
+/* identity on every point except the last lattice corner of a grid whose
+ * last x coordinate is 0.5 (exercises a map that departs late) */
+contour3d_vector_t oracle_conv_late(const contour3d_vector_t p) {
+  contour3d_vector_t r = p;
+  if (p.x == 0.5 && p.y == 0.5 && p.z == 0.5) r.z = 0.5 + 1e-3;
+  return r;
+}

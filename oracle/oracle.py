@@ -54,6 +54,11 @@ def identity_converter_address():
     return C.cast(lib().oracle_identity_converter, C.c_void_p).value
 
 
+def converter_address(name):
+    """Address of one of the oracle's test coordinate maps (cylinder, shear, late)."""
+    return C.cast(getattr(lib(), "oracle_conv_" + name), C.c_void_p).value
+
+
 def _ptr(a):
     return None if a is None else a.ctypes.data
 

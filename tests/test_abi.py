@@ -1,0 +1,80 @@
+"""The C-ABI library loads without a GPU, exports every entry point the
+headers declare, and the ctypes mirrors match the C struct layouts that an
+embedding solver compiles against."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from contour3d_amd import _abi, api
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for h in ("contour3d.h", "contour3d_amd.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        names += re.findall(r"extern\s+[\w\s\*]+?\b(contour3d_\w+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = api.lib()
+    decl = declared_functions()
+    assert "contour3d_execute" in decl
+    assert set(decl) == set(api.EXPORTS)
+    for name in decl:
+        assert hasattr(L, name), name
+
+
+def test_export_table_via_nm():
+    out = subprocess.run(["nm", "-D", "--defined-only", api.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    syms = {l.split()[-1] for l in out.splitlines() if l.strip()}
+    for name in declared_functions():
+        assert name in syms, name
+
+
+def test_struct_layouts_match_c(tmp_path):
+    src = tmp_path / "layout.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "contour3d.h"
+#define P(T, m) printf(#T "." #m " %zu\n", offsetof(T, m))
+int main(void) {
+  printf("vector %zu\ncolor %zu\ncontour %zu\nline %zu\n", sizeof(contour3d_vector_t),
+         sizeof(contour3d_color_t), sizeof(contour3d_contour_obj_t), sizeof(contour3d_line_obj_t));
+  P(contour3d_contour_obj_t, glsizes); P(contour3d_contour_obj_t, grids);
+  P(contour3d_contour_obj_t, converter); P(contour3d_contour_obj_t, threshold);
+  P(contour3d_contour_obj_t, color); P(contour3d_contour_obj_t, array);
+  P(contour3d_line_obj_t, nitems); P(contour3d_line_obj_t, converter);
+  P(contour3d_line_obj_t, color); P(contour3d_line_obj_t, width);
+  return 0;
+}
+''')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), "-I",
+                    os.path.join(ROOT, "contour3d_amd", "csrc", "single_rank"), str(src), "-o", str(exe)],
+                   check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                        check=True).stdout.splitlines())
+    got = {k: int(v) for k, v in got.items()}
+    assert got["vector"] == C.sizeof(_abi.Vector)
+    assert got["color"] == C.sizeof(_abi.Color)
+    assert got["contour"] == C.sizeof(_abi.ContourObj)
+    assert got["line"] == C.sizeof(_abi.LineObj)
+    for f in ("glsizes", "grids", "converter", "threshold", "color", "array"):
+        assert got[f"contour3d_contour_obj_t.{f}"] == getattr(_abi.ContourObj, f).offset, f
+    for f in ("nitems", "converter", "color", "width"):
+        assert got[f"contour3d_line_obj_t.{f}"] == getattr(_abi.LineObj, f).offset, f
+
+
+def test_entry_points_reject_bad_arguments_without_touching_memory():
+    L = api.lib()
+    # NULL output pointer for a mesh build is refused before any device work
+    assert L.contour3d_amd_mesh_build(None, 0, 0, None) == 1
+    assert L.contour3d_amd_set_precision(7) == 1

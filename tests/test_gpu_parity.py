@@ -1,0 +1,269 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on the
+same seeded inputs.  FP64 form: triangle sets, vertex positions, weighted
+face normals, vertex normals, z-buffers and pixels are bit-identical.
+FP32 form: the triangle set is identical; positional, normal and pixel
+errors are bounded as stated in each test."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from contour3d_amd import api, fields
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def same_bits(a, b):
+    """Bitwise equality with NaN treated as one value (the CPU and the GPU
+    produce different NaN payloads)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    na, nb = np.isnan(a), np.isnan(b)
+    return np.array_equal(na, nb) and np.array_equal(a[~na].view(np.uint64), b[~nb].view(np.uint64))
+
+
+def cases():
+    rng = np.random.default_rng(11)
+    out = []
+    out.append(("random", rng.standard_normal((12, 13, 14)), [0.0, 0.7], None, None))
+    out.append(("blobs", fields.blobs((33, 33, 33), seed=0), [0.5], None, None))
+    out.append(("turb3", fields.turbulence((20, 24, 28), kmax=8, seed=1), [-1.0, 0.0, 1.0], None, None))
+    f, g = fields.example_field(24)
+    out.append(("example_cheb", f, [-0.25, 0.25], g, None))
+    n = 14
+    r = np.linspace(0.2, 0.5, n)
+    th = np.linspace(0.0, 1.5, n + 1)
+    z = np.linspace(-0.4, 0.4, n + 2)
+    Z, TH, RR = np.meshgrid(z, th, r, indexing="ij")
+    cyl = np.sin(6 * TH) * (RR - 0.35) + 0.3 * Z
+    out.append(("cylinder_conv", cyl, [0.0, 0.05], (r, th, z), "cylinder"))
+    out.append(("shear_conv", rng.standard_normal((9, 10, 11)), [0.1], None, "shear"))
+    lin = np.linspace(-0.5, 0.5, 6)
+    out.append(("late_conv", rng.standard_normal((6, 6, 6)), [0.0], (lin, lin, lin), "late"))
+    out.append(("thin", rng.standard_normal((2, 5, 7)), [0.0], None, None))
+    ties = rng.integers(-2, 3, size=(9, 8, 7)).astype(np.float64)
+    out.append(("ties", ties, [0.0, 1.0], None, None))
+    return out
+
+
+CASES = cases()
+
+
+@pytest.mark.parametrize("name,field,thrs,grids,conv", CASES, ids=[c[0] for c in CASES])
+def test_mesh_fp64_bit_exact(orc, name, field, thrs, grids, conv):
+    banner(f"mesh fp64 {name} shape={field.shape} thr={thrs}")
+    cont = []
+    for q, t in enumerate(thrs):
+        c = dict(field=field, threshold=t, color=(255, 128, 0))
+        if grids is not None:
+            c["grids"] = grids
+        if conv is not None:
+            c["converter"] = orc.converter_address(conv)
+        cont.append(c)
+    mesh = api.Mesh(cont, precision=api.FP64)
+    try:
+        for q, t in enumerate(thrs):
+            if conv is None:
+                ref = orc.mesh(field, t, grids=grids)
+            else:
+                gx, gy, gz = grids if grids is not None else fields.grids_for(field.shape)
+                ref = oracle_mesh_converted(orc, field, t, (gx, gy, gz), conv)
+            got = mesh.copy(q)
+            T = len(ref["area"])
+            assert mesh.count(q) == T
+            if T == 0:
+                continue
+            assert np.array_equal(got["cell"].astype(np.int64), ref["cell"])
+            assert np.array_equal(got["edge"], ref["edge"])
+            assert np.array_equal(got["interior"], ref["interior"])
+            assert same_bits(got["pos"], ref["pos"])
+            assert same_bits(got["wfn"], ref["area"][:, None] * ref["fn"])
+            ins = ref["interior"]
+            assert same_bits(got["vn"][ins], ref["vn"][ins])
+    finally:
+        mesh.free()
+
+
+def oracle_mesh_converted(orc, field, thr, grids, conv):
+    """The oracle's mesh with the lattice mapped through the same C
+    converter the product receives."""
+    import ctypes as C
+    from contour3d_amd import _abi
+
+    nz, ny, nx = field.shape
+    fn = C.CFUNCTYPE(_abi.Vector, _abi.Vector)
+    L = orc.lib()
+    f = getattr(L, "oracle_conv_" + conv)
+    f.restype = _abi.Vector
+    f.argtypes = [_abi.Vector]
+    cx = np.zeros(field.shape)
+    cy = np.zeros(field.shape)
+    cz = np.zeros(field.shape)
+    gx, gy, gz = grids
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                r = f(_abi.Vector(gx[i], gy[j], gz[k]))
+                cx[k, j, i], cy[k, j, i], cz[k, j, i] = r.x, r.y, r.z
+    return orc.mesh(field, thr, grids=grids, coords=(cx, cy, cz))
+
+
+def view_for(W, H):
+    v = fields.example_view(W, H)
+    return dict(camera=v["camera"], light=v["light"], size=(W, H), center=v["center"], local=v["local"],
+                bg=(3, 5, 7))
+
+
+EDGE_LINES = [dict(edges=((-0.5, -0.5, -0.5), (-0.5, 0.5, -0.5)), nitems=24, color=(255, 255, 255), width=3.0),
+              dict(edges=((0.5, -0.5, 0.5), (0.5, 0.5, 0.5)), nitems=24, color=(200, 10, 10), width=2.0),
+              dict(edges=((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5)), nitems=7, color=(10, 200, 10), width=5.0)]
+
+
+def image_cases():
+    f_ex, g_ex = fields.example_field(32)
+    return [
+        ("blobs", [dict(field=fields.blobs((40, 40, 40), seed=2), threshold=0.5, color=(0, 255, 255))],
+         [], (160, 120)),
+        ("turb3", [dict(field=fields.turbulence((32, 32, 32), kmax=10, seed=3), threshold=t, color=c)
+                   for t, c in ((-1.0, (255, 0, 0)), (0.0, (0, 255, 0)), (1.0, (0, 0, 255)))], [], (192, 108)),
+        ("example", [dict(field=f_ex, grids=g_ex, threshold=-0.25, color=(0, 255, 255)),
+                     dict(field=f_ex, grids=g_ex, threshold=0.25, color=(255, 255, 0))], EDGE_LINES, (144, 108)),
+        ("two_fields", [dict(field=fields.blobs((20, 26, 30), seed=4), threshold=0.4, color=(250, 250, 250)),
+                        dict(field=fields.turbulence((24, 20, 22), kmax=6, seed=5), threshold=0.2,
+                             color=(120, 60, 200))], EDGE_LINES[:1], (100, 140)),
+    ]
+
+
+IMAGES = image_cases()
+
+
+@pytest.mark.parametrize("name,contours,lines,size", IMAGES, ids=[c[0] for c in IMAGES])
+def test_image_fp64_bit_exact(orc, name, contours, lines, size):
+    banner(f"image fp64 {name} size={size}")
+    v = view_for(*size)
+    rgb, depth = api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], contours,
+                                 lines, precision=api.FP64, want_depth=True)
+    ref_rgb, ref_depth = orc.execute(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"],
+                                     contours, lines)
+    assert (ref_depth > -1e300).sum() > 50
+    assert same_bits(depth, ref_depth)
+    assert np.array_equal(rgb, ref_rgb)
+
+
+def test_sphere_camera_and_converter_image(orc):
+    banner("image fp64 sphere camera + shear converter")
+    v = fields.sphere_view(0.6, 2.1, 96, 96)
+    f = fields.turbulence((18, 22, 20), kmax=5, seed=8)
+    conv = orc.converter_address("shear")
+    cont = [dict(field=f, threshold=0.3, color=(90, 200, 40), converter=conv)]
+    lines = [dict(edges=((-0.5, -0.5, 0.0), (0.5, 0.5, 0.0)), nitems=5, color=(1, 2, 3), width=2.5,
+                  converter=conv)]
+    rgb, depth = api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], cont,
+                                 lines, want_depth=True)
+    ref_rgb, ref_depth = orc.execute(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], cont,
+                                     lines)
+    assert (ref_depth > -1e300).sum() > 50
+    assert same_bits(depth, ref_depth)
+    assert np.array_equal(rgb, ref_rgb)
+
+
+def test_execute_writes_identical_ppm(orc, tmp_path):
+    banner("contour3d_execute ppm")
+    f = fields.blobs((30, 30, 30), seed=6)
+    v = view_for(120, 90)
+    cont = [dict(field=f, threshold=0.5, color=(0, 255, 255))]
+    out = tmp_path / "gpu.ppm"
+    ref = tmp_path / "cpu.ppm"
+    api.execute(v["camera"], (0, 0, 0), v["light"], v["size"], v["center"], v["local"], v["bg"], cont,
+                EDGE_LINES, fname=out)
+    orc.execute(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], cont, EDGE_LINES, fname=ref)
+    assert out.read_bytes() == ref.read_bytes()
+
+
+def test_device_resident_field_in_place(orc):
+    banner("device-resident field")
+    import torch
+
+    f = fields.turbulence((26, 26, 26), kmax=7, seed=9)
+    v = view_for(128, 96)
+    host = [dict(field=f, threshold=0.0, color=(200, 200, 0))]
+    dev = [dict(field=torch.from_numpy(f).cuda(), threshold=0.0, color=(200, 200, 0))]
+    a, da = api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], host,
+                            want_depth=True)
+    rgb_dev = torch.zeros((96, 128, 3), dtype=torch.uint8, device="cuda")
+    dep_dev = torch.zeros((96, 128), dtype=torch.float64, device="cuda")
+    api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], dev, rgb=rgb_dev,
+                    depth=dep_dev)
+    torch.cuda.synchronize()
+    assert np.array_equal(rgb_dev.cpu().numpy(), a)
+    assert same_bits(dep_dev.cpu().numpy(), da)
+
+
+def test_repeated_frames_identical():
+    banner("repeat")
+    f = fields.turbulence((24, 24, 24), kmax=6, seed=10)
+    v = view_for(80, 60)
+    cont = [dict(field=f, threshold=t, color=(255, 0, 0)) for t in (-0.5, 0.5)]
+    outs = [api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], cont,
+                            want_depth=True) for _ in range(3)]
+    for o in outs[1:]:
+        assert np.array_equal(o[0], outs[0][0])
+        assert same_bits(o[1], outs[0][1])
+
+
+def test_fp32_form_topology_exact_and_errors_bounded(orc):
+    banner("fp32 form")
+    f = fields.turbulence((40, 40, 40), kmax=10, seed=12)
+    thrs = [-1.0, 0.0, 1.0]
+    cont = [dict(field=f, threshold=t, color=(255, 255, 255)) for t in thrs]
+    m64 = api.Mesh(cont, precision=api.FP64)
+    m32 = api.Mesh(cont, precision=api.FP32)
+    try:
+        for q in range(3):
+            a, b = m64.copy(q), m32.copy(q)
+            assert np.array_equal(a["cell"], b["cell"])
+            assert np.array_equal(a["edges"], b["edges"])
+            verr = np.abs(a["pos"] - b["pos"]).max()
+            assert verr < 2e-6, verr          # lattice spacing 1/40; FP32 ulp near 0.5 is 6e-8
+            ins = a["interior"] & np.all(np.isfinite(a["vn"]), axis=(1, 2))
+            cosang = np.clip(np.sum(a["vn"][ins] * b["vn"][ins], axis=2), -1, 1)
+            ang = np.degrees(np.arccos(cosang)).max()
+            assert ang < 0.05, ang
+    finally:
+        m64.free()
+        m32.free()
+    v = view_for(160, 120)
+    r64, d64 = api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], cont,
+                               precision=api.FP64, want_depth=True)
+    r32, d32 = api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], cont,
+                               precision=api.FP32, want_depth=True)
+    cov = d64 > -1e300
+    mism = np.any(r64 != r32, axis=2).sum()
+    assert mism <= 0.02 * cov.sum(), (mism, cov.sum())
+    both = cov & (d32 > -1e300)
+    assert np.abs(d64[both] - d32[both]).max() < 1e-5
+
+
+def test_errors_return_one():
+    banner("error paths")
+    f = np.zeros((1, 4, 4))
+    v = view_for(16, 16)
+    with pytest.raises(RuntimeError):
+        api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"],
+                        [dict(field=f, threshold=0.0, color=(1, 1, 1))])
+    with pytest.raises(RuntimeError):
+        api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], [],
+                        [dict(edges=((0, 0, 0), (1, 1, 1)), nitems=1, color=(1, 1, 1), width=1.0)])
+    with pytest.raises(RuntimeError):
+        api.execute(v["camera"], (0, 0, 0), v["light"], v["size"], v["center"], v["local"], v["bg"], [],
+                    fname="/nonexistent_dir/x.ppm")
+    # the library still works after failures
+    g = fields.blobs((12, 12, 12), seed=1)
+    api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"],
+                    [dict(field=g, threshold=0.5, color=(1, 1, 1))])
