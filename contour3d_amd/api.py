@@ -46,6 +46,13 @@ def lib():
     """Load the native library; raises if it has not been built."""
     global _lib
     if _lib is None:
+        # torch bundles its own HIP runtime under the same soname; loading it
+        # first makes this library bind to that copy, so a process that also
+        # uses torch holds one HIP runtime instead of two
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"native library missing: {LIB_PATH} (run python contour3d_amd/build.py)")
         L = C.CDLL(LIB_PATH)

@@ -64,7 +64,7 @@ def turbulence(shape, kmax=64, seed=0, kmin=1):
     """Spectral synthetic turbulence, normalised to mean 0 and variance 1."""
     amp, phase = _spectrum(shape, kmax, seed, kmin)
     spec = amp * np.exp(1j * phase)
-    f = np.fft.irfftn(spec, s=shape)
+    f = np.fft.irfftn(spec, s=shape, axes=(0, 1, 2))
     f -= f.mean()
     f /= f.std()
     return np.ascontiguousarray(f)
@@ -84,7 +84,7 @@ def turbulence_device(shape, kmax=64, seed=0, kmin=1, device="cuda"):
     del amp, phase
     spec = torch.polar(a, p)
     del a, p
-    f = torch.fft.irfftn(spec, s=shape)
+    f = torch.fft.irfftn(spec, s=shape, dim=(0, 1, 2))
     del spec
     f -= f.mean()
     f /= f.std(unbiased=False)
