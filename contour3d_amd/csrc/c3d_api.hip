@@ -134,7 +134,10 @@ struct Prof {
 
 struct contour3d_amd_mesh {
   int precision = 0;
-  c3d::DevBuf pos, wfn, vn, cell, edges;
+  int precision_built = -1;  // form the buffers were sized for
+  c3d::DevBuf pos, wfn, cell, edges;
+  c3d::DevBuf bstart;  // per-tile triangle starts of every object's field
+  c3d::DevBuf meta;    // ObjMeta per object
   uint64_t cap = 0;
   uint64_t total = 0;
   std::vector<uint64_t> base, count;
@@ -142,9 +145,10 @@ struct contour3d_amd_mesh {
   void release() {
     pos.release();
     wfn.release();
-    vn.release();
     cell.release();
     edges.release();
+    bstart.release();
+    meta.release();
     cap = 0;
   }
 };
@@ -163,12 +167,12 @@ struct Lib {
   std::vector<std::unique_ptr<Geom>> geoms;
   std::vector<std::unique_ptr<DevBuf>> uploads;
   DevBuf cnt, start, tiles, gidx, gout;
-  DevBuf key, id, rgb, depth, fpix, fdep, frgb, obase, orgb;
-  HostBuf hgout, hrgb;
+  DevBuf key, id, rgb, depth, fpix, fdep, frgb, frags, fcount, vntmp;
+  uint32_t frag_cap = 0;
+  HostBuf hgout, hrgb, hfcount;
   // host sources of asynchronous uploads stay alive until the next call
   std::vector<uint32_t> h_boff;
-  std::vector<uint64_t> h_obase;
-  std::vector<uint8_t> h_orgb;
+  std::vector<ObjMeta> h_meta;
   Frags h_frags;
   contour3d_amd_mesh frame_mesh;  // reused by contour3d_execute
   Prof prof;
@@ -189,8 +193,8 @@ int ensure_init(Lib& L, int device) {
   }
   C3D_HIP(hipSetDevice(device));
   C3D_HIP(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
-  C3D_HIP(upload_tables());
   L.nbr = build_nbr_table();
+  C3D_HIP(upload_tables(L.nbr));
   L.device = device;
   L.ready = true;
   return 0;
@@ -386,7 +390,6 @@ MeshView<R> view_of(const contour3d_amd_mesh& M) {
   MeshView<R> m;
   m.pos = M.pos.as<R>();
   m.wfn = M.wfn.as<R>();
-  m.vn = M.vn.as<R>();
   m.cell = M.cell.as<uint32_t>();
   m.edges = M.edges.as<uint32_t>();
   m.cap = M.cap;
@@ -523,17 +526,16 @@ int build_mesh_t(Lib& L, contour3d_amd_mesh& M, const contour3d_contour_obj_t* o
     log_error("%llu triangles exceed the 2^32 triangle ids of one frame", (unsigned long long)M.total);
     return 1;
   }
-  if (M.cap < M.total || M.cap == 0) {
+  if (M.cap < M.total || M.cap == 0 || M.precision_built != M.precision) {
+    M.precision_built = M.precision;
     const uint64_t cap = M.total + M.total / 4 + 1024;
     M.pos.release();
     M.wfn.release();
-    M.vn.release();
     M.cell.release();
     M.edges.release();
     M.cap = 0;
     C3D_HIP(M.pos.ensure(9 * cap * sizeof(R)));
     C3D_HIP(M.wfn.ensure(3 * cap * sizeof(R)));
-    C3D_HIP(M.vn.ensure(9 * cap * sizeof(R)));
     C3D_HIP(M.cell.ensure(cap * sizeof(uint32_t)));
     C3D_HIP(M.edges.ensure(cap * sizeof(uint32_t)));
     M.cap = cap;
@@ -548,20 +550,25 @@ int build_mesh_t(Lib& L, contour3d_amd_mesh& M, const contour3d_contour_obj_t* o
     return 1;
   if (prof_end(L)) return 1;
 
-  if (prof_begin(L, CONTOUR3D_AMD_STAGE_SMOOTH)) return 1;
+  // the mesh keeps the tile starts and per-object metadata that the
+  // on-demand vertex-normal gather reads
+  C3D_HIP(M.bstart.ensure((nbt + 1) * sizeof(uint64_t)));
+  C3D_HIP(hipMemcpyAsync(M.bstart.p, L.start.p, (nbt + 1) * sizeof(uint64_t), hipMemcpyDeviceToDevice, L.st));
+  std::vector<ObjMeta>& hm = L.h_meta;
+  hm.assign(n, ObjMeta{});
   for (size_t o = 0; o < n; o++) {
-    SmoothObj so;
-    so.base = M.base[o];
-    so.count = M.count[o];
-    so.bstart = L.start.as<uint64_t>() + boff[o];
-    so.ncx = fd[o].ncx;
-    so.ncy = fd[o].ncy;
-    so.ncz = fd[o].ncz;
-    so.div_ncx = fd[o].div_ncx;
-    so.div_ncy = fd[o].div_ncy;
-    C3D_HIP(launch_smooth<R>(so, L.nbr, mv, L.st));
+    hm[o].base = M.base[o];
+    hm[o].count = M.count[o];
+    hm[o].bstart = M.bstart.as<uint64_t>() + boff[o];
+    hm[o].ncx = fd[o].ncx;
+    hm[o].ncy = fd[o].ncy;
+    hm[o].ncz = fd[o].ncz;
+    hm[o].rgb[0] = objs[o].color.r;
+    hm[o].rgb[1] = objs[o].color.g;
+    hm[o].rgb[2] = objs[o].color.b;
   }
-  if (prof_end(L)) return 1;
+  C3D_HIP(M.meta.ensure(n * sizeof(ObjMeta)));
+  C3D_HIP(hipMemcpyAsync(M.meta.p, hm.data(), n * sizeof(ObjMeta), hipMemcpyHostToDevice, L.st));
   return 0;
 }
 
@@ -729,37 +736,49 @@ int render_t(Lib& L, const contour3d_amd_mesh& M, const HostView& hv, const Frag
     C3D_HIP(hipMemcpyAsync(L.frgb.p, F.rgb.data(), 3 * size_t(nf), hipMemcpyHostToDevice, L.st));
   }
   const size_t nobj = M.base.size();
-  std::vector<uint64_t>& ob = L.h_obase;
-  std::vector<uint8_t>& orgb = L.h_orgb;
-  ob.assign(nobj ? nobj : 1, 0);
-  orgb.assign(3 * (nobj ? nobj : 1), 0);
-  for (size_t o = 0; o < nobj; o++) {
-    ob[o] = M.base[o];
-    orgb[3 * o + 0] = M.color[o].r;
-    orgb[3 * o + 1] = M.color[o].g;
-    orgb[3 * o + 2] = M.color[o].b;
-  }
-  C3D_HIP(L.obase.ensure(ob.size() * sizeof(uint64_t)));
-  C3D_HIP(L.orgb.ensure(orgb.size()));
-  C3D_HIP(hipMemcpyAsync(L.obase.p, ob.data(), ob.size() * sizeof(uint64_t), hipMemcpyHostToDevice, L.st));
-  C3D_HIP(hipMemcpyAsync(L.orgb.p, orgb.data(), orgb.size(), hipMemcpyHostToDevice, L.st));
-
   const ViewDev<R> v = dev_view<R>(hv);
   const MeshView<R> mv = view_of<R>(M);
   const uint32_t id_base = uint32_t(M.total + 1);
   if (prof_begin(L, CONTOUR3D_AMD_STAGE_RASTER)) return 1;
   C3D_HIP(launch_zinit(L.key.as<uint64_t>(), id, npx, fp64 ? 0 : 1, L.st));
-  for (int pass = 0; pass < (fp64 ? 2 : 1); pass++) {
-    C3D_HIP(launch_raster<R>(mv, M.total, v, pass, L.key.as<uint64_t>(), id, L.st));
-    C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, id_base, pass, fp64 ? 0 : 1,
+  if constexpr (sizeof(R) == 8) {
+    // depth pass recording covered fragments, then the id pass over that
+    // list; a list that outgrew its buffer falls back to a second pass over
+    // the geometry and grows the buffer for the next frame
+    if (L.frag_cap == 0) L.frag_cap = 1u << 22;
+    C3D_HIP(L.frags.ensure(size_t(L.frag_cap) * sizeof(Frag)));
+    C3D_HIP(L.fcount.ensure(2 * sizeof(uint32_t)));
+    C3D_HIP(L.hfcount.ensure(2 * sizeof(uint32_t)));
+    C3D_HIP(launch_fill_u32(L.fcount.as<uint32_t>(), 0u, 2, L.st));
+    C3D_HIP(launch_raster_frags(mv, M.total, v, L.key.as<uint64_t>(), L.frags.as<Frag>(), L.frag_cap,
+                                L.fcount.as<uint32_t>(), L.st));
+    C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, id_base, 0, 0,
+                              L.key.as<uint64_t>(), id, L.st));
+    C3D_HIP(hipMemcpyAsync(L.hfcount.p, L.fcount.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, L.st));
+    C3D_HIP(hipStreamSynchronize(L.st));
+    const uint32_t nfr = static_cast<const uint32_t*>(L.hfcount.p)[0];
+    const uint32_t overflow = static_cast<const uint32_t*>(L.hfcount.p)[1];
+    if (!overflow && nfr <= L.frag_cap) {
+      C3D_HIP(launch_frag_ids(L.frags.as<Frag>(), nfr, L.key.as<uint64_t>(), id, L.st));
+    } else {
+      C3D_HIP(launch_raster<R>(mv, M.total, v, 1, L.key.as<uint64_t>(), id, L.st));
+      if (nfr > L.frag_cap) {
+        const uint64_t grow = uint64_t(nfr) + nfr / 4;
+        L.frag_cap = uint32_t(grow > 0xffffffffull ? 0xffffffffull : grow);
+      }
+    }
+    C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, id_base, 1, 0,
+                              L.key.as<uint64_t>(), id, L.st));
+  } else {
+    C3D_HIP(launch_raster<R>(mv, M.total, v, 0, L.key.as<uint64_t>(), id, L.st));
+    C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, id_base, 0, 1,
                               L.key.as<uint64_t>(), id, L.st));
   }
   if (prof_end(L)) return 1;
   if (prof_begin(L, CONTOUR3D_AMD_STAGE_RESOLVE)) return 1;
-  ObjColors oc{L.obase.as<uint64_t>(), L.orgb.as<uint8_t>(), int(nobj ? nobj : 1)};
   const uint8_t bg[3] = {hv.bg.r, hv.bg.g, hv.bg.b};
-  C3D_HIP(launch_resolve<R>(mv, M.total, v, oc, L.frgb.as<uint8_t>(), bg, L.key.as<uint64_t>(), id, rgb_dev,
-                            depth_dev, L.st));
+  C3D_HIP(launch_resolve<R>(mv, M.total, v, M.meta.as<ObjMeta>(), int(nobj), L.frgb.as<uint8_t>(), bg,
+                            L.key.as<uint64_t>(), id, rgb_dev, depth_dev, L.st));
   if (prof_end(L)) return 1;
   return 0;
 }
@@ -865,11 +884,13 @@ int contour3d_amd_finalize(void) {
   for (auto& u : L.uploads) u->release();
   L.uploads.clear();
   c3d::DevBuf* bufs[] = {&L.cnt, &L.start, &L.tiles, &L.gidx, &L.gout, &L.key, &L.id, &L.rgb,
-                         &L.depth, &L.fpix, &L.fdep, &L.frgb, &L.obase, &L.orgb};
+                         &L.depth, &L.fpix, &L.fdep, &L.frgb, &L.frags, &L.fcount, &L.vntmp};
   for (auto* b : bufs) b->release();
   L.frame_mesh.release();
+  L.frag_cap = 0;
   L.hgout.release();
   L.hrgb.release();
+  L.hfcount.release();
   for (auto e : L.prof.pool) (void)hipEventDestroy(e);
   L.prof.pool.clear();
   L.prof.pending.clear();
@@ -992,36 +1013,54 @@ int contour3d_amd_mesh_copy(const contour3d_amd_mesh_t* mesh, size_t obj, double
   const uint64_t n = mesh->count[obj], b = mesh->base[obj], cap = mesh->cap;
   if (n == 0) return 0;
   const size_t rs = mesh->precision == CONTOUR3D_AMD_FP64 ? 8 : 4;
-  std::vector<unsigned char> tmp(n * rs);
-  auto fetch = [&](const c3d::DevBuf& buf, int ncomp, int comp, double* dst, int stride) -> int {
-    const unsigned char* src = static_cast<const unsigned char*>(buf.p) + (uint64_t(comp) * cap + b) * rs;
-    if (hipMemcpy(tmp.data(), src, n * rs, hipMemcpyDeviceToHost) != hipSuccess) {
-      c3d::log_error("mesh_copy: device copy failed");
-      return 1;
-    }
-    (void)ncomp;
-    for (uint64_t t = 0; t < n; t++) {
+  std::vector<unsigned char> tmp(n * rs * 4);
+  auto real_at = [&](const unsigned char* q) {
+    if (rs == 8) {
       double x;
-      if (rs == 8) {
-        std::memcpy(&x, tmp.data() + t * 8, 8);
-      } else {
-        float y;
-        std::memcpy(&y, tmp.data() + t * 4, 4);
-        x = y;
+      std::memcpy(&x, q, 8);
+      return x;
+    }
+    float y;
+    std::memcpy(&y, q, 4);
+    return double(y);
+  };
+  auto soa = [&](const c3d::DevBuf& buf, int ncomp, double* dst) -> int {
+    for (int comp = 0; comp < ncomp; comp++) {
+      const unsigned char* src = static_cast<const unsigned char*>(buf.p) + (uint64_t(comp) * cap + b) * rs;
+      if (hipMemcpy(tmp.data(), src, n * rs, hipMemcpyDeviceToHost) != hipSuccess) {
+        c3d::log_error("mesh_copy: device copy failed");
+        return 1;
       }
-      dst[t * stride + comp] = x;
+      for (uint64_t t = 0; t < n; t++) dst[t * ncomp + comp] = real_at(tmp.data() + t * rs);
     }
     return 0;
   };
-  for (int c = 0; pos && c < 9; c++)
-    if (fetch(mesh->pos, 9, c, pos, 9)) return 1;
-  for (int c = 0; wfn && c < 3; c++)
-    if (fetch(mesh->wfn, 3, c, wfn, 3)) return 1;
-  for (int c = 0; vn && c < 9; c++)
-    if (fetch(mesh->vn, 9, c, vn, 9)) return 1;
-  if (cell && hipMemcpy(cell, mesh->cell.as<uint32_t>() + b, n * 4, hipMemcpyDeviceToHost) != hipSuccess) return 1;
+  if (pos && soa(mesh->pos, 9, pos)) return 1;
+  if (wfn && soa(mesh->wfn, 3, wfn)) return 1;
   if (edges && hipMemcpy(edges, mesh->edges.as<uint32_t>() + b, n * 4, hipMemcpyDeviceToHost) != hipSuccess)
     return 1;
+  if (vn) {
+    // the mesh keeps no per-triangle vertex normals; they are gathered now
+    // with the same sums the resolve pass uses for nearest triangles
+    if (c3d::ensure_init(L, 0)) return 1;
+    if (L.vntmp.ensure(n * 9 * sizeof(double)) != hipSuccess) {
+      c3d::log_error("mesh_copy: allocation failed");
+      return 1;
+    }
+    const c3d::ObjMeta* md = mesh->meta.as<c3d::ObjMeta>() + obj;
+    hipError_t e = mesh->precision == CONTOUR3D_AMD_FP64
+                       ? c3d::launch_vertex_normals<double>(c3d::view_of<double>(*mesh), md, n,
+                                                            L.vntmp.as<double>(), L.st)
+                       : c3d::launch_vertex_normals<float>(c3d::view_of<float>(*mesh), md, n,
+                                                           L.vntmp.as<double>(), L.st);
+    if (e == hipSuccess) e = hipStreamSynchronize(L.st);
+    if (e == hipSuccess) e = hipMemcpy(vn, L.vntmp.p, n * 9 * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      c3d::log_error("mesh_copy: vertex normals failed: %s", hipGetErrorString(e));
+      return 1;
+    }
+  }
+  if (cell && hipMemcpy(cell, mesh->cell.as<uint32_t>() + b, n * 4, hipMemcpyDeviceToHost) != hipSuccess) return 1;
   return 0;
 }
 

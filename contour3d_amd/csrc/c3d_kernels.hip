@@ -78,12 +78,6 @@ constexpr uint64_t kTetOther = (3ull | 1ull << 3) | (1ull | 5ull << 3) << 6 |
                                (5ull | 4ull << 3) << 12 | (4ull | 6ull << 3) << 18 |
                                (6ull | 2ull << 3) << 24 | (2ull | 3ull << 3) << 30;
 
-__device__ __forceinline__ uint32_t tet_corner(int t, int v) {
-  if (v == 0) return 0;
-  if (v == 1) return 7;
-  return uint32_t(kTetOther >> (6 * t + 3 * (v - 2))) & 7u;
-}
-
 // 4-bit mask of tetra t from the cell's 8-bit corner code (bit n = tetra
 // vertex n above the threshold, triangulate.c:176-179)
 __device__ __forceinline__ uint32_t tet_mask(uint32_t code, int t) {
@@ -184,41 +178,83 @@ __device__ __forceinline__ V3<R> corner_pos(const FieldDev& f, const CellLoc& L,
 }
 
 // ----------------------------------------------------------------------
+// block-wide exclusive scan of a 64-bit value per thread (256 threads); used
+// on four 16-bit counters packed in one word, one per threshold
+
+__device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* s_wave, uint64_t& total) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  uint64_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) s_wave[wave] = x;
+  __syncthreads();
+  uint64_t off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; w++) {
+    const uint64_t s = s_wave[w];
+    off += w < wave ? s : 0ull;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return off + x - v;
+}
+
+// the eight corner values of cell c (corner q = i + 2j + 4k offsets)
+__device__ __forceinline__ void load_corners(const FieldDev& f, const CellLoc& L, double v[8]) {
+  const double* p = f.f + L.p0;
+  const uint64_t sy = f.nx, sz = uint64_t(f.nx) * f.ny;
+  v[0] = p[0];
+  v[1] = p[1];
+  v[2] = p[sy];
+  v[3] = p[sy + 1];
+  v[4] = p[sz];
+  v[5] = p[sz + 1];
+  v[6] = p[sz + sy];
+  v[7] = p[sz + sy + 1];
+}
+
+// triangle counts of up to four thresholds packed 16 bits apart
+__device__ __forceinline__ uint64_t packed_counts(const ThrSet& ts, const double v[8], uint32_t code[kMaxThr]) {
+  uint64_t packed = 0;
+#pragma unroll
+  for (int q = 0; q < kMaxThr; q++) {
+    uint32_t cd = 0;
+    if (q < ts.n) {
+#pragma unroll
+      for (int n = 0; n < 8; n++) cd |= uint32_t(ts.thr[q] < v[n]) << n;
+    }
+    code[q] = cd;
+    packed |= uint64_t(cell_ntri(cd)) << (16 * q);
+  }
+  return packed;
+}
+
+// ----------------------------------------------------------------------
 // tessellation: count pass
 
 __global__ __launch_bounds__(kThreads) void k_count(FieldDev f, ThrSet ts, uint32_t* __restrict__ cnt) {
   const uint32_t tile = xcd_tile(f.nblocks);
   if (tile >= f.nblocks) return;
-  __shared__ uint32_t s_wave[kThreads / 64];
-  uint32_t acc[kMaxThr];
+  __shared__ uint64_t s_wave[kThreads / 64];
+  uint64_t acc = 0;
 #pragma unroll
-  for (int q = 0; q < kMaxThr; q++) acc[q] = 0;
   for (int chunk = 0; chunk < kCellsPerBlock / kThreads; chunk++) {
     const uint32_t c = tile * kCellsPerBlock + chunk * kThreads + threadIdx.x;
     if (c < f.ncells) {
-      const CellLoc L = locate(f, c);
       double v[8];
-#pragma unroll
-      for (int q = 0; q < 8; q++) v[q] = f.f[L.p0 + corner_offset(f, q)];
-#pragma unroll
-      for (int q = 0; q < kMaxThr; q++) {
-        if (q < ts.n) {
-          uint32_t code = 0;
-#pragma unroll
-          for (int n = 0; n < 8; n++) code |= uint32_t(ts.thr[q] < v[n]) << n;
-          acc[q] += cell_ntri(code);
-        }
-      }
+      load_corners(f, locate(f, c), v);
+      uint32_t code[kMaxThr];
+      acc += packed_counts(ts, v, code);
     }
   }
-#pragma unroll
-  for (int q = 0; q < kMaxThr; q++) {
-    if (q < ts.n) {
-      uint32_t tot;
-      block_excl_scan(acc[q], s_wave, tot);
-      if (threadIdx.x == 0) cnt[ts.boff[q] + tile] = tot;
-    }
-  }
+  uint64_t tot;
+  block_excl_scan64(acc, s_wave, tot);
+  if (threadIdx.x < uint32_t(ts.n)) cnt[ts.boff[threadIdx.x] + tile] = uint32_t(tot >> (16 * threadIdx.x)) & 0xffffu;
 }
 
 // ----------------------------------------------------------------------
@@ -307,15 +343,21 @@ __device__ __forceinline__ V3<R> interp(double thr, V3<R> p0, double v0, V3<R> p
   return vadd(p0, vscale(fct, vsub(p1, p0)));
 }
 
+// Each tile of 1024 consecutive cells is processed in chunks of 256: every
+// thread classifies one cell against all thresholds, a block scan of the
+// packed counts places the chunk's triangles (threshold-major, then cell,
+// then slot order), and the whole block then computes the queued triangles
+// one per thread so that the stores of consecutive triangles coalesce.
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_emit(FieldDev f, ThrSet ts, const uint64_t* __restrict__ start,
                                                    MeshView<R> m) {
   const uint32_t tile = xcd_tile(f.nblocks);
   if (tile >= f.nblocks) return;
   __shared__ double s_val[8][kThreads];
+  __shared__ double s_grid[6][kThreads];  // x0 x1 y0 y1 z0 z1 of each cell (identity map)
   __shared__ uint8_t s_code[kMaxThr][kThreads];
-  __shared__ uint16_t s_q[kThreads * 12];
-  __shared__ uint32_t s_wave[kThreads / 64];
+  __shared__ uint16_t s_q[kThreads * 12 * kMaxThr];
+  __shared__ uint64_t s_wave[kThreads / 64];
   uint64_t base[kMaxThr];
 #pragma unroll
   for (int q = 0; q < kMaxThr; q++) base[q] = q < ts.n ? start[ts.boff[q] + tile] : 0;
@@ -328,79 +370,111 @@ __global__ __launch_bounds__(kThreads) void k_emit(FieldDev f, ThrSet ts, const 
     double v[8];
     if (valid) {
       const CellLoc L = locate(f, c);
-#pragma unroll
-      for (int q = 0; q < 8; q++) v[q] = f.f[L.p0 + corner_offset(f, q)];
+      load_corners(f, L, v);
+      if (!f.cx) {
+        s_grid[0][threadIdx.x] = f.gx[L.i];
+        s_grid[1][threadIdx.x] = f.gx[L.i + 1];
+        s_grid[2][threadIdx.x] = f.gy[L.j];
+        s_grid[3][threadIdx.x] = f.gy[L.j + 1];
+        s_grid[4][threadIdx.x] = f.gz[L.k];
+        s_grid[5][threadIdx.x] = f.gz[L.k + 1];
+      }
     } else {
 #pragma unroll
       for (int q = 0; q < 8; q++) v[q] = 0.0;
     }
 #pragma unroll
     for (int q = 0; q < 8; q++) s_val[q][threadIdx.x] = v[q];
-
-    for (int th = 0; th < ts.n; th++) {
-      const double thr = ts.thr[th];
-      uint32_t code = 0;
+    uint32_t code[kMaxThr];
+    const uint64_t packed = valid ? packed_counts(ts, v, code) : 0ull;
+    uint64_t tot;
+    const uint64_t ex = block_excl_scan64(packed, s_wave, tot);
+    uint32_t qoff[kMaxThr + 1];
+    qoff[0] = 0;
 #pragma unroll
-      for (int n = 0; n < 8; n++) code |= uint32_t(thr < v[n]) << n;
-      const uint32_t nt = valid ? cell_ntri(code) : 0u;
-      uint32_t total;
-      uint32_t o = block_excl_scan(nt, s_wave, total);
-      s_code[th][threadIdx.x] = uint8_t(code);
-      if (nt) {
-        for (int t = 0; t < 6; t++) {
-          const uint32_t k = tet_ntri(tet_mask(code, t));
-          for (uint32_t s = 0; s < k; s++) s_q[o++] = uint16_t(threadIdx.x | t << 8 | s << 11);
+    for (int q = 0; q < kMaxThr; q++) {
+      const uint32_t tq = uint32_t(tot >> (16 * q)) & 0xffffu;
+      qoff[q + 1] = qoff[q] + tq;
+      if (q < ts.n) {
+        const uint32_t eq = uint32_t(ex >> (16 * q)) & 0xffffu;
+        if (valid) {
+          s_code[q][threadIdx.x] = uint8_t(code[q]);
+          uint32_t o = qoff[q] + eq;
+          const uint32_t cd = code[q];
+          if (uint32_t(packed >> (16 * q)) & 0xffffu) {
+            for (int t = 0; t < 6; t++) {
+              const uint32_t k = tet_ntri(tet_mask(cd, t));
+              for (uint32_t sl = 0; sl < k; sl++)
+                s_q[o++] = uint16_t(threadIdx.x | t << 8 | sl << 11 | q << 12);
+            }
+          }
         }
       }
-      __syncthreads();
-      for (uint32_t it = threadIdx.x; it < total; it += kThreads) {
-        const uint32_t item = s_q[it];
-        const uint32_t l = item & 255u;
-        const int t = int(item >> 8) & 7;
-        const uint32_t s = item >> 11;
-        const uint32_t cc = c0 + l;
-        const CellLoc L = locate(f, cc);
-        const uint32_t cd = s_code[th][l];
-        const uint32_t mt = tet_mask(cd, t);
-        const bool rev = mt >= 8;
-        const uint32_t mm = rev ? 15u - mt : mt;
-        const uint32_t ent = c_tri[(t * 8 + mm) * 2 + s];
-        V3<R> P[3];
-        uint32_t E = 0;
-#pragma unroll
-        for (int vv = 0; vv < 3; vv++) {
-          const int q = rev ? 2 - vv : vv;   // triangulate.c:108-112
-          const uint32_t tail = (ent >> (6 * q)) & 7u;
-          const uint32_t head = (ent >> (6 * q + 3)) & 7u;
-          P[vv] = interp<R>(thr, corner_pos<R>(f, L, tail), s_val[tail][l], corner_pos<R>(f, L, head),
-                            s_val[head][l]);
-          E |= edge_id(tail, head) << (5 * vv);
-        }
-        // face normal and area (triangulate.c:75-93); the smoothing weight
-        // area * unit normal is formed as normal.c:39-40 forms it
-        const V3<R> n = vcross(vsub(P[1], P[0]), vsub(P[2], P[0]));
-        const R area = sqrt(vdot(n, n));
-        const V3<R> fn = vscale(R(1) / area, n);
-        const V3<R> w = vscale(area, fn);
-        const bool in = L.i >= 1 && L.i + 2 <= f.ncx && L.j >= 1 && L.j + 2 <= f.ncy && L.k >= 1 &&
-                        L.k + 2 <= f.ncz;
-        const uint64_t g = base[th] + it;
-        const uint64_t cap = m.cap;
-#pragma unroll
-        for (int vv = 0; vv < 3; vv++) {
-          m.pos[(3 * vv + 0) * cap + g] = P[vv].x;
-          m.pos[(3 * vv + 1) * cap + g] = P[vv].y;
-          m.pos[(3 * vv + 2) * cap + g] = P[vv].z;
-        }
-        m.wfn[0 * cap + g] = w.x;
-        m.wfn[1 * cap + g] = w.y;
-        m.wfn[2 * cap + g] = w.z;
-        m.cell[g] = cc;
-        m.edges[g] = E | uint32_t(in) << 15;
-      }
-      base[th] += total;
-      __syncthreads();
     }
+    __syncthreads();
+    const uint32_t total = qoff[kMaxThr];
+    for (uint32_t it = threadIdx.x; it < total; it += kThreads) {
+      const uint32_t item = s_q[it];
+      const uint32_t l = item & 255u;
+      const int t = int(item >> 8) & 7;
+      const uint32_t sl = (item >> 11) & 1u;
+      const int q = int(item >> 12);
+      const double thr = ts.thr[q];
+      const uint32_t cc = c0 + l;
+      const CellLoc L = locate(f, cc);
+      const uint32_t mt = tet_mask(s_code[q][l], t);
+      const bool rev = mt >= 8;
+      const uint32_t mm = rev ? 15u - mt : mt;
+      const uint32_t ent = c_tri[(t * 8 + mm) * 2 + sl];
+      V3<R> P[3];
+      uint32_t E = 0;
+#pragma unroll
+      for (int vv = 0; vv < 3; vv++) {
+        const int qq = rev ? 2 - vv : vv;   // triangulate.c:108-112
+        const uint32_t tail = (ent >> (6 * qq)) & 7u;
+        const uint32_t head = (ent >> (6 * qq + 3)) & 7u;
+        V3<R> pt, ph;
+        if (f.cx) {
+          pt = corner_pos<R>(f, L, tail);
+          ph = corner_pos<R>(f, L, head);
+        } else {
+          pt = {R(s_grid[tail & 1u][l]), R(s_grid[2 + ((tail >> 1) & 1u)][l]), R(s_grid[4 + (tail >> 2)][l])};
+          ph = {R(s_grid[head & 1u][l]), R(s_grid[2 + ((head >> 1) & 1u)][l]), R(s_grid[4 + (head >> 2)][l])};
+        }
+        P[vv] = interp<R>(thr, pt, s_val[tail][l], ph, s_val[head][l]);
+        E |= edge_id(tail, head) << (5 * vv);
+      }
+      // face normal and area (triangulate.c:75-93); the smoothing weight
+      // area * unit normal is formed as normal.c:39-40 forms it
+      const V3<R> n = vcross(vsub(P[1], P[0]), vsub(P[2], P[0]));
+      const R area = sqrt(vdot(n, n));
+      const V3<R> fn = vscale(R(1) / area, n);
+      const V3<R> w = vscale(area, fn);
+      const bool in = L.i >= 1 && L.i + 2 <= f.ncx && L.j >= 1 && L.j + 2 <= f.ncy && L.k >= 1 &&
+                      L.k + 2 <= f.ncz;
+      uint32_t qsel = 0;
+#pragma unroll
+      for (int z = 1; z < kMaxThr; z++) qsel = q == z ? qoff[z] : qsel;
+      uint64_t bq = base[0];
+#pragma unroll
+      for (int z = 1; z < kMaxThr; z++) bq = q == z ? base[z] : bq;
+      const uint64_t g = bq + (it - qsel);
+      const uint64_t cap = m.cap;
+#pragma unroll
+      for (int vv = 0; vv < 3; vv++) {
+        m.pos[(3 * vv + 0) * cap + g] = P[vv].x;
+        m.pos[(3 * vv + 1) * cap + g] = P[vv].y;
+        m.pos[(3 * vv + 2) * cap + g] = P[vv].z;
+      }
+      m.wfn[0 * cap + g] = w.x;
+      m.wfn[1 * cap + g] = w.y;
+      m.wfn[2 * cap + g] = w.z;
+      m.edges[g] = E | uint32_t(in) << 15;
+      m.cell[g] = cc;
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxThr; q++) base[q] += qoff[q + 1] - qoff[q];
+    __syncthreads();
   }
 }
 
@@ -408,8 +482,10 @@ __global__ __launch_bounds__(kThreads) void k_emit(FieldDev f, ThrSet ts, const 
 // vertex-normal smoothing: gather over the cells sharing each vertex's
 // lattice edge, in increasing cell order and slot order (normal.c:6-165)
 
-__device__ __forceinline__ uint64_t lower_bound_cell(const uint32_t* __restrict__ cell, uint64_t lo,
-                                                     uint64_t hi, uint32_t key) {
+__constant__ NbrTable c_nbr;
+
+__device__ __forceinline__ uint64_t lower_bound_cell(const uint32_t* __restrict__ cell, uint64_t lo, uint64_t hi,
+                                                     uint32_t key) {
   while (lo < hi) {
     const uint64_t mid = (lo + hi) >> 1;
     if (cell[mid] < key) lo = mid + 1;
@@ -418,40 +494,52 @@ __device__ __forceinline__ uint64_t lower_bound_cell(const uint32_t* __restrict_
   return lo;
 }
 
+// Unit vertex normal of a vertex on cell-local lattice edge e of cell c
+// (normal.c:46-165): the sum of area * unit face normal over every triangle
+// of the cells sharing the edge that has a vertex on it -- cells in
+// increasing order, triangles in slot order, starting from +0 -- then
+// normalised.  A cell's triangles are found from its tile's triangle range
+// (triangles are stored in cell order) by binary search on the cell index.
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_smooth(SmoothObj o, NbrTable nt, MeshView<R> m) {
-  const uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
-  if (t >= o.count) return;
-  const uint64_t g = o.base + t;
-  const uint32_t ed = m.edges[g];
-  if (!((ed >> 15) & 1u)) return;
-  const uint32_t c = m.cell[g];
+__device__ V3<R> vertex_normal(const MeshView<R>& m, const ObjMeta& om, uint32_t c, uint32_t e) {
+  const int64_t sy = om.ncx, sz = int64_t(om.ncx) * om.ncy;
   const uint64_t cap = m.cap;
-  const int64_t sy = o.ncx;
-  const int64_t sz = int64_t(o.ncx) * o.ncy;
-#pragma unroll 1
-  for (int vv = 0; vv < 3; vv++) {
-    const uint32_t e = (ed >> (5 * vv)) & 31u;
-    V3<R> sum = {R(0), R(0), R(0)};
-    for (int q = 0; q < nt.n[e]; q++) {
-      const int dk = nt.v[e][q][0], dj = nt.v[e][q][1], di = nt.v[e][q][2];
-      const uint32_t e2 = uint32_t(nt.v[e][q][3]);
-      const uint32_t c2 = uint32_t(int64_t(c) + dk * sz + dj * sy + di);
-      const uint32_t tl = c2 / kCellsPerBlock;
-      const uint64_t lo = o.base + o.bstart[tl] - o.bstart[0];
-      const uint64_t hi = o.base + o.bstart[tl + 1] - o.bstart[0];
-      for (uint64_t u = lower_bound_cell(m.cell, lo, hi, c2); u < hi && m.cell[u] == c2; u++) {
-        const uint32_t eu = m.edges[u];
-        if ((eu & 31u) == e2 || ((eu >> 5) & 31u) == e2 || ((eu >> 10) & 31u) == e2) {
-          const V3<R> w = {m.wfn[u], m.wfn[cap + u], m.wfn[2 * cap + u]};
-          sum = vadd(sum, w);
-        }
-      }
+  V3<R> sum = {R(0), R(0), R(0)};
+  const int nn = c_nbr.n[e];
+  for (int q = 0; q < nn; q++) {
+    const int dk = c_nbr.v[e][q][0], dj = c_nbr.v[e][q][1], di = c_nbr.v[e][q][2];
+    const uint32_t want = uint32_t(c_nbr.v[e][q][3]);
+    const uint32_t c2 = uint32_t(int64_t(c) + dk * sz + dj * sy + di);
+    const uint32_t tl = c2 / kCellsPerBlock;
+    const uint64_t hi = om.bstart[tl + 1];
+    for (uint64_t u = lower_bound_cell(m.cell, om.bstart[tl], hi, c2); u < hi && m.cell[u] == c2; u++) {
+      const uint32_t eu = m.edges[u];
+      if ((eu & 31u) == want || ((eu >> 5) & 31u) == want || ((eu >> 10) & 31u) == want)
+        sum = vadd(sum, V3<R>{m.wfn[u], m.wfn[cap + u], m.wfn[2 * cap + u]});
     }
-    const V3<R> n = vunit(sum);
-    m.vn[(3 * vv + 0) * cap + g] = n.x;
-    m.vn[(3 * vv + 1) * cap + g] = n.y;
-    m.vn[(3 * vv + 2) * cap + g] = n.z;
+  }
+  return vunit(sum);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_vertex_normals(MeshView<R> m, const ObjMeta* __restrict__ meta,
+                                                             uint64_t count, double* __restrict__ out) {
+  const uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+  if (t >= count) return;
+  const ObjMeta om = *meta;
+  const uint64_t g = om.base + t;
+  const uint32_t ed = m.edges[g];
+  const bool in = (ed >> 15) & 1u;
+  for (int vv = 0; vv < 3; vv++) {
+    V3<double> n = {__longlong_as_double(0x7ff8000000000000ll), __longlong_as_double(0x7ff8000000000000ll),
+                    __longlong_as_double(0x7ff8000000000000ll)};
+    if (in) {
+      const V3<R> r = vertex_normal<R>(m, om, m.cell[g], (ed >> (5 * vv)) & 31u);
+      n = {double(r.x), double(r.y), double(r.z)};
+    }
+    out[9 * t + 3 * vv + 0] = n.x;
+    out[9 * t + 3 * vv + 1] = n.y;
+    out[9 * t + 3 * vv + 2] = n.z;
   }
 }
 
@@ -578,19 +666,88 @@ __global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, uint64_t ntr
       if (!pixel_weights(v, P, i, j, w0, w1, w2)) continue;
       const R dep = pixel_depth(P, w0, w1, w2);
       const uint64_t px = uint64_t(j) * uint64_t(v.W) + uint64_t(i);
+      // the stored keys only grow, so a key below any value read from the
+      // buffer can never change it and skips the atomic
       if constexpr (sizeof(R) == 8) {
         const uint64_t k = ord64(dep);
         if (PASS == 0) {
-          atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
-        } else if (k == __hip_atomic_load(key + px, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+        } else if (k == key[px]) {
           atomicMax(id + px, uint32_t(t + 1));
         }
       } else {
         const uint64_t k = uint64_t(ord32(dep)) << 32 | uint64_t(t + 1);
-        atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+        if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
       }
     }
   }
+}
+
+// FP64 depth pass that also keeps the fragments that may end up nearest: a
+// fragment whose key is below a value already in the z-buffer can never
+// win (keys only grow), every other one goes through atomicMax and is kept
+// when the value it replaced was not larger.  The kept fragments collect in
+// a workgroup list in LDS and are flushed with one global reservation per
+// workgroup; *counter[1] is set when a list overflows (LDS or global), in
+// which case the id pass has to walk the geometry instead.
+constexpr int kFragLds = 2048;
+
+__global__ __launch_bounds__(kThreads) void k_raster_frags(MeshView<double> m, uint64_t ntri, ViewDev<double> v,
+                                                           uint64_t* __restrict__ key, Frag* __restrict__ frags,
+                                                           uint32_t cap, uint32_t* __restrict__ counter) {
+  __shared__ Frag s_frag[kFragLds];
+  __shared__ uint32_t s_n, s_base;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  const uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+  Projected<double> P;
+  if (t < ntri && ((m.edges[t] >> 15) & 1u) && setup_tri(m, t, v, P)) {
+    for (int32_t j = P.j0; j <= P.j1; j++) {
+      for (int32_t i = P.i0; i <= P.i1; i++) {
+        double w0, w1, w2;
+        if (!pixel_weights(v, P, i, j, w0, w1, w2)) continue;
+        const uint64_t px = uint64_t(j) * uint64_t(v.W) + uint64_t(i);
+        const uint64_t k = ord64(pixel_depth(P, w0, w1, w2));
+        if (k < key[px]) continue;
+        const uint64_t old = atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+        if (k < old) continue;
+        const uint32_t o = atomicAdd(&s_n, 1u);
+        if (o < uint32_t(kFragLds)) {
+          Frag fr;
+          fr.px = uint32_t(px);
+          fr.id = uint32_t(t + 1);
+          fr.key = k;
+          s_frag[o] = fr;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t n = s_n < uint32_t(kFragLds) ? s_n : uint32_t(kFragLds);
+  if (threadIdx.x == 0) {
+    s_base = n ? atomicAdd(counter, n) : 0u;
+    if (s_n > uint32_t(kFragLds)) atomicOr(counter + 1, 1u);
+  }
+  __syncthreads();
+  const uint32_t b0 = s_base;
+  if (threadIdx.x == 0 && n && uint64_t(b0) + n > cap) atomicOr(counter + 1, 1u);
+  for (uint32_t q = threadIdx.x; q < n; q += kThreads) {
+    if (uint64_t(b0) + q < cap) frags[b0 + q] = s_frag[q];
+  }
+}
+
+// the latest triangle among the fragments at each pixel's final depth
+__global__ void k_frag_ids(const Frag* __restrict__ frags, uint32_t n, const uint64_t* __restrict__ key,
+                           uint32_t* __restrict__ id) {
+  const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  const Frag fr = frags[f];
+  if (fr.key == key[fr.px]) atomicMax(id + fr.px, fr.id);
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t val, uint64_t n) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = val;
 }
 
 __global__ void k_zinit(uint64_t* __restrict__ key, uint32_t* __restrict__ id, uint64_t npx, uint64_t k0) {
@@ -619,11 +776,14 @@ __global__ void k_line_frags(const uint32_t* __restrict__ pix, const double* __r
   }
 }
 
-// final colour of every pixel from the winning fragment; rgb rows are
-// written top-down (y index H-1 first) as the PPM writer packs them
-// (output.c:113-123)
+// Final colour of every pixel from the winning fragment.  The vertex normals
+// of a winning triangle are gathered here, on demand: the image needs them
+// only where a triangle is nearest, so the smoothing of normal.c is done for
+// those triangles alone, with the same sums.  rgb rows are written top-down
+// (y index H-1 first) as the PPM writer packs them (output.c:113-123).
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_resolve(MeshView<R> m, uint64_t ntri, ViewDev<R> v, ObjColors oc,
+__global__ __launch_bounds__(kThreads) void k_resolve(MeshView<R> m, uint64_t ntri, ViewDev<R> v,
+                                                      const ObjMeta* __restrict__ meta, int nobj,
                                                       const uint8_t* __restrict__ frag_rgb, uchar4 bg,
                                                       const uint64_t* __restrict__ key,
                                                       const uint32_t* __restrict__ idbuf, uint8_t* __restrict__ rgb,
@@ -646,28 +806,28 @@ __global__ __launch_bounds__(kThreads) void k_resolve(MeshView<R> m, uint64_t nt
   uint8_t r = bg.x, gg = bg.y, b = bg.z;
   if (id != 0 && id <= ntri) {
     const uint64_t t = id - 1;
+    int lo = 0, hi = nobj - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (meta[mid].base <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const ObjMeta om = meta[lo];
     Projected<R> P;
     setup_tri(m, t, v, P);
     R w0, w1, w2;
     pixel_weights(v, P, i, j, w0, w1, w2);
-    const uint64_t cap = m.cap;
+    const uint32_t c = m.cell[t];
+    const uint32_t ed = m.edges[t];
     V3<R> nv[3];
-#pragma unroll
-    for (int n = 0; n < 3; n++) {
-      nv[n] = {m.vn[(3 * n + 0) * cap + t], m.vn[(3 * n + 1) * cap + t], m.vn[(3 * n + 2) * cap + t]};
-    }
+#pragma unroll 1
+    for (int n = 0; n < 3; n++) nv[n] = vertex_normal<R>(m, om, c, (ed >> (5 * n)) & 31u);
     const V3<R> n = vunit(vadd(vscale(w0, nv[0]), vadd(vscale(w1, nv[1]), vscale(w2, nv[2]))));
     const V3<R> light = {v.light[0], v.light[1], v.light[2]};
     const R fct = fmax(R(0.15), fabs(vdot(n, light)));
-    int lo = 0, hi = oc.n - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (oc.base[mid] <= t) lo = mid;
-      else hi = mid - 1;
-    }
-    r = uint8_t(fct * R(oc.rgb[3 * lo + 0]));
-    gg = uint8_t(fct * R(oc.rgb[3 * lo + 1]));
-    b = uint8_t(fct * R(oc.rgb[3 * lo + 2]));
+    r = uint8_t(fct * R(om.rgb[0]));
+    gg = uint8_t(fct * R(om.rgb[1]));
+    b = uint8_t(fct * R(om.rgb[2]));
   } else if (id > ntri) {
     const uint64_t f = id - ntri - 1;
     r = frag_rgb[3 * f + 0];
@@ -684,7 +844,7 @@ __global__ __launch_bounds__(kThreads) void k_resolve(MeshView<R> m, uint64_t nt
 // ----------------------------------------------------------------------
 // host launchers
 
-hipError_t upload_tables() {
+hipError_t upload_tables(const NbrTable& nt) {
   // case table of triangulate.c:204-270: reduced mask -> triangles as
   // (tail, head) tetra-vertex pairs
   static const int ntri[8] = {0, 1, 1, 2, 1, 2, 2, 1};
@@ -714,7 +874,9 @@ hipError_t upload_tables() {
       }
     }
   }
-  return hipMemcpyToSymbol(HIP_SYMBOL(c_tri), h, sizeof(h));
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tri), h, sizeof(h));
+  if (e != hipSuccess) return e;
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_nbr), &nt, sizeof(nt));
 }
 
 NbrTable build_nbr_table() {
@@ -788,7 +950,7 @@ hipError_t launch_count(const FieldDev& f, const ThrSet& ts, uint32_t* cnt, hipS
 
 hipError_t launch_scan(const uint32_t* cnt, uint64_t* start, uint64_t n, uint64_t* tile_tmp, hipStream_t st) {
   const uint64_t ntile = (n + kScanTile - 1) / kScanTile;
-  if (ntile == 0) return hipMemsetAsync(start, 0, sizeof(uint64_t), st);
+  if (ntile == 0) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_scan_reduce, dim3(uint32_t(ntile)), dim3(kThreads), 0, st, cnt, n, tile_tmp);
   hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kThreads), 0, st, tile_tmp, ntile);
   hipLaunchKernelGGL(k_scan_apply, dim3(uint32_t(ntile)), dim3(kThreads), 0, st, cnt, n,
@@ -810,10 +972,31 @@ hipError_t launch_emit(const FieldDev& f, const ThrSet& ts, const uint64_t* star
 }
 
 template <typename R>
-hipError_t launch_smooth(const SmoothObj& o, const NbrTable& nt, const MeshView<R>& m, hipStream_t st) {
-  if (o.count == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_smooth<R>, dim3(uint32_t((o.count + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, o,
-                     nt, m);
+hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, uint64_t count, double* out,
+                                 hipStream_t st) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vertex_normals<R>, dim3(uint32_t((count + kThreads - 1) / kThreads)), dim3(kThreads), 0, st,
+                     m, meta_dev, count, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_raster_frags(const MeshView<double>& m, uint64_t ntri, const ViewDev<double>& v, uint64_t* key,
+                               Frag* frags, uint32_t cap, uint32_t* counter, hipStream_t st) {
+  if (ntri == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_raster_frags, dim3(uint32_t((ntri + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, m,
+                     ntri, v, key, frags, cap, counter);
+  return hipGetLastError();
+}
+
+hipError_t launch_frag_ids(const Frag* frags, uint32_t n, const uint64_t* key, uint32_t* id, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_frag_ids, dim3((n + 255) / 256), dim3(256), 0, st, frags, n, key, id);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t val, uint64_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, st, p, val, n);
   return hipGetLastError();
 }
 
@@ -844,13 +1027,13 @@ hipError_t launch_line_frags(const uint32_t* pix, const double* depth, uint32_t 
 }
 
 template <typename R>
-hipError_t launch_resolve(const MeshView<R>& m, uint64_t ntri, const ViewDev<R>& v, const ObjColors& oc,
+hipError_t launch_resolve(const MeshView<R>& m, uint64_t ntri, const ViewDev<R>& v, const ObjMeta* meta, int nobj,
                           const uint8_t* frag_rgb, const uint8_t bg[3], const uint64_t* key, const uint32_t* id,
                           uint8_t* rgb, double* depth, hipStream_t st) {
   const uint64_t npx = uint64_t(v.W) * uint64_t(v.H);
   const uchar4 b = make_uchar4(bg[0], bg[1], bg[2], 0);
   hipLaunchKernelGGL(k_resolve<R>, dim3(uint32_t((npx + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, m, ntri,
-                     v, oc, frag_rgb, b, key, id, rgb, depth);
+                     v, meta, nobj, frag_rgb, b, key, id, rgb, depth);
   return hipGetLastError();
 }
 
@@ -858,17 +1041,19 @@ template hipError_t launch_emit<double>(const FieldDev&, const ThrSet&, const ui
                                         hipStream_t);
 template hipError_t launch_emit<float>(const FieldDev&, const ThrSet&, const uint64_t*, const MeshView<float>&,
                                        hipStream_t);
-template hipError_t launch_smooth<double>(const SmoothObj&, const NbrTable&, const MeshView<double>&, hipStream_t);
-template hipError_t launch_smooth<float>(const SmoothObj&, const NbrTable&, const MeshView<float>&, hipStream_t);
+template hipError_t launch_vertex_normals<double>(const MeshView<double>&, const ObjMeta*, uint64_t, double*,
+                                                  hipStream_t);
+template hipError_t launch_vertex_normals<float>(const MeshView<float>&, const ObjMeta*, uint64_t, double*,
+                                                 hipStream_t);
 template hipError_t launch_raster<double>(const MeshView<double>&, uint64_t, const ViewDev<double>&, int, uint64_t*,
                                           uint32_t*, hipStream_t);
 template hipError_t launch_raster<float>(const MeshView<float>&, uint64_t, const ViewDev<float>&, int, uint64_t*,
                                          uint32_t*, hipStream_t);
 template hipError_t launch_resolve<double>(const MeshView<double>&, uint64_t, const ViewDev<double>&,
-                                           const ObjColors&, const uint8_t*, const uint8_t*, const uint64_t*,
+                                           const ObjMeta*, int, const uint8_t*, const uint8_t*, const uint64_t*,
                                            const uint32_t*, uint8_t*, double*, hipStream_t);
-template hipError_t launch_resolve<float>(const MeshView<float>&, uint64_t, const ViewDev<float>&, const ObjColors&,
-                                          const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*,
+template hipError_t launch_resolve<float>(const MeshView<float>&, uint64_t, const ViewDev<float>&, const ObjMeta*,
+                                          int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*,
                                           uint8_t*, double*, hipStream_t);
 
 }  // namespace c3d
