@@ -74,51 +74,6 @@ __device__ __forceinline__ V3<R> vunit(V3<R> a) {
 // Cube corner c = i + 2j + 4k.  The six tetrahedra share corners 0 and 7;
 // their other two corners, in tetra-vertex order 2 and 3, are (3,1) (1,5)
 // (5,4) (4,6) (6,2) (2,3) (triangulate.c:145-152), packed 6 bits per tetra.
-constexpr uint64_t kTetOther = (3ull | 1ull << 3) | (1ull | 5ull << 3) << 6 |
-                               (5ull | 4ull << 3) << 12 | (4ull | 6ull << 3) << 18 |
-                               (6ull | 2ull << 3) << 24 | (2ull | 3ull << 3) << 30;
-
-// 4-bit mask of tetra t from the cell's 8-bit corner code (bit n = tetra
-// vertex n above the threshold, triangulate.c:176-179)
-__device__ __forceinline__ uint32_t tet_mask(uint32_t code, int t) {
-  const uint32_t a = uint32_t(kTetOther >> (6 * t)) & 7u;
-  const uint32_t b = uint32_t(kTetOther >> (6 * t + 3)) & 7u;
-  return (code & 1u) | ((code >> 7) & 1u) << 1 | ((code >> a) & 1u) << 2 | ((code >> b) & 1u) << 3;
-}
-
-// 0 or 4 vertices above: none; 1 or 3: one triangle; 2: two (a quad)
-__device__ __forceinline__ uint32_t tet_ntri(uint32_t m) {
-  const uint32_t pc = __popc(m);
-  return pc == 2 ? 2u : (pc & 1u);
-}
-
-__device__ __forceinline__ uint32_t cell_ntri(uint32_t code) {
-  uint32_t n = 0;
-#pragma unroll
-  for (int t = 0; t < 6; t++) n += tet_ntri(tet_mask(code, t));
-  return n;
-}
-
-// (tetra, reduced mask 1..7, triangle 0/1) -> three (tail, head) cube-corner
-// pairs, 6 bits per vertex; filled from the case table of
-// triangulate.c:204-270 by upload_tables().
-__constant__ uint32_t c_tri[6 * 8 * 2];
-
-// cell-local id of the lattice edge between cube corners a and b
-// (numbering of triangulate.c:23-32)
-__device__ __forceinline__ uint32_t edge_id(uint32_t a, uint32_t b) {
-  const uint32_t lo = a < b ? a : b;
-  switch (a ^ b) {
-    case 1: return lo >> 1;
-    case 2: return 4 + (lo & 1) + 2 * (lo >> 2);
-    case 4: return 8 + (lo & 3);
-    case 6: return 12 + (lo & 1);
-    case 5: return 14 + ((lo >> 1) & 1);
-    case 3: return 16 + (lo >> 2);
-    default: return 18;
-  }
-}
-
 // ----------------------------------------------------------------------
 // block-wide exclusive scan of one value per thread (256 threads)
 
@@ -218,18 +173,35 @@ __device__ __forceinline__ void load_corners(const FieldDev& f, const CellLoc& L
   v[7] = p[sz + sy + 1];
 }
 
-// triangle counts of up to four thresholds packed 16 bits apart
-__device__ __forceinline__ uint64_t packed_counts(const ThrSet& ts, const double v[8], uint32_t code[kMaxThr]) {
+// Topology tables, built once by upload_tables from the tetra and case
+// tables of triangulate.c:145-152, 204-270:
+//   c_ntri[code]            triangles of a cell with that 8-bit corner code
+//   c_cslot[code * 12 + s]  triangle slot s of such a cell: bits 0-17 hold the
+//                           (tail, head) cube corners of its vertices 0, 1, 2
+//                           (3 + 3 bits each, reversal of triangulate.c:108-112
+//                           already applied), bits 18-32 the three 5-bit
+//                           cell-local edge ids (triangulate.c:23-32)
+__constant__ uint8_t c_ntri[256];
+__constant__ uint64_t c_cslot[256 * 12];
+
+__device__ __forceinline__ uint32_t corner_code(double thr, const double v[8]) {
+  uint32_t code = 0;
+#pragma unroll
+  for (int n = 0; n < 8; n++) code |= uint32_t(thr < v[n]) << n;
+  return code;
+}
+
+// triangle counts of up to four thresholds packed 16 bits apart, from an
+// LDS copy of c_ntri
+__device__ __forceinline__ uint64_t packed_counts(const ThrSet& ts, const double v[8], uint32_t code[kMaxThr],
+                                                  const uint8_t* s_ntri) {
   uint64_t packed = 0;
 #pragma unroll
   for (int q = 0; q < kMaxThr; q++) {
     uint32_t cd = 0;
-    if (q < ts.n) {
-#pragma unroll
-      for (int n = 0; n < 8; n++) cd |= uint32_t(ts.thr[q] < v[n]) << n;
-    }
+    if (q < ts.n) cd = corner_code(ts.thr[q], v);
     code[q] = cd;
-    packed |= uint64_t(cell_ntri(cd)) << (16 * q);
+    packed |= uint64_t(s_ntri[cd]) << (16 * q);
   }
   return packed;
 }
@@ -241,6 +213,9 @@ __global__ __launch_bounds__(kThreads) void k_count(FieldDev f, ThrSet ts, uint3
   const uint32_t tile = xcd_tile(f.nblocks);
   if (tile >= f.nblocks) return;
   __shared__ uint64_t s_wave[kThreads / 64];
+  __shared__ uint8_t s_ntri[256];
+  s_ntri[threadIdx.x] = c_ntri[threadIdx.x];
+  __syncthreads();
   uint64_t acc = 0;
 #pragma unroll
   for (int chunk = 0; chunk < kCellsPerBlock / kThreads; chunk++) {
@@ -249,7 +224,7 @@ __global__ __launch_bounds__(kThreads) void k_count(FieldDev f, ThrSet ts, uint3
       double v[8];
       load_corners(f, locate(f, c), v);
       uint32_t code[kMaxThr];
-      acc += packed_counts(ts, v, code);
+      acc += packed_counts(ts, v, code, s_ntri);
     }
   }
   uint64_t tot;
@@ -356,11 +331,16 @@ __global__ __launch_bounds__(kThreads) void k_emit(FieldDev f, ThrSet ts, const 
   __shared__ double s_val[8][kThreads];
   __shared__ double s_grid[6][kThreads];  // x0 x1 y0 y1 z0 z1 of each cell (identity map)
   __shared__ uint8_t s_code[kMaxThr][kThreads];
-  __shared__ uint16_t s_q[kThreads * 12 * kMaxThr];
+  __shared__ uint16_t s_pref[kMaxThr][kThreads];  // exclusive prefix of the cells' counts
   __shared__ uint64_t s_wave[kThreads / 64];
+  __shared__ uint8_t s_ntri[256];
+  s_ntri[threadIdx.x] = c_ntri[threadIdx.x];
   uint64_t base[kMaxThr];
 #pragma unroll
   for (int q = 0; q < kMaxThr; q++) base[q] = q < ts.n ? start[ts.boff[q] + tile] : 0;
+  R* const px0 = m.pos;
+  const uint64_t cap = m.cap;
+  __syncthreads();
 
   for (int chunk = 0; chunk < kCellsPerBlock / kThreads; chunk++) {
     const uint32_t c0 = tile * kCellsPerBlock + chunk * kThreads;
@@ -386,53 +366,42 @@ __global__ __launch_bounds__(kThreads) void k_emit(FieldDev f, ThrSet ts, const 
 #pragma unroll
     for (int q = 0; q < 8; q++) s_val[q][threadIdx.x] = v[q];
     uint32_t code[kMaxThr];
-    const uint64_t packed = valid ? packed_counts(ts, v, code) : 0ull;
+    const uint64_t packed = valid ? packed_counts(ts, v, code, s_ntri) : 0ull;
     uint64_t tot;
     const uint64_t ex = block_excl_scan64(packed, s_wave, tot);
     uint32_t qoff[kMaxThr + 1];
     qoff[0] = 0;
 #pragma unroll
     for (int q = 0; q < kMaxThr; q++) {
-      const uint32_t tq = uint32_t(tot >> (16 * q)) & 0xffffu;
-      qoff[q + 1] = qoff[q] + tq;
-      if (q < ts.n) {
-        const uint32_t eq = uint32_t(ex >> (16 * q)) & 0xffffu;
-        if (valid) {
-          s_code[q][threadIdx.x] = uint8_t(code[q]);
-          uint32_t o = qoff[q] + eq;
-          const uint32_t cd = code[q];
-          if (uint32_t(packed >> (16 * q)) & 0xffffu) {
-            for (int t = 0; t < 6; t++) {
-              const uint32_t k = tet_ntri(tet_mask(cd, t));
-              for (uint32_t sl = 0; sl < k; sl++)
-                s_q[o++] = uint16_t(threadIdx.x | t << 8 | sl << 11 | q << 12);
-            }
-          }
-        }
-      }
+      qoff[q + 1] = qoff[q] + (uint32_t(tot >> (16 * q)) & 0xffffu);
+      s_pref[q][threadIdx.x] = uint16_t(ex >> (16 * q));
+      if (valid) s_code[q][threadIdx.x] = uint8_t(code[q]);
     }
     __syncthreads();
     const uint32_t total = qoff[kMaxThr];
     for (uint32_t it = threadIdx.x; it < total; it += kThreads) {
-      const uint32_t item = s_q[it];
-      const uint32_t l = item & 255u;
-      const int t = int(item >> 8) & 7;
-      const uint32_t sl = (item >> 11) & 1u;
-      const int q = int(item >> 12);
+      // threshold, then the last cell whose first triangle is at or before
+      // the item (cells without triangles share their successor's prefix)
+      int q = 0;
+#pragma unroll
+      for (int z = 1; z < kMaxThr; z++) q += it >= qoff[z] ? 1 : 0;
+      uint32_t qsel = 0;
+#pragma unroll
+      for (int z = 1; z < kMaxThr; z++) qsel = q == z ? qoff[z] : qsel;
+      const uint32_t r = it - qsel;
+      uint32_t l = 0;
+#pragma unroll
+      for (uint32_t step = kThreads / 2; step; step >>= 1) l += s_pref[q][l + step] <= r ? step : 0u;
+      const uint32_t sl = r - s_pref[q][l];
       const double thr = ts.thr[q];
       const uint32_t cc = c0 + l;
       const CellLoc L = locate(f, cc);
-      const uint32_t mt = tet_mask(s_code[q][l], t);
-      const bool rev = mt >= 8;
-      const uint32_t mm = rev ? 15u - mt : mt;
-      const uint32_t ent = c_tri[(t * 8 + mm) * 2 + sl];
+      const uint64_t ent = c_cslot[uint32_t(s_code[q][l]) * 12u + sl];
       V3<R> P[3];
-      uint32_t E = 0;
 #pragma unroll
       for (int vv = 0; vv < 3; vv++) {
-        const int qq = rev ? 2 - vv : vv;   // triangulate.c:108-112
-        const uint32_t tail = (ent >> (6 * qq)) & 7u;
-        const uint32_t head = (ent >> (6 * qq + 3)) & 7u;
+        const uint32_t tail = uint32_t(ent >> (6 * vv)) & 7u;
+        const uint32_t head = uint32_t(ent >> (6 * vv + 3)) & 7u;
         V3<R> pt, ph;
         if (f.cx) {
           pt = corner_pos<R>(f, L, tail);
@@ -442,7 +411,6 @@ __global__ __launch_bounds__(kThreads) void k_emit(FieldDev f, ThrSet ts, const 
           ph = {R(s_grid[head & 1u][l]), R(s_grid[2 + ((head >> 1) & 1u)][l]), R(s_grid[4 + (head >> 2)][l])};
         }
         P[vv] = interp<R>(thr, pt, s_val[tail][l], ph, s_val[head][l]);
-        E |= edge_id(tail, head) << (5 * vv);
       }
       // face normal and area (triangulate.c:75-93); the smoothing weight
       // area * unit normal is formed as normal.c:39-40 forms it
@@ -452,24 +420,22 @@ __global__ __launch_bounds__(kThreads) void k_emit(FieldDev f, ThrSet ts, const 
       const V3<R> w = vscale(area, fn);
       const bool in = L.i >= 1 && L.i + 2 <= f.ncx && L.j >= 1 && L.j + 2 <= f.ncy && L.k >= 1 &&
                       L.k + 2 <= f.ncz;
-      uint32_t qsel = 0;
-#pragma unroll
-      for (int z = 1; z < kMaxThr; z++) qsel = q == z ? qoff[z] : qsel;
       uint64_t bq = base[0];
 #pragma unroll
       for (int z = 1; z < kMaxThr; z++) bq = q == z ? base[z] : bq;
-      const uint64_t g = bq + (it - qsel);
-      const uint64_t cap = m.cap;
+      const uint64_t g = bq + r;
+      R* p = px0 + g;
 #pragma unroll
       for (int vv = 0; vv < 3; vv++) {
-        m.pos[(3 * vv + 0) * cap + g] = P[vv].x;
-        m.pos[(3 * vv + 1) * cap + g] = P[vv].y;
-        m.pos[(3 * vv + 2) * cap + g] = P[vv].z;
+        p[(3 * vv + 0) * cap] = P[vv].x;
+        p[(3 * vv + 1) * cap] = P[vv].y;
+        p[(3 * vv + 2) * cap] = P[vv].z;
       }
-      m.wfn[0 * cap + g] = w.x;
-      m.wfn[1 * cap + g] = w.y;
-      m.wfn[2 * cap + g] = w.z;
-      m.edges[g] = E | uint32_t(in) << 15;
+      R* wp = m.wfn + g;
+      wp[0] = w.x;
+      wp[cap] = w.y;
+      wp[2 * cap] = w.z;
+      m.edges[g] = uint32_t(ent >> 18) | uint32_t(in) << 15;
       m.cell[g] = cc;
     }
 #pragma unroll
@@ -860,21 +826,46 @@ hipError_t upload_tables(const NbrTable& nt) {
   };
   static const int tet[6][4] = {{0, 7, 3, 1}, {0, 7, 1, 5}, {0, 7, 5, 4},
                                 {0, 7, 4, 6}, {0, 7, 6, 2}, {0, 7, 2, 3}};
-  uint32_t h[6 * 8 * 2] = {};
-  for (int t = 0; t < 6; t++) {
-    for (int mm = 1; mm < 8; mm++) {
-      for (int s = 0; s < ntri[mm]; s++) {
-        uint32_t e = 0;
-        for (int q = 0; q < 3; q++) {
+  hipError_t e;
+  // per corner code: triangle count and every slot's corners and edge ids
+  static uint8_t nt8[256];
+  static uint64_t cs[256 * 12];
+  auto eid = [](uint32_t a, uint32_t b) -> uint32_t {
+    const uint32_t lo = a < b ? a : b;
+    switch (a ^ b) {
+      case 1: return lo >> 1;
+      case 2: return 4 + (lo & 1) + 2 * (lo >> 2);
+      case 4: return 8 + (lo & 3);
+      case 6: return 12 + (lo & 1);
+      case 5: return 14 + ((lo >> 1) & 1);
+      case 3: return 16 + (lo >> 2);
+      default: return 18;
+    }
+  };
+  for (int code = 0; code < 256; code++) {
+    int sl = 0;
+    for (int t = 0; t < 6; t++) {
+      int mk = 0;
+      for (int n = 0; n < 4; n++) mk |= ((code >> tet[t][n]) & 1) << n;
+      const bool rev = mk >= 8;
+      const int mm = rev ? 15 - mk : mk;
+      for (int s = 0; s < ntri[mm]; s++, sl++) {
+        uint64_t ent = 0;
+        for (int v = 0; v < 3; v++) {
+          const int q = rev ? 2 - v : v;
           const uint32_t tail = uint32_t(tet[t][pairs[mm][s][q][0]]);
           const uint32_t head = uint32_t(tet[t][pairs[mm][s][q][1]]);
-          e |= (tail | head << 3) << (6 * q);
+          ent |= uint64_t(tail | head << 3) << (6 * v);
+          ent |= uint64_t(eid(tail, head)) << (18 + 5 * v);
         }
-        h[(t * 8 + mm) * 2 + s] = e;
+        cs[code * 12 + sl] = ent;
       }
     }
+    nt8[code] = uint8_t(sl);
   }
-  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tri), h, sizeof(h));
+  e = hipMemcpyToSymbol(HIP_SYMBOL(c_ntri), nt8, sizeof(nt8));
+  if (e != hipSuccess) return e;
+  e = hipMemcpyToSymbol(HIP_SYMBOL(c_cslot), cs, sizeof(cs));
   if (e != hipSuccess) return e;
   return hipMemcpyToSymbol(HIP_SYMBOL(c_nbr), &nt, sizeof(nt));
 }
