@@ -31,20 +31,24 @@ W, H = 1920, 1080
 
 
 def algorithmic_bytes(stage, T, npts, npx, fp64):
-    """Irreducible bytes per frame for each stage: field read once shared by
-    all thresholds; per triangle 3 positions + face normal written by
-    tessellation, edge keys + weighted normal read and 3 vertex normals
-    written by smoothing, positions + vertex normals read by the raster,
-    plus the key init/read and RGB write per pixel."""
+    """Irreducible HBM bytes per frame of each stage.
+
+    count    the field, read once for all thresholds (8 B per lattice point)
+    emit     the field once + per triangle 3 positions and the weighted face
+             normal (96 B in FP64, 48 B in FP32): the tessellation bytes
+    raster   per triangle its 3 positions, plus the z-buffer keys written
+             once and read once per pixel
+    resolve  the keys read and the RGB bytes written per pixel (the vertex
+             normals of the nearest triangles are gathered here and add
+             O(pixels) bytes, not counted)
+    """
     r = 8 if fp64 else 4
     if stage == "count":
         return 8 * npts
     if stage == "emit":
-        return 8 * npts + (12 * r) * T
-    if stage == "smooth":
-        return (24 + 3 * r + 9 * r) * T
+        return 8 * npts + 12 * r * T
     if stage == "raster":
-        return (18 * r) * T + 16 * npx
+        return 9 * r * T + 16 * npx
     if stage == "resolve":
         return 8 * npx + 3 * npx
     return 0
@@ -96,10 +100,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--n", type=int, default=N, help="lattice points per axis (default 512)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 1 GiB device-to-device three times (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": x.numel() * 8, "copies": 3}))
+        return
 
     import torch
 
@@ -125,13 +141,15 @@ def main():
         T = ntri / max(1, args.steps)
         stages = {}
         for s in api.STAGES:
+            if s == "smooth":
+                continue  # vertex normals are gathered inside the resolve pass
             per = ms[s] / max(1, args.steps)
             b = algorithmic_bytes(s, T, npts, npx, prec == api.FP64)
             stages[s] = dict(ms=round(per, 4), gbs=round(b / (per * 1e-3) / 1e9, 1) if per > 0 and b else None)
         results[name] = dict(dt=dt, T=T, stages=stages, frame_ms=dt / args.steps * 1e3)
 
     r = results["fp64"]
-    dom = max(("count", "emit", "smooth", "raster", "resolve"), key=lambda s: r["stages"][s]["ms"])
+    dom = max(("count", "emit", "raster", "resolve"), key=lambda s: r["stages"][s]["ms"])
     dom_b = algorithmic_bytes(dom, r["T"], npts, npx, True)
     achieved = dom_b / (r["stages"][dom]["ms"] * 1e-3) / 1e9
     tess_b = algorithmic_bytes("emit", r["T"], npts, npx, True)

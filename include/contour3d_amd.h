@@ -124,14 +124,17 @@ extern int contour3d_amd_mesh_free(contour3d_amd_mesh_t * mesh);
 
 /* ---- profiling --------------------------------------------------------- */
 
-/* Stages timed with HIP events on the library stream when profiling is on. */
+/* Stages timed with HIP events on the library stream when profiling is on.
+ * A frame needs the smoothed vertex normals (normal.c) only of the triangle
+ * nearest to each pixel; they are gathered inside the RESOLVE stage, so the
+ * SMOOTH stage stays at zero for frames. */
 enum {
   CONTOUR3D_AMD_STAGE_COUNT = 0,   /* classify + per-tile triangle counts */
   CONTOUR3D_AMD_STAGE_SCAN = 1,    /* tile scan + totals */
   CONTOUR3D_AMD_STAGE_EMIT = 2,    /* triangle emission */
-  CONTOUR3D_AMD_STAGE_SMOOTH = 3,  /* vertex normals */
+  CONTOUR3D_AMD_STAGE_SMOOTH = 3,  /* vertex normals of every triangle (not used by frames) */
   CONTOUR3D_AMD_STAGE_RASTER = 4,  /* z-buffer passes incl. lines */
-  CONTOUR3D_AMD_STAGE_RESOLVE = 5, /* colour resolve + y-flip pack */
+  CONTOUR3D_AMD_STAGE_RESOLVE = 5, /* vertex normals of the nearest triangles, colour, y-flip pack */
   CONTOUR3D_AMD_NSTAGES = 6
 };
 
