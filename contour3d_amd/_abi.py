@@ -1,8 +1,9 @@
-"""ctypes mirrors of the public C structs in include/contour3d.h.
+"""ctypes mirrors of the public C structs in include/contour3d.h and
+include/contour3d_amd.h.
 
-The layouts follow the reference interface member for member
+The layouts of contour3d.h follow the reference interface member for member
 (include/contour3d.h:10-56); tests/test_abi.py checks sizes and offsets
-against the C compiler's view of the same header.
+against the C compiler's view of both headers.
 """
 import ctypes as C
 
@@ -49,6 +50,26 @@ class LineObj(C.Structure):
         ("converter", C.c_void_p),
         ("color", Color),
         ("width", C.c_double),
+    ]
+
+
+class Image(C.Structure):
+    """contour3d_amd_image_t (no reference counterpart): the arguments of
+    one contour3d_amd_execute_rgb call, one entry of a batch."""
+
+    _fields_ = [
+        ("camera_position", Vector),
+        ("light_direction", Vector),
+        ("screen_sizes", C.c_size_t * 2),
+        ("screen_center", Vector),
+        ("screen_local", Vector * 2),
+        ("bg_color", Color),
+        ("num_contours", C.c_size_t),
+        ("contour_objs", C.POINTER(ContourObj)),
+        ("num_lines", C.c_size_t),
+        ("line_objs", C.POINTER(LineObj)),
+        ("rgb", C.c_void_p),
+        ("depth", C.c_void_p),
     ]
 
 

@@ -29,6 +29,7 @@ EXPORTS = (
     "contour3d_amd_set_precision",
     "contour3d_amd_identity",
     "contour3d_amd_execute_rgb",
+    "contour3d_amd_execute_batch",
     "contour3d_amd_mesh_build",
     "contour3d_amd_mesh_count",
     "contour3d_amd_mesh_copy",
@@ -65,6 +66,8 @@ def lib():
         L.contour3d_amd_execute_rgb.argtypes = [V, V, C.POINTER(C.c_size_t), V, V, C.POINTER(_abi.Color),
                                                 C.c_size_t, C.POINTER(_abi.ContourObj), C.c_size_t,
                                                 C.POINTER(_abi.LineObj), C.c_void_p, C.c_void_p, C.c_int]
+        L.contour3d_amd_execute_batch.restype = C.c_int
+        L.contour3d_amd_execute_batch.argtypes = [C.c_size_t, C.POINTER(_abi.Image), C.c_int]
         L.contour3d_amd_init.argtypes = [C.c_int]
         L.contour3d_amd_set_precision.argtypes = [C.c_int]
         L.contour3d_amd_mesh_build.argtypes = [C.POINTER(_abi.ContourObj), C.c_size_t, C.c_int,
@@ -194,6 +197,55 @@ def execute_rgb(camera, light, size, center, local, bg, contours, lines=(), prec
     if rc != 0:
         raise RuntimeError("contour3d_amd_execute_rgb failed")
     return rgb, depth
+
+
+def execute_batch(images, precision=FP64):
+    """Render several images in one frame (contour3d_amd_execute_batch).
+
+    `images` is a list of dicts with the keyword arguments of execute_rgb
+    (camera, light, size, center, local, bg, contours, lines, rgb, depth,
+    want_depth).  Images whose `contours` list is the same Python object
+    share its contour structs, so the library sees equal lists and
+    tessellates them once.  Returns a list of (rgb, depth)."""
+    keep = _Keep()
+    arr = (_abi.Image * max(1, len(images)))()
+    out = []
+    made = {}
+    for n, im in enumerate(images):
+        W, H = int(im["size"][0]), int(im["size"][1])
+        cs = im.get("contours", ())
+        key = id(cs)
+        if key not in made:
+            made[key] = (contour_objs(cs, keep), len(cs))
+        objs, nc = made[key]
+        lines = im.get("lines", ())
+        lobjs = line_objs(lines, keep)
+        keep.items += [objs, lobjs]
+        rgb = im.get("rgb")
+        if rgb is None:
+            rgb = np.zeros((H, W, 3), np.uint8)
+        depth = im.get("depth")
+        if depth is None and im.get("want_depth"):
+            depth = np.zeros((H, W), np.float64)
+        e = arr[n]
+        e.camera_position = _abi.vec(im["camera"])
+        e.light_direction = _abi.vec(im["light"])
+        e.screen_sizes[:] = [W, H]
+        e.screen_center = _abi.vec(im["center"])
+        e.screen_local[0] = _abi.vec(im["local"][0])
+        e.screen_local[1] = _abi.vec(im["local"][1])
+        e.bg_color = _abi.color(im["bg"])
+        e.num_contours = nc
+        e.contour_objs = C.cast(objs, C.POINTER(_abi.ContourObj))
+        e.num_lines = len(lines)
+        e.line_objs = C.cast(lobjs, C.POINTER(_abi.LineObj))
+        e.rgb = _addr(rgb)
+        e.depth = None if depth is None else _addr(depth)
+        keep.items += [rgb, depth]
+        out.append((rgb, depth))
+    if lib().contour3d_amd_execute_batch(len(images), arr, int(precision)) != 0:
+        raise RuntimeError("contour3d_amd_execute_batch failed")
+    return out
 
 
 class Mesh:

@@ -140,6 +140,7 @@ struct contour3d_amd_mesh {
   c3d::DevBuf meta;    // ObjMeta per object
   uint64_t cap = 0;
   uint64_t total = 0;
+  uint64_t ntiles = 0;  // entries of bstart minus one
   std::vector<uint64_t> base, count;
   std::vector<contour3d_color_t> color;
   void release() {
@@ -166,17 +167,37 @@ struct Lib {
   uint64_t stamp = 0;
   std::vector<std::unique_ptr<Geom>> geoms;
   std::vector<std::unique_ptr<DevBuf>> uploads;
-  DevBuf cnt, start, tiles, gidx, gout;
-  DevBuf key, id, rgb, depth, fpix, fdep, frgb, frags, fcount, vntmp;
+  DevBuf cnt, tiles, tjobs, rjobs, hdr;
+  DevBuf key, id, rgb, depth, fpix, fdep, frgb, frags, vntmp;
   uint32_t frag_cap = 0;
-  HostBuf hgout, hrgb, hfcount;
+  HostBuf hrgb, hstage;
   // host sources of asynchronous uploads stay alive until the next call
-  std::vector<uint32_t> h_boff;
+  std::vector<TessJob> h_tjobs;
   std::vector<ObjMeta> h_meta;
+  std::vector<RasterJob<double>> h_rjobs64;
+  std::vector<RasterJob<float>> h_rjobs32;
+  uint32_t h_nline = 0;
   Frags h_frags;
+  // workgroups of the tessellation kernels: one per tile unless capped
+  uint32_t tess_cap = 0;
+  uint32_t tess_grid(uint32_t ntiles) const {
+    const uint32_t g = tess_cap && tess_cap < ntiles ? tess_cap : ntiles;
+    return g < 8 ? 8 : g;
+  }
+  template <typename R>
+  std::vector<RasterJob<R>>& rjobs_host();
   contour3d_amd_mesh frame_mesh;  // reused by contour3d_execute
   Prof prof;
 };
+
+template <>
+std::vector<RasterJob<double>>& Lib::rjobs_host<double>() {
+  return h_rjobs64;
+}
+template <>
+std::vector<RasterJob<float>>& Lib::rjobs_host<float>() {
+  return h_rjobs32;
+}
 
 Lib& lib() {
   static Lib L;
@@ -397,193 +418,6 @@ MeshView<R> view_of(const contour3d_amd_mesh& M) {
 }
 
 // ----------------------------------------------------------------------
-// mesh build: count -> scan -> emit -> smooth
-
-template <typename R>
-int build_mesh_t(Lib& L, contour3d_amd_mesh& M, const contour3d_contour_obj_t* objs, size_t n) {
-  std::vector<FieldDev> fd(n);
-  std::vector<int> group(n, -1);
-  std::vector<uint32_t>& boff = L.h_boff;
-  boff.assign(n + 1, 0);
-  std::vector<const double*> upl_src;
-  std::vector<void*> upl_dst;
-  // resolve fields and geometry
-  for (size_t o = 0; o < n; o++) {
-    const contour3d_contour_obj_t& ob = objs[o];
-    if (validate_obj(ob, o)) return 1;
-    FieldDev& f = fd[o];
-    f.nx = uint32_t(ob.glsizes[0]);
-    f.ny = uint32_t(ob.glsizes[1]);
-    f.nz = uint32_t(ob.glsizes[2]);
-    f.ncx = f.nx - 1;
-    f.ncy = f.ny - 1;
-    f.ncz = f.nz - 1;
-    f.ncells = f.ncx * f.ncy * f.ncz;
-    f.nblocks = (f.ncells + kCellsPerBlock - 1) / kCellsPerBlock;
-    f.div_ncx = make_fastdiv(f.ncx);
-    f.div_ncy = make_fastdiv(f.ncy);
-    if (is_device_ptr(ob.array)) {
-      f.f = ob.array;
-    } else {
-      const size_t bytes = size_t(f.nx) * f.ny * f.nz * sizeof(double);
-      void* dst = nullptr;
-      for (size_t q = 0; q < upl_src.size(); q++)
-        if (upl_src[q] == ob.array) dst = upl_dst[q];
-      if (!dst) {
-        if (L.uploads.size() <= upl_src.size()) L.uploads.push_back(std::make_unique<DevBuf>());
-        DevBuf& b = *L.uploads[upl_src.size()];
-        C3D_HIP(b.ensure(bytes));
-        C3D_HIP(hipMemcpyAsync(b.p, ob.array, bytes, hipMemcpyHostToDevice, L.st));
-        upl_src.push_back(ob.array);
-        upl_dst.push_back(b.p);
-        dst = b.p;
-      }
-      f.f = static_cast<const double*>(dst);
-    }
-    Geom* g = nullptr;
-    if (get_geometry(L, ob, &g)) return 1;
-    f.gx = g->grids.as<double>();
-    f.gy = f.gx + f.nx;
-    f.gz = f.gy + f.ny;
-    if (g->identity) {
-      f.cx = f.cy = f.cz = nullptr;
-    } else {
-      const size_t npts = size_t(f.nx) * f.ny * f.nz;
-      f.cx = g->table.as<double>();
-      f.cy = f.cx + npts;
-      f.cz = f.cy + npts;
-    }
-    boff[o + 1] = boff[o] + f.nblocks;
-    for (size_t q = 0; q < o && group[o] < 0; q++) {
-      if (fd[q].f == f.f && fd[q].nx == f.nx && fd[q].ny == f.ny && fd[q].nz == f.nz && fd[q].gx == f.gx &&
-          fd[q].cx == f.cx)
-        group[o] = group[q];
-    }
-    if (group[o] < 0) group[o] = int(o);
-  }
-  const uint64_t nbt = boff[n];
-  M.precision = sizeof(R) == 8 ? CONTOUR3D_AMD_FP64 : CONTOUR3D_AMD_FP32;
-  M.base.assign(n, 0);
-  M.count.assign(n, 0);
-  M.color.resize(n);
-  for (size_t o = 0; o < n; o++) M.color[o] = objs[o].color;
-  M.total = 0;
-  if (n == 0) return 0;
-
-  const uint64_t ntile = (nbt + 1023) / 1024;
-  C3D_HIP(L.cnt.ensure(nbt * sizeof(uint32_t)));
-  C3D_HIP(L.start.ensure((nbt + 1) * sizeof(uint64_t)));
-  C3D_HIP(L.tiles.ensure((ntile + 1) * sizeof(uint64_t)));
-  C3D_HIP(L.gidx.ensure((n + 1) * sizeof(uint32_t)));
-  C3D_HIP(L.gout.ensure((n + 1) * sizeof(uint64_t)));
-  C3D_HIP(L.hgout.ensure((n + 1) * sizeof(uint64_t)));
-
-  // thresholds of objects sharing a field are classified in one read
-  auto for_each_thrset = [&](auto&& fn) -> int {
-    for (size_t o = 0; o < n; o++) {
-      if (group[o] != int(o)) continue;
-      ThrSet ts{};
-      ts.n = 0;
-      for (size_t q = o; q < n; q++) {
-        if (group[q] != int(o)) continue;
-        ts.thr[ts.n] = objs[q].threshold;
-        ts.boff[ts.n] = boff[q];
-        ts.n++;
-        if (ts.n == kMaxThr) {
-          if (fn(fd[o], ts)) return 1;
-          ts.n = 0;
-        }
-      }
-      if (ts.n && fn(fd[o], ts)) return 1;
-    }
-    return 0;
-  };
-
-  if (prof_begin(L, CONTOUR3D_AMD_STAGE_COUNT)) return 1;
-  if (for_each_thrset([&](const FieldDev& f, const ThrSet& ts) -> int {
-        C3D_HIP(launch_count(f, ts, L.cnt.as<uint32_t>(), L.st));
-        return 0;
-      }))
-    return 1;
-  if (prof_end(L)) return 1;
-
-  if (prof_begin(L, CONTOUR3D_AMD_STAGE_SCAN)) return 1;
-  C3D_HIP(launch_scan(L.cnt.as<uint32_t>(), L.start.as<uint64_t>(), nbt, L.tiles.as<uint64_t>(), L.st));
-  C3D_HIP(hipMemcpyAsync(L.gidx.p, boff.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, L.st));
-  C3D_HIP(launch_gather_u64(L.start.as<uint64_t>(), L.gidx.as<uint32_t>(), L.gout.as<uint64_t>(), int(n + 1),
-                            L.st));
-  C3D_HIP(hipMemcpyAsync(L.hgout.p, L.gout.p, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, L.st));
-  if (prof_end(L)) return 1;
-  C3D_HIP(hipStreamSynchronize(L.st));
-  const uint64_t* hb = static_cast<const uint64_t*>(L.hgout.p);
-  for (size_t o = 0; o < n; o++) {
-    M.base[o] = hb[o];
-    M.count[o] = hb[o + 1] - hb[o];
-  }
-  M.total = hb[n];
-  L.prof.ntri += M.total;
-  if (M.total >= (uint64_t(1) << 32) - (uint64_t(1) << 24)) {
-    log_error("%llu triangles exceed the 2^32 triangle ids of one frame", (unsigned long long)M.total);
-    return 1;
-  }
-  if (M.cap < M.total || M.cap == 0 || M.precision_built != M.precision) {
-    M.precision_built = M.precision;
-    const uint64_t cap = M.total + M.total / 4 + 1024;
-    M.pos.release();
-    M.wfn.release();
-    M.cell.release();
-    M.edges.release();
-    M.cap = 0;
-    C3D_HIP(M.pos.ensure(9 * cap * sizeof(R)));
-    C3D_HIP(M.wfn.ensure(3 * cap * sizeof(R)));
-    C3D_HIP(M.cell.ensure(cap * sizeof(uint32_t)));
-    C3D_HIP(M.edges.ensure(cap * sizeof(uint32_t)));
-    M.cap = cap;
-  }
-  const MeshView<R> mv = view_of<R>(M);
-
-  if (prof_begin(L, CONTOUR3D_AMD_STAGE_EMIT)) return 1;
-  if (for_each_thrset([&](const FieldDev& f, const ThrSet& ts) -> int {
-        C3D_HIP(launch_emit<R>(f, ts, L.start.as<uint64_t>(), mv, L.st));
-        return 0;
-      }))
-    return 1;
-  if (prof_end(L)) return 1;
-
-  // the mesh keeps the tile starts and per-object metadata that the
-  // on-demand vertex-normal gather reads
-  C3D_HIP(M.bstart.ensure((nbt + 1) * sizeof(uint64_t)));
-  C3D_HIP(hipMemcpyAsync(M.bstart.p, L.start.p, (nbt + 1) * sizeof(uint64_t), hipMemcpyDeviceToDevice, L.st));
-  std::vector<ObjMeta>& hm = L.h_meta;
-  hm.assign(n, ObjMeta{});
-  for (size_t o = 0; o < n; o++) {
-    hm[o].base = M.base[o];
-    hm[o].count = M.count[o];
-    hm[o].bstart = M.bstart.as<uint64_t>() + boff[o];
-    hm[o].ncx = fd[o].ncx;
-    hm[o].ncy = fd[o].ncy;
-    hm[o].ncz = fd[o].ncz;
-    hm[o].rgb[0] = objs[o].color.r;
-    hm[o].rgb[1] = objs[o].color.g;
-    hm[o].rgb[2] = objs[o].color.b;
-  }
-  C3D_HIP(M.meta.ensure(n * sizeof(ObjMeta)));
-  C3D_HIP(hipMemcpyAsync(M.meta.p, hm.data(), n * sizeof(ObjMeta), hipMemcpyHostToDevice, L.st));
-  return 0;
-}
-
-int build_mesh(Lib& L, contour3d_amd_mesh& M, const contour3d_contour_obj_t* objs, size_t n, int precision) {
-  if (n && !objs) {
-    log_error("contour objects pointer is NULL");
-    return 1;
-  }
-  if (precision == CONTOUR3D_AMD_FP64) return build_mesh_t<double>(L, M, objs, n);
-  if (precision == CONTOUR3D_AMD_FP32) return build_mesh_t<float>(L, M, objs, n);
-  log_error("unknown precision %d", precision);
-  return 1;
-}
-
-// ----------------------------------------------------------------------
 // view and lines (host FP64, exactly as the reference forms them)
 
 using V = contour3d_vector_t;
@@ -716,110 +550,480 @@ int line_fragments(const HostView& v, size_t nlines, const contour3d_line_obj_t*
 }
 
 // ----------------------------------------------------------------------
-// render: z-buffer passes, lines, resolve
+// frames
+//
+// A frame is a list of images.  Every contour object of the frame is
+// tessellated by one count, scan, plan and emit sequence over a device table
+// of field passes; every image is then rasterised and resolved by one
+// launch per stage over a device table of images.  An image whose object
+// list equals an earlier image's (same fields, lattices, converters,
+// thresholds and colours) reuses that image's triangles.
+
+struct Image {
+  HostView hv;
+  size_t nobj = 0;
+  const contour3d_contour_obj_t* objs = nullptr;
+  size_t nlines = 0;
+  const contour3d_line_obj_t* lines = nullptr;
+  uint8_t* rgb = nullptr;  // caller's buffer
+  double* depth = nullptr; // caller's buffer or null
+  bool rgb_dev = false, dep_dev = false;
+  uint32_t obj0 = 0, obj1 = 0;
+  uint64_t px0 = 0;
+};
+
+bool same_obj(const contour3d_contour_obj_t& a, const contour3d_contour_obj_t& b) {
+  return a.array == b.array && a.converter == b.converter && a.pencil == b.pencil &&
+         std::memcmp(a.glsizes, b.glsizes, sizeof(a.glsizes)) == 0 &&
+         std::memcmp(a.grids, b.grids, sizeof(a.grids)) == 0 &&
+         std::memcmp(&a.threshold, &b.threshold, sizeof(double)) == 0 && a.color.r == b.color.r &&
+         a.color.g == b.color.g && a.color.b == b.color.b;
+}
 
 template <typename R>
-int render_t(Lib& L, const contour3d_amd_mesh& M, const HostView& hv, const Frags& F, uint8_t* rgb_dev,
-             double* depth_dev) {
-  const uint64_t npx = uint64_t(hv.W) * uint64_t(hv.H);
-  const bool fp64 = sizeof(R) == 8;
-  C3D_HIP(L.key.ensure(npx * sizeof(uint64_t)));
-  if (fp64) C3D_HIP(L.id.ensure(npx * sizeof(uint32_t)));
-  uint32_t* id = fp64 ? L.id.as<uint32_t>() : nullptr;
-  const uint32_t nf = uint32_t(F.pix.size());
-  if (nf) {
-    C3D_HIP(L.fpix.ensure(nf * sizeof(uint32_t)));
-    C3D_HIP(L.fdep.ensure(nf * sizeof(double)));
-    C3D_HIP(L.frgb.ensure(3 * size_t(nf)));
-    C3D_HIP(hipMemcpyAsync(L.fpix.p, F.pix.data(), nf * sizeof(uint32_t), hipMemcpyHostToDevice, L.st));
-    C3D_HIP(hipMemcpyAsync(L.fdep.p, F.depth.data(), nf * sizeof(double), hipMemcpyHostToDevice, L.st));
-    C3D_HIP(hipMemcpyAsync(L.frgb.p, F.rgb.data(), 3 * size_t(nf), hipMemcpyHostToDevice, L.st));
-  }
-  const size_t nobj = M.base.size();
-  const ViewDev<R> v = dev_view<R>(hv);
-  const MeshView<R> mv = view_of<R>(M);
-  const uint32_t id_base = uint32_t(M.total + 1);
-  if (prof_begin(L, CONTOUR3D_AMD_STAGE_RASTER)) return 1;
-  C3D_HIP(launch_zinit(L.key.as<uint64_t>(), id, npx, fp64 ? 0 : 1, L.st));
-  if constexpr (sizeof(R) == 8) {
-    // depth pass recording covered fragments, then the id pass over that
-    // list; a list that outgrew its buffer falls back to a second pass over
-    // the geometry and grows the buffer for the next frame
-    if (L.frag_cap == 0) L.frag_cap = 1u << 22;
-    C3D_HIP(L.frags.ensure(size_t(L.frag_cap) * sizeof(Frag)));
-    C3D_HIP(L.fcount.ensure(2 * sizeof(uint32_t)));
-    C3D_HIP(L.hfcount.ensure(2 * sizeof(uint32_t)));
-    C3D_HIP(launch_fill_u32(L.fcount.as<uint32_t>(), 0u, 2, L.st));
-    C3D_HIP(launch_raster_frags(mv, M.total, v, L.key.as<uint64_t>(), L.frags.as<Frag>(), L.frag_cap,
-                                L.fcount.as<uint32_t>(), L.st));
-    C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, id_base, 0, 0,
-                              L.key.as<uint64_t>(), id, L.st));
-    C3D_HIP(hipMemcpyAsync(L.hfcount.p, L.fcount.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, L.st));
-    C3D_HIP(hipStreamSynchronize(L.st));
-    const uint32_t nfr = static_cast<const uint32_t*>(L.hfcount.p)[0];
-    const uint32_t overflow = static_cast<const uint32_t*>(L.hfcount.p)[1];
-    if (!overflow && nfr <= L.frag_cap) {
-      C3D_HIP(launch_frag_ids(L.frags.as<Frag>(), nfr, L.key.as<uint64_t>(), id, L.st));
+int ensure_capacity(contour3d_amd_mesh& M) {
+  if (M.cap >= M.total && M.cap != 0 && M.precision_built == M.precision) return 0;
+  M.precision_built = M.precision;
+  const uint64_t cap = M.total + M.total / 4 + 1024;
+  M.pos.release();
+  M.wfn.release();
+  M.cell.release();
+  M.edges.release();
+  M.cap = 0;
+  C3D_HIP(M.pos.ensure(9 * cap * sizeof(R)));
+  C3D_HIP(M.wfn.ensure(3 * cap * sizeof(R)));
+  C3D_HIP(M.cell.ensure(cap * sizeof(uint32_t)));
+  C3D_HIP(M.edges.ensure(cap * sizeof(uint32_t)));
+  M.cap = cap;
+  return 0;
+}
+
+// Count, scan, plan and emit every object of `objs` into M.  `rjobs`
+// (device, njobs entries) receive their triangle ranges and work offsets
+// from the plan kernel.  Synchronises once, after the plan, to size the
+// mesh; on return the emission is enqueued.
+template <typename R>
+int tessellate(Lib& L, contour3d_amd_mesh& M, const std::vector<const contour3d_contour_obj_t*>& objs,
+               RasterJob<R>* rjobs, int njobs, FrameHdr& hh) {
+  const size_t n = objs.size();
+  std::vector<FieldDev> fd(n);
+  std::vector<int> group(n, -1);
+  std::vector<uint32_t> boff(n + 1, 0);
+  std::vector<const double*> upl_src;
+  std::vector<void*> upl_dst;
+  for (size_t o = 0; o < n; o++) {
+    const contour3d_contour_obj_t& ob = *objs[o];
+    if (validate_obj(ob, o)) return 1;
+    FieldDev& f = fd[o];
+    f.nx = uint32_t(ob.glsizes[0]);
+    f.ny = uint32_t(ob.glsizes[1]);
+    f.nz = uint32_t(ob.glsizes[2]);
+    f.ncx = f.nx - 1;
+    f.ncy = f.ny - 1;
+    f.ncz = f.nz - 1;
+    f.ncells = f.ncx * f.ncy * f.ncz;
+    f.nblocks = (f.ncells + kCellsPerBlock - 1) / kCellsPerBlock;
+    f.div_ncx = make_fastdiv(f.ncx);
+    f.div_ncy = make_fastdiv(f.ncy);
+    if (is_device_ptr(ob.array)) {
+      f.f = ob.array;
     } else {
-      C3D_HIP(launch_raster<R>(mv, M.total, v, 1, L.key.as<uint64_t>(), id, L.st));
-      if (nfr > L.frag_cap) {
-        const uint64_t grow = uint64_t(nfr) + nfr / 4;
-        L.frag_cap = uint32_t(grow > 0xffffffffull ? 0xffffffffull : grow);
+      const size_t bytes = size_t(f.nx) * f.ny * f.nz * sizeof(double);
+      void* dst = nullptr;
+      for (size_t q = 0; q < upl_src.size(); q++)
+        if (upl_src[q] == ob.array) dst = upl_dst[q];
+      if (!dst) {
+        if (L.uploads.size() <= upl_src.size()) L.uploads.push_back(std::make_unique<DevBuf>());
+        DevBuf& b = *L.uploads[upl_src.size()];
+        C3D_HIP(b.ensure(bytes));
+        C3D_HIP(hipMemcpyAsync(b.p, ob.array, bytes, hipMemcpyHostToDevice, L.st));
+        upl_src.push_back(ob.array);
+        upl_dst.push_back(b.p);
+        dst = b.p;
+      }
+      f.f = static_cast<const double*>(dst);
+    }
+    Geom* g = nullptr;
+    if (get_geometry(L, ob, &g)) return 1;
+    f.gx = g->grids.as<double>();
+    f.gy = f.gx + f.nx;
+    f.gz = f.gy + f.ny;
+    if (g->identity) {
+      f.cx = f.cy = f.cz = nullptr;
+    } else {
+      const size_t npts = size_t(f.nx) * f.ny * f.nz;
+      f.cx = g->table.as<double>();
+      f.cy = f.cx + npts;
+      f.cz = f.cy + npts;
+    }
+    if (uint64_t(boff[o]) + f.nblocks >= (uint64_t(1) << 31)) {
+      log_error("frame exceeds 2^31 tessellation tiles");
+      return 1;
+    }
+    boff[o + 1] = boff[o] + f.nblocks;
+    for (size_t q = 0; q < o && group[o] < 0; q++) {
+      if (fd[q].f == f.f && fd[q].nx == f.nx && fd[q].ny == f.ny && fd[q].nz == f.nz && fd[q].gx == f.gx &&
+          fd[q].cx == f.cx)
+        group[o] = group[q];
+    }
+    if (group[o] < 0) group[o] = int(o);
+  }
+  const uint64_t nbt = boff[n];
+  M.precision = sizeof(R) == 8 ? CONTOUR3D_AMD_FP64 : CONTOUR3D_AMD_FP32;
+  M.base.assign(n, 0);
+  M.count.assign(n, 0);
+  M.color.resize(n);
+  for (size_t o = 0; o < n; o++) M.color[o] = objs[o]->color;
+  M.ntiles = nbt;
+
+  // field passes: thresholds of objects sharing a field are classified in
+  // one read of it, up to kMaxThr at a time
+  std::vector<TessJob>& tj = L.h_tjobs;
+  tj.clear();
+  uint64_t jt = 0;
+  for (size_t o = 0; o < n; o++) {
+    if (group[o] != int(o)) continue;
+    TessJob job{};
+    job.f = fd[o];
+    for (size_t q = o; q < n; q++) {
+      if (group[q] != int(o)) continue;
+      job.ts.thr[job.ts.n] = objs[q]->threshold;
+      job.ts.boff[job.ts.n] = boff[q];
+      job.ts.n++;
+      if (job.ts.n == kMaxThr) {
+        job.tile0 = uint32_t(jt);
+        jt += fd[o].nblocks;
+        tj.push_back(job);
+        job.ts = ThrSet{};
       }
     }
-    C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, id_base, 1, 0,
-                              L.key.as<uint64_t>(), id, L.st));
-  } else {
-    C3D_HIP(launch_raster<R>(mv, M.total, v, 0, L.key.as<uint64_t>(), id, L.st));
-    C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, id_base, 0, 1,
-                              L.key.as<uint64_t>(), id, L.st));
+    if (job.ts.n) {
+      job.tile0 = uint32_t(jt);
+      jt += fd[o].nblocks;
+      tj.push_back(job);
+    }
   }
+  if (jt >= (uint64_t(1) << 31)) {
+    log_error("frame exceeds 2^31 tessellation tiles");
+    return 1;
+  }
+  const uint32_t ntj = uint32_t(jt);
+
+  C3D_HIP(L.cnt.ensure((nbt + 1) * sizeof(uint32_t)));
+  C3D_HIP(M.bstart.ensure((nbt + 1) * sizeof(uint64_t)));
+  C3D_HIP(L.tiles.ensure(((nbt + kScanTile - 1) / kScanTile + 1) * sizeof(uint64_t)));
+  C3D_HIP(L.tjobs.ensure((tj.size() + 1) * sizeof(TessJob)));
+  C3D_HIP(M.meta.ensure((n + 1) * sizeof(ObjMeta)));
+  C3D_HIP(L.hdr.ensure(sizeof(FrameHdr)));
+  C3D_HIP(L.hstage.ensure(sizeof(FrameHdr) + (n + 1) * sizeof(ObjMeta)));
+  std::vector<ObjMeta>& hm = L.h_meta;
+  hm.assign(n, ObjMeta{});
+  for (size_t o = 0; o < n; o++) {
+    hm[o].bstart = M.bstart.as<uint64_t>() + boff[o];
+    hm[o].ncx = fd[o].ncx;
+    hm[o].ncy = fd[o].ncy;
+    hm[o].ncz = fd[o].ncz;
+    hm[o].tile_off = boff[o];
+    hm[o].rgb[0] = objs[o]->color.r;
+    hm[o].rgb[1] = objs[o]->color.g;
+    hm[o].rgb[2] = objs[o]->color.b;
+  }
+  if (n) {
+    C3D_HIP(hipMemcpyAsync(M.meta.p, hm.data(), n * sizeof(ObjMeta), hipMemcpyHostToDevice, L.st));
+    C3D_HIP(hipMemcpyAsync(L.tjobs.p, tj.data(), tj.size() * sizeof(TessJob), hipMemcpyHostToDevice, L.st));
+  }
+  const TessJob* tjd = L.tjobs.as<TessJob>();
+  FrameHdr* hdr = L.hdr.as<FrameHdr>();
+
+  if (prof_begin(L, CONTOUR3D_AMD_STAGE_COUNT)) return 1;
+  C3D_HIP(launch_count(tjd, int(tj.size()), ntj, L.cnt.as<uint32_t>(), L.tess_grid(ntj), L.st));
   if (prof_end(L)) return 1;
-  if (prof_begin(L, CONTOUR3D_AMD_STAGE_RESOLVE)) return 1;
-  const uint8_t bg[3] = {hv.bg.r, hv.bg.g, hv.bg.b};
-  C3D_HIP(launch_resolve<R>(mv, M.total, v, M.meta.as<ObjMeta>(), int(nobj), L.frgb.as<uint8_t>(), bg,
-                            L.key.as<uint64_t>(), id, rgb_dev, depth_dev, L.st));
+  if (prof_begin(L, CONTOUR3D_AMD_STAGE_SCAN)) return 1;
+  if (nbt) C3D_HIP(launch_scan(L.cnt.as<uint32_t>(), M.bstart.as<uint64_t>(), nbt, L.tiles.as<uint64_t>(), L.st));
+  C3D_HIP(launch_plan<R>(M.bstart.as<uint64_t>(), nbt, M.meta.as<ObjMeta>(), int(n), rjobs, njobs, hdr,
+                         ~uint64_t(0), L.st));
+  unsigned char* hs = static_cast<unsigned char*>(L.hstage.p);
+  C3D_HIP(hipMemcpyAsync(hs, hdr, sizeof(FrameHdr), hipMemcpyDeviceToHost, L.st));
+  if (n)
+    C3D_HIP(hipMemcpyAsync(hs + sizeof(FrameHdr), M.meta.p, n * sizeof(ObjMeta), hipMemcpyDeviceToHost, L.st));
+  if (prof_end(L)) return 1;
+  C3D_HIP(hipStreamSynchronize(L.st));
+  std::memcpy(&hh, hs, sizeof(FrameHdr));
+  for (size_t o = 0; o < n; o++) {
+    ObjMeta om;
+    std::memcpy(&om, hs + sizeof(FrameHdr) + o * sizeof(ObjMeta), sizeof(ObjMeta));
+    M.base[o] = om.base;
+    M.count[o] = om.count;
+  }
+  M.total = hh.ntri;
+  L.prof.ntri += M.total;
+  if (M.total >= (uint64_t(1) << 32) - (uint64_t(1) << 24)) {
+    log_error("%llu triangles exceed the 2^32 triangle ids of one frame", (unsigned long long)M.total);
+    return 1;
+  }
+  if (ensure_capacity<R>(M)) return 1;
+  if (prof_begin(L, CONTOUR3D_AMD_STAGE_EMIT)) return 1;
+  C3D_HIP(launch_emit<R>(tjd, int(tj.size()), ntj, M.bstart.as<uint64_t>(), view_of<R>(M), hdr, L.tess_grid(ntj),
+                         L.st));
   if (prof_end(L)) return 1;
   return 0;
 }
 
-int render(Lib& L, const contour3d_amd_mesh& M, const HostView& hv, size_t nlines,
-           const contour3d_line_obj_t* lines, uint8_t* rgb, double* depth, bool rgb_host_only) {
-  if (hv.W == 0 || hv.H == 0 || hv.W > (1u << 16) || hv.H > (1u << 16) ||
-      uint64_t(hv.W) * uint64_t(hv.H) >= (uint64_t(1) << 31)) {
-    log_error("screen size %zu x %zu out of range", hv.W, hv.H);
-    return 1;
-  }
-  if (nlines && !lines) {
-    log_error("line objects pointer is NULL");
-    return 1;
-  }
+// Line fragments of every image, appended to L.h_frags with frame pixel
+// indices.
+int frame_lines(Lib& L, const std::vector<Image>& imgs) {
   Frags& F = L.h_frags;
   F.pix.clear();
   F.depth.clear();
   F.rgb.clear();
-  if (line_fragments(hv, nlines, lines, F)) return 1;
-  const uint64_t npx = uint64_t(hv.W) * uint64_t(hv.H);
-  const bool rgb_dev = !rgb_host_only && is_device_ptr(rgb);
-  const bool dep_dev = depth && is_device_ptr(depth);
-  uint8_t* rgb_target = rgb;
-  if (!rgb_dev) {
-    C3D_HIP(L.rgb.ensure(npx * 3));
-    rgb_target = L.rgb.as<uint8_t>();
+  for (const Image& im : imgs) {
+    if (im.nlines && !im.lines) {
+      log_error("line objects pointer is NULL");
+      return 1;
+    }
+    const size_t f0 = F.pix.size();
+    if (line_fragments(im.hv, im.nlines, im.lines, F)) return 1;
+    for (size_t q = f0; q < F.pix.size(); q++) F.pix[q] += uint32_t(im.px0);
   }
-  double* dep_target = depth;
-  if (depth && !dep_dev) {
-    C3D_HIP(L.depth.ensure(npx * sizeof(double)));
-    dep_target = L.depth.as<double>();
+  if (F.pix.size() >= (size_t(1) << 31)) {
+    log_error("too many line fragments in one frame");
+    return 1;
   }
-  int rc = M.precision == CONTOUR3D_AMD_FP64 ? render_t<double>(L, M, hv, F, rgb_target, dep_target)
-                                             : render_t<float>(L, M, hv, F, rgb_target, dep_target);
-  if (rc) return rc;
-  if (!rgb_dev) C3D_HIP(hipMemcpyAsync(rgb, rgb_target, npx * 3, hipMemcpyDeviceToHost, L.st));
-  if (depth && !dep_dev)
-    C3D_HIP(hipMemcpyAsync(depth, dep_target, npx * sizeof(double), hipMemcpyDeviceToHost, L.st));
+  return 0;
+}
+
+// z-buffer passes, lines and resolve of every image of a frame over the
+// mesh M; then the host copies and one synchronisation.
+template <typename R>
+int raster_frame(Lib& L, const contour3d_amd_mesh& M, std::vector<Image>& imgs, const RasterJob<R>* rjobs,
+                 uint64_t npx, const FrameHdr& hh) {
+  const bool fp64 = sizeof(R) == 8;
+  const int njobs = int(imgs.size());
+  const Frags& F = L.h_frags;
+  const uint32_t nf = uint32_t(F.pix.size());
+  FrameHdr* hdr = L.hdr.as<FrameHdr>();
+  C3D_HIP(L.key.ensure(npx * sizeof(uint64_t)));
+  if (fp64) C3D_HIP(L.id.ensure(npx * sizeof(uint32_t)));
+  uint32_t* id = fp64 ? L.id.as<uint32_t>() : nullptr;
+  uint64_t* key = L.key.as<uint64_t>();
+  C3D_HIP(L.fpix.ensure((nf + 1) * sizeof(uint32_t)));
+  C3D_HIP(L.fdep.ensure((nf + 1) * sizeof(double)));
+  C3D_HIP(L.frgb.ensure(3 * size_t(nf + 1)));
+  if (nf) {
+    C3D_HIP(hipMemcpyAsync(L.fpix.p, F.pix.data(), nf * sizeof(uint32_t), hipMemcpyHostToDevice, L.st));
+    C3D_HIP(hipMemcpyAsync(L.fdep.p, F.depth.data(), nf * sizeof(double), hipMemcpyHostToDevice, L.st));
+    C3D_HIP(hipMemcpyAsync(L.frgb.p, F.rgb.data(), 3 * size_t(nf), hipMemcpyHostToDevice, L.st));
+  }
+  L.h_nline = nf;
+  C3D_HIP(hipMemcpyAsync(reinterpret_cast<unsigned char*>(hdr) + offsetof(FrameHdr, nline), &L.h_nline,
+                         sizeof(uint32_t), hipMemcpyHostToDevice, L.st));
+  const MeshView<R> mv = view_of<R>(M);
+  const uint32_t rgrid = uint32_t(std::min<uint64_t>((hh.nwork + kThreads - 1) / kThreads + 8, 1u << 30)) / 8 * 8;
+  if (prof_begin(L, CONTOUR3D_AMD_STAGE_RASTER)) return 1;
+  C3D_HIP(launch_zinit(key, id, npx, fp64 ? 0 : 1, hdr, L.st));
+  if constexpr (sizeof(R) == 8) {
+    // depth pass keeping the fragments that may end up nearest, then the id
+    // pass over that list; when the list overflows, the geometry id pass
+    // runs instead (both decide on the device) and the list grows for the
+    // next frame
+    if (L.frag_cap == 0) L.frag_cap = 1u << 22;
+    C3D_HIP(L.frags.ensure(size_t(L.frag_cap) * sizeof(Frag)));
+    C3D_HIP(launch_raster_frags(mv, rjobs, njobs, hdr, key, L.frags.as<Frag>(), L.frag_cap, rgrid, L.st));
+    C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, hdr, 0, 0, key, id, L.st));
+    C3D_HIP(launch_frag_ids(L.frags.as<Frag>(), L.frag_cap, hdr, key, id,
+                            std::min<uint32_t>((L.frag_cap + 255) / 256, 4096), L.st));
+    C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, 1, key, id, uint32_t(persistent_blocks()), L.st));
+    C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, hdr, 1, 0, key, id, L.st));
+  } else {
+    C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, 0, key, id, rgrid, L.st));
+    C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, hdr, 0, 1, key, id, L.st));
+  }
+  if (prof_end(L)) return 1;
+  if (prof_begin(L, CONTOUR3D_AMD_STAGE_RESOLVE)) return 1;
+  C3D_HIP(launch_resolve<R>(mv, rjobs, njobs, npx, M.meta.as<ObjMeta>(), int(M.base.size()), L.frgb.as<uint8_t>(),
+                            hdr, key, id, 1u << 30, L.st));
+  if (prof_end(L)) return 1;
+  for (const Image& im : imgs) {
+    const uint64_t n = uint64_t(im.hv.W) * uint64_t(im.hv.H);
+    if (!im.rgb_dev)
+      C3D_HIP(hipMemcpyAsync(im.rgb, L.rgb.as<uint8_t>() + 3 * im.px0, 3 * n, hipMemcpyDeviceToHost, L.st));
+    if (im.depth && !im.dep_dev)
+      C3D_HIP(hipMemcpyAsync(im.depth, L.depth.as<double>() + im.px0, n * sizeof(double), hipMemcpyDeviceToHost,
+                             L.st));
+  }
+  C3D_HIP(hipMemcpyAsync(L.hstage.p, hdr, sizeof(FrameHdr), hipMemcpyDeviceToHost, L.st));
   C3D_HIP(hipStreamSynchronize(L.st));
+  FrameHdr fin;
+  std::memcpy(&fin, L.hstage.p, sizeof(FrameHdr));
+  if (fp64 && (fin.overflow & 2u) && fin.nfrag > L.frag_cap) {
+    const uint64_t grow = uint64_t(fin.nfrag) + fin.nfrag / 4;
+    L.frag_cap = uint32_t(grow > 0xffffffffull ? 0xffffffffull : grow);
+  }
   return prof_collect(L);
+}
+
+template <typename R>
+int build_mesh_t(Lib& L, contour3d_amd_mesh& M, const contour3d_contour_obj_t* objs, size_t n) {
+  std::vector<const contour3d_contour_obj_t*> v(n);
+  for (size_t o = 0; o < n; o++) v[o] = objs + o;
+  FrameHdr hh{};
+  return tessellate<R>(L, M, v, nullptr, 0, hh);
+}
+
+int build_mesh(Lib& L, contour3d_amd_mesh& M, const contour3d_contour_obj_t* objs, size_t n, int precision) {
+  if (n && !objs) {
+    log_error("contour objects pointer is NULL");
+    return 1;
+  }
+  if (precision == CONTOUR3D_AMD_FP64) return build_mesh_t<double>(L, M, objs, n);
+  if (precision == CONTOUR3D_AMD_FP32) return build_mesh_t<float>(L, M, objs, n);
+  log_error("unknown precision %d", precision);
+  return 1;
+}
+
+// Validates the images, places them in the frame's pixel space and sets up
+// their output targets.
+int frame_setup(Lib& L, std::vector<Image>& imgs, uint64_t& npx) {
+  npx = 0;
+  for (size_t q = 0; q < imgs.size(); q++) {
+    Image& im = imgs[q];
+    const HostView& hv = im.hv;
+    if (hv.W == 0 || hv.H == 0 || hv.W > (1u << 16) || hv.H > (1u << 16) ||
+        uint64_t(hv.W) * uint64_t(hv.H) >= (uint64_t(1) << 31)) {
+      log_error("screen size %zu x %zu out of range", hv.W, hv.H);
+      return 1;
+    }
+    if (!im.rgb) {
+      log_error("rgb output pointer is NULL");
+      return 1;
+    }
+    if (im.nobj && !im.objs) {
+      log_error("contour objects pointer is NULL");
+      return 1;
+    }
+    // every image starts on a workgroup boundary of the frame's pixels
+    im.px0 = (npx + kThreads - 1) / kThreads * kThreads;
+    npx = im.px0 + uint64_t(hv.W) * uint64_t(hv.H);
+  }
+  if (npx >= (uint64_t(1) << 31)) {
+    log_error("%llu pixels in one frame exceed 2^31", (unsigned long long)npx);
+    return 1;
+  }
+  C3D_HIP(L.rgb.ensure(3 * npx + 1));
+  bool any_dep = false;
+  for (const Image& im : imgs) any_dep = any_dep || (im.depth && !im.dep_dev);
+  if (any_dep) C3D_HIP(L.depth.ensure(npx * sizeof(double)));
+  return 0;
+}
+
+template <typename R>
+void fill_rjobs(Lib& L, const std::vector<Image>& imgs) {
+  std::vector<RasterJob<R>>& rj = L.rjobs_host<R>();
+  rj.assign(imgs.size(), RasterJob<R>{});
+  for (size_t q = 0; q < imgs.size(); q++) {
+    const Image& im = imgs[q];
+    RasterJob<R>& j = rj[q];
+    j.v = dev_view<R>(im.hv);
+    j.px0 = im.px0;
+    j.obj0 = im.obj0;
+    j.obj1 = im.obj1;
+    j.rgb = im.rgb_dev ? im.rgb : L.rgb.as<uint8_t>() + 3 * im.px0;
+    j.depth = im.depth ? (im.dep_dev ? im.depth : L.depth.as<double>() + im.px0) : nullptr;
+    j.bg[0] = im.hv.bg.r;
+    j.bg[1] = im.hv.bg.g;
+    j.bg[2] = im.hv.bg.b;
+  }
+}
+
+// Tessellate the objects of every image and render them all.
+template <typename R>
+int run_frame_t(Lib& L, contour3d_amd_mesh& M, std::vector<Image>& imgs) {
+  uint64_t npx = 0;
+  if (frame_setup(L, imgs, npx)) return 1;
+  // the frame's object list: each image's objects, unless an earlier image
+  // shows exactly the same ones
+  std::vector<const contour3d_contour_obj_t*> objs;
+  for (size_t q = 0; q < imgs.size(); q++) {
+    Image& im = imgs[q];
+    bool shared = false;
+    for (size_t p = 0; p < q && !shared; p++) {
+      const Image& ip = imgs[p];
+      if (ip.nobj != im.nobj) continue;
+      bool eq = true;
+      for (size_t o = 0; o < im.nobj && eq; o++) eq = same_obj(ip.objs[o], im.objs[o]);
+      if (eq) {
+        im.obj0 = ip.obj0;
+        im.obj1 = ip.obj1;
+        shared = true;
+      }
+    }
+    if (shared) continue;
+    im.obj0 = uint32_t(objs.size());
+    for (size_t o = 0; o < im.nobj; o++) objs.push_back(im.objs + o);
+    im.obj1 = uint32_t(objs.size());
+  }
+  if (frame_lines(L, imgs)) return 1;
+  fill_rjobs<R>(L, imgs);
+  C3D_HIP(L.rjobs.ensure(imgs.size() * sizeof(RasterJob<R>)));
+  C3D_HIP(hipMemcpyAsync(L.rjobs.p, L.rjobs_host<R>().data(), imgs.size() * sizeof(RasterJob<R>),
+                         hipMemcpyHostToDevice, L.st));
+  RasterJob<R>* rjd = L.rjobs.as<RasterJob<R>>();
+  FrameHdr hh{};
+  if (tessellate<R>(L, M, objs, rjd, int(imgs.size()), hh)) return 1;
+  return raster_frame<R>(L, M, imgs, rjd, npx, hh);
+}
+
+int run_frame(Lib& L, contour3d_amd_mesh& M, std::vector<Image>& imgs, int precision) {
+  if (precision == CONTOUR3D_AMD_FP64) return run_frame_t<double>(L, M, imgs);
+  if (precision == CONTOUR3D_AMD_FP32) return run_frame_t<float>(L, M, imgs);
+  log_error("unknown precision %d", precision);
+  return 1;
+}
+
+// Render an already built mesh (all its objects) as one image.
+template <typename R>
+int render_mesh_t(Lib& L, const contour3d_amd_mesh& M, std::vector<Image>& imgs) {
+  uint64_t npx = 0;
+  if (frame_setup(L, imgs, npx)) return 1;
+  for (Image& im : imgs) {
+    im.obj0 = 0;
+    im.obj1 = uint32_t(M.base.size());
+  }
+  if (frame_lines(L, imgs)) return 1;
+  fill_rjobs<R>(L, imgs);
+  C3D_HIP(L.rjobs.ensure(imgs.size() * sizeof(RasterJob<R>)));
+  C3D_HIP(hipMemcpyAsync(L.rjobs.p, L.rjobs_host<R>().data(), imgs.size() * sizeof(RasterJob<R>),
+                         hipMemcpyHostToDevice, L.st));
+  RasterJob<R>* rjd = L.rjobs.as<RasterJob<R>>();
+  C3D_HIP(L.hdr.ensure(sizeof(FrameHdr)));
+  C3D_HIP(L.hstage.ensure(sizeof(FrameHdr)));
+  // the plan kernel recomputes the ranges from the mesh's tile starts
+  C3D_HIP(launch_plan<R>(M.bstart.as<uint64_t>(), M.ntiles, M.meta.as<ObjMeta>(), int(M.base.size()), rjd,
+                         int(imgs.size()), L.hdr.as<FrameHdr>(), ~uint64_t(0), L.st));
+  FrameHdr hh{};
+  hh.ntri = M.total;
+  hh.nwork = (M.total + kThreads) * imgs.size();  // bound on the aligned work space
+  return raster_frame<R>(L, M, imgs, rjd, npx, hh);
+}
+
+int render_mesh(Lib& L, const contour3d_amd_mesh& M, std::vector<Image>& imgs) {
+  return M.precision == CONTOUR3D_AMD_FP64 ? render_mesh_t<double>(L, M, imgs) : render_mesh_t<float>(L, M, imgs);
+}
+
+Image make_image(const V* cam, const V* light, const size_t sizes[2], const V* ctr, const V local[2],
+                 const contour3d_color_t* bg, size_t nobj, const contour3d_contour_obj_t* objs, size_t nlines,
+                 const contour3d_line_obj_t* lines, uint8_t* rgb, double* depth, bool rgb_host_only) {
+  Image im;
+  im.hv = make_view(cam, light, sizes, ctr, local, bg);
+  im.nobj = nobj;
+  im.objs = objs;
+  im.nlines = nlines;
+  im.lines = lines;
+  im.rgb = rgb;
+  im.depth = depth;
+  im.rgb_dev = !rgb_host_only && rgb && is_device_ptr(rgb);
+  im.dep_dev = depth && is_device_ptr(depth);
+  return im;
 }
 
 // output.c:125-140
@@ -883,14 +1087,13 @@ int contour3d_amd_finalize(void) {
   L.geoms.clear();
   for (auto& u : L.uploads) u->release();
   L.uploads.clear();
-  c3d::DevBuf* bufs[] = {&L.cnt, &L.start, &L.tiles, &L.gidx, &L.gout, &L.key, &L.id, &L.rgb,
-                         &L.depth, &L.fpix, &L.fdep, &L.frgb, &L.frags, &L.fcount, &L.vntmp};
+  c3d::DevBuf* bufs[] = {&L.cnt, &L.tiles, &L.tjobs, &L.rjobs, &L.hdr, &L.key, &L.id, &L.rgb,
+                         &L.depth, &L.fpix, &L.fdep, &L.frgb, &L.frags, &L.vntmp};
   for (auto* b : bufs) b->release();
   L.frame_mesh.release();
   L.frag_cap = 0;
-  L.hgout.release();
   L.hrgb.release();
-  L.hfcount.release();
+  L.hstage.release();
   for (auto e : L.prof.pool) (void)hipEventDestroy(e);
   L.prof.pool.clear();
   L.prof.pending.clear();
@@ -925,10 +1128,30 @@ int contour3d_amd_execute_rgb(const contour3d_vector_t* camera_position, const c
     c3d::log_error("rgb output pointer is NULL");
     return 1;
   }
-  const auto hv = c3d::make_view(camera_position, light_direction, screen_sizes, screen_center, screen_local,
-                                 bg_color);
-  if (c3d::build_mesh(L, L.frame_mesh, contour_objs, num_contours, precision)) return 1;
-  return c3d::render(L, L.frame_mesh, hv, num_lines, line_objs, rgb, depth, false);
+  std::vector<c3d::Image> imgs{c3d::make_image(camera_position, light_direction, screen_sizes, screen_center,
+                                               screen_local, bg_color, num_contours, contour_objs, num_lines,
+                                               line_objs, rgb, depth, false)};
+  return c3d::run_frame(L, L.frame_mesh, imgs, precision);
+}
+
+int contour3d_amd_execute_batch(size_t num_images, const contour3d_amd_image_t* images, int precision) {
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  if (c3d::ensure_init(L, 0)) return 1;
+  if (num_images == 0) return 0;
+  if (!images) {
+    c3d::log_error("images pointer is NULL");
+    return 1;
+  }
+  std::vector<c3d::Image> imgs;
+  imgs.reserve(num_images);
+  for (size_t q = 0; q < num_images; q++) {
+    const contour3d_amd_image_t& im = images[q];
+    imgs.push_back(c3d::make_image(&im.camera_position, &im.light_direction, im.screen_sizes, &im.screen_center,
+                                   im.screen_local, &im.bg_color, im.num_contours, im.contour_objs, im.num_lines,
+                                   im.line_objs, im.rgb, im.depth, false));
+  }
+  return c3d::run_frame(L, L.frame_mesh, imgs, precision);
 }
 
 int contour3d_execute(const sdecomp_info_t* const sdecomp_info, const contour3d_vector_t* const camera_position,
@@ -949,27 +1172,24 @@ int contour3d_execute(const sdecomp_info_t* const sdecomp_info, const contour3d_
     c3d::log_error("output file name is NULL");
     return 1;
   }
-  const auto hv = c3d::make_view(camera_position, light_direction, screen_sizes, screen_center, screen_local,
-                                 bg_color);
-  if (hv.W == 0 || hv.H == 0) {
-    c3d::log_error("screen size %zu x %zu out of range", hv.W, hv.H);
+  const size_t W = screen_sizes[0], H = screen_sizes[1];
+  if (W == 0 || H == 0 || W > (1u << 16) || H > (1u << 16)) {
+    c3d::log_error("screen size %zu x %zu out of range", W, H);
     return 1;
   }
-  if (c3d::build_mesh(L, L.frame_mesh, contour3d_contour_objs, num_contours, L.precision)) {
-    c3d::log_error("contour processing failed");
-    return 1;
-  }
-  const size_t bytes = hv.W * hv.H * 3;
-  if (L.hrgb.ensure(bytes) != hipSuccess) {
+  if (L.hrgb.ensure(W * H * 3) != hipSuccess) {
     c3d::log_error("failed to allocate image buffer");
     return 1;
   }
   uint8_t* host = static_cast<uint8_t*>(L.hrgb.p);
-  if (c3d::render(L, L.frame_mesh, hv, num_lines, contour3d_line_objs, host, nullptr, true)) {
+  std::vector<c3d::Image> imgs{c3d::make_image(camera_position, light_direction, screen_sizes, screen_center,
+                                               screen_local, bg_color, num_contours, contour3d_contour_objs,
+                                               num_lines, contour3d_line_objs, host, nullptr, true)};
+  if (c3d::run_frame(L, L.frame_mesh, imgs, L.precision)) {
     c3d::log_error("image output failed");
     return 1;
   }
-  return c3d::write_ppm(fname, hv.W, hv.H, host);
+  return c3d::write_ppm(fname, W, H, host);
 }
 
 int contour3d_amd_mesh_build(const contour3d_contour_obj_t* contour_objs, size_t num_contours, int precision,
@@ -1078,9 +1298,10 @@ int contour3d_amd_mesh_render(const contour3d_amd_mesh_t* mesh, const contour3d_
   if (c3d::ensure_init(L, 0)) return 1;
   if (!c3d::args_ok(camera_position, light_direction, screen_sizes, screen_center, screen_local, bg_color))
     return 1;
-  const auto hv = c3d::make_view(camera_position, light_direction, screen_sizes, screen_center, screen_local,
-                                 bg_color);
-  return c3d::render(L, *mesh, hv, num_lines, line_objs, rgb, depth, false);
+  std::vector<c3d::Image> imgs{c3d::make_image(camera_position, light_direction, screen_sizes, screen_center,
+                                               screen_local, bg_color, 0, nullptr, num_lines, line_objs, rgb,
+                                               depth, false)};
+  return c3d::render_mesh(L, *mesh, imgs);
 }
 
 int contour3d_amd_mesh_free(contour3d_amd_mesh_t* mesh) {

@@ -12,6 +12,7 @@ namespace c3d {
 constexpr int kThreads = 256;           // one workgroup = 4 wave64
 constexpr int kCellsPerBlock = 1024;    // tessellation tile: 4 chunks of 256 cells
 constexpr int kMaxThr = 4;              // thresholds classified per field read
+constexpr int kScanTile = 1024;         // entries per workgroup of the tile-count scan
 
 // Division by a runtime constant with a multiply-high (Granlund-Montgomery,
 // "round-up" variant): q = (umulhi(n, m) + n) >> s for every 32-bit n.
@@ -38,11 +39,20 @@ struct FieldDev {
 };
 
 // Thresholds classified in one pass over a field, each with the offset of
-// its object's first tile in the global tile-count array.
+// its object's first tile in the frame's tile-count array.
 struct ThrSet {
   double thr[kMaxThr];
   uint32_t boff[kMaxThr];
   int n;
+};
+
+// One pass over one field: the unit of work of the tessellation kernels.
+// Jobs are stored in a device table; their tiles are numbered consecutively
+// from tile0.
+struct TessJob {
+  FieldDev f;
+  ThrSet ts;
+  uint32_t tile0;
 };
 
 // Triangle storage, structure of arrays with capacity cap: component c of
@@ -63,6 +73,7 @@ struct ObjMeta {
   uint64_t count;
   const uint64_t* bstart;  // triangle start of each tile of the object's field (nblocks + 1)
   uint32_t ncx, ncy, ncz;
+  uint32_t tile_off;       // offset of the object's first tile in the frame's tile arrays
   uint8_t rgb[4];
 };
 
@@ -92,43 +103,80 @@ struct ViewDev {
   int32_t W, H;
 };
 
+// One image of a frame: its camera, its contour objects [obj0, obj1) (their
+// triangles are contiguous in the mesh; images showing the same objects
+// share them), its pixels in the frame's z-buffer from px0, and where its
+// colours (and depths) go.  The raster kernels walk a work space that
+// concatenates every image's triangle range: image j owns work items
+// [wk0, wk0 + tri1 - tri0).
+template <typename R>
+struct RasterJob {
+  ViewDev<R> v;
+  uint64_t tri0, tri1;   // filled on the device by k_plan
+  uint64_t wk0;          // filled on the device by k_plan
+  uint64_t px0;
+  uint32_t obj0, obj1;
+  uint8_t* rgb;          // device, W*H*3 in PPM row order
+  double* depth;         // device or null
+  uint8_t bg[4];
+};
+
+// Frame-wide values on the device.  k_plan writes ntri, nwork and the
+// overflow bit 1; k_zinit clears nfrag and bit 2; the host writes nline.
+struct FrameHdr {
+  uint64_t ntri;        // triangles in the mesh
+  uint64_t nwork;       // raster work items (sum of the images' triangle ranges)
+  uint32_t overflow;    // bit 1: mesh capacity too small; bit 2: FP64 fragment list overflowed
+  uint32_t nfrag;       // FP64 fragments kept by the depth pass
+  uint32_t nline;       // line fragments of the frame
+  uint32_t pad;
+};
+
 // ----------------------------------------------------------------------
 // launchers (c3d_kernels.hip); all enqueue on `st` and return hipError_t
 
 hipError_t upload_tables(const NbrTable& nt);
-hipError_t launch_count(const FieldDev& f, const ThrSet& ts, uint32_t* cnt, hipStream_t st);
+int persistent_blocks();  // workgroups that fill the device once (multiple of 8)
+
+hipError_t launch_count(const TessJob* jobs, int njobs, uint32_t ntiles, uint32_t* cnt, uint32_t grid,
+                        hipStream_t st);
 hipError_t launch_scan(const uint32_t* cnt, uint64_t* start, uint64_t n, uint64_t* tile_tmp,
                        hipStream_t st);
-hipError_t launch_gather_u64(const uint64_t* src, const uint32_t* idx, uint64_t* dst, int n,
-                             hipStream_t st);
+// object bases/counts, frame total and per-image triangle ranges from the
+// tile starts (one workgroup)
 template <typename R>
-hipError_t launch_emit(const FieldDev& f, const ThrSet& ts, const uint64_t* start,
-                       const MeshView<R>& m, hipStream_t st);
+hipError_t launch_plan(const uint64_t* start, uint64_t ntiles, ObjMeta* meta, int nobj, RasterJob<R>* rjobs,
+                       int njobs, FrameHdr* hdr, uint64_t cap, hipStream_t st);
+template <typename R>
+hipError_t launch_emit(const TessJob* jobs, int njobs, uint32_t ntiles, const uint64_t* start,
+                       const MeshView<R>& m, FrameHdr* hdr, uint32_t grid, hipStream_t st);
 // unit vertex normals of every triangle of one object, 9 doubles per
 // triangle (NaN rows for triangles outside the rendered interior)
 template <typename R>
 hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, uint64_t count,
                                  double* out, hipStream_t st);
-hipError_t launch_zinit(uint64_t* key, uint32_t* id, uint64_t npx, int precision, hipStream_t st);
+hipError_t launch_zinit(uint64_t* key, uint32_t* id, uint64_t npx, int precision, FrameHdr* hdr,
+                        hipStream_t st);
+// FP32: one pass over packed (depth << 32 | id) keys.  FP64 pass 1 over the
+// geometry (used when the fragment list of the depth pass overflowed).
 template <typename R>
-hipError_t launch_raster(const MeshView<R>& m, uint64_t ntri, const ViewDev<R>& v, int pass,
-                         uint64_t* key, uint32_t* id, hipStream_t st);
-// FP64 depth pass that keeps the fragments that may end up nearest; the
-// count goes to counter[0], counter[1] is set on overflow
-hipError_t launch_raster_frags(const MeshView<double>& m, uint64_t ntri, const ViewDev<double>& v,
-                               uint64_t* key, Frag* frags, uint32_t cap, uint32_t* counter,
+hipError_t launch_raster(const MeshView<R>& m, const RasterJob<R>* rjobs, int njobs, const FrameHdr* hdr,
+                         int pass, uint64_t* key, uint32_t* id, uint32_t grid, hipStream_t st);
+// FP64 depth pass that keeps the fragments that may end up nearest
+hipError_t launch_raster_frags(const MeshView<double>& m, const RasterJob<double>* rjobs, int njobs,
+                               FrameHdr* hdr, uint64_t* key, Frag* frags, uint32_t cap, uint32_t grid,
                                hipStream_t st);
-// FP64 id pass over the fragment list
-hipError_t launch_frag_ids(const Frag* frags, uint32_t n, const uint64_t* key, uint32_t* id,
-                           hipStream_t st);
-hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t st);
-hipError_t launch_line_frags(const uint32_t* pix, const double* depth, uint32_t nfrag,
-                             uint32_t id_base, int pass, int precision, uint64_t* key, uint32_t* id,
-                             hipStream_t st);
+// FP64 id pass over the fragment list (reads the count from hdr; does
+// nothing when the list overflowed, and the geometry id pass of
+// launch_raster(pass 1) then does the work)
+hipError_t launch_frag_ids(const Frag* frags, uint32_t cap, const FrameHdr* hdr, const uint64_t* key,
+                           uint32_t* id, uint32_t grid, hipStream_t st);
+// line fragments [0, hdr->nline) of the frame; `cap` sizes the grid
+hipError_t launch_line_frags(const uint32_t* pix, const double* depth, uint32_t cap, const FrameHdr* hdr,
+                             int pass, int precision, uint64_t* key, uint32_t* id, hipStream_t st);
 template <typename R>
-hipError_t launch_resolve(const MeshView<R>& m, uint64_t ntri, const ViewDev<R>& v,
-                          const ObjMeta* meta, int nobj, const uint8_t* frag_rgb, const uint8_t bg[3],
-                          const uint64_t* key, const uint32_t* id, uint8_t* rgb, double* depth,
-                          hipStream_t st);
+hipError_t launch_resolve(const MeshView<R>& m, const RasterJob<R>* rjobs, int njobs, uint64_t npx_total,
+                          const ObjMeta* meta, int nobj, const uint8_t* frag_rgb, const FrameHdr* hdr,
+                          const uint64_t* key, const uint32_t* id, uint32_t grid, hipStream_t st);
 
 }  // namespace c3d

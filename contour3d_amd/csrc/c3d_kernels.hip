@@ -29,11 +29,22 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
   return uint32_t((uint64_t(__umulhi(n, f.m)) + n) >> f.s);
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous range of tiles so neighbouring z-planes meet in its own L2.
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t ntiles) {
-  const uint32_t per = (ntiles + 7) / 8;
-  return (blockIdx.x % 8) * per + blockIdx.x / 8;
+// Workgroups are dispatched round-robin over the 8 XCDs.  Work items
+// (tiles) are split into 8 contiguous segments, one per XCD, and the XCD's
+// workgroups stride through their segment, so neighbouring z-planes meet in
+// that XCD's L2.  The grid must be a multiple of 8.
+struct XcdRange {
+  uint32_t begin, end, step;
+};
+
+__device__ __forceinline__ XcdRange xcd_range(uint32_t n) {
+  const uint32_t x = blockIdx.x % 8u, w = blockIdx.x / 8u, per_xcd = gridDim.x / 8u;
+  const uint32_t seg = (n + 7u) / 8u;
+  XcdRange r;
+  r.begin = x * seg + w;
+  r.end = min((x + 1u) * seg, n);
+  r.step = per_xcd;
+  return r;
 }
 
 template <typename R>
@@ -209,33 +220,48 @@ __device__ __forceinline__ uint64_t packed_counts(const ThrSet& ts, const double
 // ----------------------------------------------------------------------
 // tessellation: count pass
 
-__global__ __launch_bounds__(kThreads) void k_count(FieldDev f, ThrSet ts, uint32_t* __restrict__ cnt) {
-  const uint32_t tile = xcd_tile(f.nblocks);
-  if (tile >= f.nblocks) return;
+__device__ __forceinline__ uint32_t find_tjob(const TessJob* __restrict__ jobs, int n, uint32_t tile) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].tile0 <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  return uint32_t(lo);
+}
+
+__global__ __launch_bounds__(kThreads) void k_count(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
+                                                    uint32_t* __restrict__ cnt) {
   __shared__ uint64_t s_wave[kThreads / 64];
   __shared__ uint8_t s_ntri[256];
   s_ntri[threadIdx.x] = c_ntri[threadIdx.x];
   __syncthreads();
-  uint64_t acc = 0;
+  const XcdRange R = xcd_range(ntiles);
+  for (uint32_t gt = R.begin; gt < R.end; gt += R.step) {
+    const uint32_t jb = find_tjob(jobs, njobs, gt);
+    const FieldDev f = jobs[jb].f;
+    const ThrSet ts = jobs[jb].ts;
+    const uint32_t tile = gt - jobs[jb].tile0;
+    uint64_t acc = 0;
 #pragma unroll
-  for (int chunk = 0; chunk < kCellsPerBlock / kThreads; chunk++) {
-    const uint32_t c = tile * kCellsPerBlock + chunk * kThreads + threadIdx.x;
-    if (c < f.ncells) {
-      double v[8];
-      load_corners(f, locate(f, c), v);
-      uint32_t code[kMaxThr];
-      acc += packed_counts(ts, v, code, s_ntri);
+    for (int chunk = 0; chunk < kCellsPerBlock / kThreads; chunk++) {
+      const uint32_t c = tile * kCellsPerBlock + chunk * kThreads + threadIdx.x;
+      if (c < f.ncells) {
+        double v[8];
+        load_corners(f, locate(f, c), v);
+        uint32_t code[kMaxThr];
+        acc += packed_counts(ts, v, code, s_ntri);
+      }
     }
+    uint64_t tot;
+    block_excl_scan64(acc, s_wave, tot);
+    if (threadIdx.x < uint32_t(ts.n)) cnt[ts.boff[threadIdx.x] + tile] = uint32_t(tot >> (16 * threadIdx.x)) & 0xffffu;
   }
-  uint64_t tot;
-  block_excl_scan64(acc, s_wave, tot);
-  if (threadIdx.x < uint32_t(ts.n)) cnt[ts.boff[threadIdx.x] + tile] = uint32_t(tot >> (16 * threadIdx.x)) & 0xffffu;
 }
 
 // ----------------------------------------------------------------------
 // scan of tile counts into triangle starts (u64, exclusive, start[n] = total)
 
-constexpr int kScanTile = 1024;
 
 __global__ __launch_bounds__(kThreads) void k_scan_reduce(const uint32_t* __restrict__ cnt, uint64_t n,
                                                           uint64_t* __restrict__ tile_sum) {
@@ -295,14 +321,40 @@ __global__ __launch_bounds__(kThreads) void k_scan_apply(const uint32_t* __restr
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[n] = tile_off[gridDim.x];
 }
 
-__global__ void k_gather_u64(const uint64_t* __restrict__ src, const uint32_t* __restrict__ idx,
-                             uint64_t* __restrict__ dst, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[idx[i]];
+// Object bases and counts, the frame total and each image's triangle range,
+// read off the tile starts on the device (so that no host round trip sits
+// between counting and emitting when a frame is replayed from a graph).
+template <typename R>
+__global__ void k_plan(const uint64_t* __restrict__ start, uint64_t ntiles, ObjMeta* __restrict__ meta, int nobj,
+                       RasterJob<R>* __restrict__ rjobs, int njobs, FrameHdr* __restrict__ hdr, uint64_t cap) {
+  // objects are never empty (at least one cell), so nobj > 0 implies ntiles > 0
+  for (int o = threadIdx.x; o < nobj; o += blockDim.x) {
+    const uint64_t b = start[meta[o].tile_off];
+    const uint64_t e = o + 1 < nobj ? start[meta[o + 1].tile_off] : start[ntiles];
+    meta[o].base = b;
+    meta[o].count = e - b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint64_t ntri = ntiles ? start[ntiles] : 0;
+    uint64_t wk = 0;
+    for (int j = 0; j < njobs; j++) {
+      const uint64_t t0 = rjobs[j].obj0 < uint32_t(nobj) ? meta[rjobs[j].obj0].base : ntri;
+      const uint64_t t1 = rjobs[j].obj1 < uint32_t(nobj) ? meta[rjobs[j].obj1].base : ntri;
+      // each image's work starts on a workgroup boundary, so that every
+      // workgroup round of the raster kernels belongs to one image
+      wk = (wk + kThreads - 1) / kThreads * kThreads;
+      rjobs[j].tri0 = t0;
+      rjobs[j].tri1 = t1;
+      rjobs[j].wk0 = wk;
+      wk += t1 - t0;
+    }
+    hdr->ntri = ntri;
+    hdr->nwork = wk;
+    hdr->nfrag = 0;
+    hdr->overflow = ntri > cap ? 1u : 0u;
+  }
 }
-
-// ----------------------------------------------------------------------
-// tessellation: emit pass
 
 // linear interpolation along a tetra edge (triangulate.c:40-73); the 1e-8
 // test is on the FP64 corner values in both forms
@@ -318,16 +370,19 @@ __device__ __forceinline__ V3<R> interp(double thr, V3<R> p0, double v0, V3<R> p
   return vadd(p0, vscale(fct, vsub(p1, p0)));
 }
 
+// ----------------------------------------------------------------------
+// tessellation: emit pass
+//
 // Each tile of 1024 consecutive cells is processed in chunks of 256: every
 // thread classifies one cell against all thresholds, a block scan of the
 // packed counts places the chunk's triangles (threshold-major, then cell,
-// then slot order), and the whole block then computes the queued triangles
-// one per thread so that the stores of consecutive triangles coalesce.
+// then slot order), and the whole block then computes the chunk's triangles
+// one per thread -- finding its cell by binary search over the chunk's
+// prefix -- so that the stores of consecutive triangles coalesce.
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_emit(FieldDev f, ThrSet ts, const uint64_t* __restrict__ start,
-                                                   MeshView<R> m) {
-  const uint32_t tile = xcd_tile(f.nblocks);
-  if (tile >= f.nblocks) return;
+__global__ __launch_bounds__(kThreads) void k_emit(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
+                                                   const uint64_t* __restrict__ start, MeshView<R> m,
+                                                   FrameHdr* __restrict__ hdr) {
   __shared__ double s_val[8][kThreads];
   __shared__ double s_grid[6][kThreads];  // x0 x1 y0 y1 z0 z1 of each cell (identity map)
   __shared__ uint8_t s_code[kMaxThr][kThreads];
@@ -335,120 +390,136 @@ __global__ __launch_bounds__(kThreads) void k_emit(FieldDev f, ThrSet ts, const 
   __shared__ uint64_t s_wave[kThreads / 64];
   __shared__ uint8_t s_ntri[256];
   s_ntri[threadIdx.x] = c_ntri[threadIdx.x];
-  uint64_t base[kMaxThr];
-#pragma unroll
-  for (int q = 0; q < kMaxThr; q++) base[q] = q < ts.n ? start[ts.boff[q] + tile] : 0;
-  R* const px0 = m.pos;
   const uint64_t cap = m.cap;
   __syncthreads();
-
-  for (int chunk = 0; chunk < kCellsPerBlock / kThreads; chunk++) {
-    const uint32_t c0 = tile * kCellsPerBlock + chunk * kThreads;
-    if (c0 >= f.ncells) break;
-    const uint32_t c = c0 + threadIdx.x;
-    const bool valid = c < f.ncells;
-    double v[8];
-    if (valid) {
-      const CellLoc L = locate(f, c);
-      load_corners(f, L, v);
-      if (!f.cx) {
-        s_grid[0][threadIdx.x] = f.gx[L.i];
-        s_grid[1][threadIdx.x] = f.gx[L.i + 1];
-        s_grid[2][threadIdx.x] = f.gy[L.j];
-        s_grid[3][threadIdx.x] = f.gy[L.j + 1];
-        s_grid[4][threadIdx.x] = f.gz[L.k];
-        s_grid[5][threadIdx.x] = f.gz[L.k + 1];
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < 8; q++) v[q] = 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < 8; q++) s_val[q][threadIdx.x] = v[q];
-    uint32_t code[kMaxThr];
-    const uint64_t packed = valid ? packed_counts(ts, v, code, s_ntri) : 0ull;
-    uint64_t tot;
-    const uint64_t ex = block_excl_scan64(packed, s_wave, tot);
-    uint32_t qoff[kMaxThr + 1];
-    qoff[0] = 0;
+  const XcdRange XR = xcd_range(ntiles);
+  for (uint32_t gt = XR.begin; gt < XR.end; gt += XR.step) {
+    const uint32_t jb = find_tjob(jobs, njobs, gt);
+    const FieldDev f = jobs[jb].f;
+    const ThrSet ts = jobs[jb].ts;
+    const uint32_t tile = gt - jobs[jb].tile0;
+    uint64_t base[kMaxThr];
+    double thrs[kMaxThr];
 #pragma unroll
     for (int q = 0; q < kMaxThr; q++) {
-      qoff[q + 1] = qoff[q] + (uint32_t(tot >> (16 * q)) & 0xffffu);
-      s_pref[q][threadIdx.x] = uint16_t(ex >> (16 * q));
-      if (valid) s_code[q][threadIdx.x] = uint8_t(code[q]);
+      base[q] = q < ts.n ? start[ts.boff[q] + tile] : 0;
+      thrs[q] = ts.thr[q];
     }
-    __syncthreads();
-    const uint32_t total = qoff[kMaxThr];
-    for (uint32_t it = threadIdx.x; it < total; it += kThreads) {
-      // threshold, then the last cell whose first triangle is at or before
-      // the item (cells without triangles share their successor's prefix)
-      int q = 0;
-#pragma unroll
-      for (int z = 1; z < kMaxThr; z++) q += it >= qoff[z] ? 1 : 0;
-      uint32_t qsel = 0;
-#pragma unroll
-      for (int z = 1; z < kMaxThr; z++) qsel = q == z ? qoff[z] : qsel;
-      const uint32_t r = it - qsel;
-      uint32_t l = 0;
-#pragma unroll
-      for (uint32_t step = kThreads / 2; step; step >>= 1) l += s_pref[q][l + step] <= r ? step : 0u;
-      const uint32_t sl = r - s_pref[q][l];
-      const double thr = ts.thr[q];
-      const uint32_t cc = c0 + l;
-      const CellLoc L = locate(f, cc);
-      const uint64_t ent = c_cslot[uint32_t(s_code[q][l]) * 12u + sl];
-      V3<R> P[3];
-#pragma unroll
-      for (int vv = 0; vv < 3; vv++) {
-        const uint32_t tail = uint32_t(ent >> (6 * vv)) & 7u;
-        const uint32_t head = uint32_t(ent >> (6 * vv + 3)) & 7u;
-        V3<R> pt, ph;
-        if (f.cx) {
-          pt = corner_pos<R>(f, L, tail);
-          ph = corner_pos<R>(f, L, head);
-        } else {
-          pt = {R(s_grid[tail & 1u][l]), R(s_grid[2 + ((tail >> 1) & 1u)][l]), R(s_grid[4 + (tail >> 2)][l])};
-          ph = {R(s_grid[head & 1u][l]), R(s_grid[2 + ((head >> 1) & 1u)][l]), R(s_grid[4 + (head >> 2)][l])};
+    for (int chunk = 0; chunk < kCellsPerBlock / kThreads; chunk++) {
+      const uint32_t c0 = tile * kCellsPerBlock + chunk * kThreads;
+      if (c0 >= f.ncells) break;
+      const uint32_t c = c0 + threadIdx.x;
+      const bool valid = c < f.ncells;
+      double v[8];
+      if (valid) {
+        const CellLoc L = locate(f, c);
+        load_corners(f, L, v);
+        if (!f.cx) {
+          s_grid[0][threadIdx.x] = f.gx[L.i];
+          s_grid[1][threadIdx.x] = f.gx[L.i + 1];
+          s_grid[2][threadIdx.x] = f.gy[L.j];
+          s_grid[3][threadIdx.x] = f.gy[L.j + 1];
+          s_grid[4][threadIdx.x] = f.gz[L.k];
+          s_grid[5][threadIdx.x] = f.gz[L.k + 1];
         }
-        P[vv] = interp<R>(thr, pt, s_val[tail][l], ph, s_val[head][l]);
-      }
-      // face normal and area (triangulate.c:75-93); the smoothing weight
-      // area * unit normal is formed as normal.c:39-40 forms it
-      const V3<R> n = vcross(vsub(P[1], P[0]), vsub(P[2], P[0]));
-      const R area = sqrt(vdot(n, n));
-      const V3<R> fn = vscale(R(1) / area, n);
-      const V3<R> w = vscale(area, fn);
-      const bool in = L.i >= 1 && L.i + 2 <= f.ncx && L.j >= 1 && L.j + 2 <= f.ncy && L.k >= 1 &&
-                      L.k + 2 <= f.ncz;
-      uint64_t bq = base[0];
+      } else {
 #pragma unroll
-      for (int z = 1; z < kMaxThr; z++) bq = q == z ? base[z] : bq;
-      const uint64_t g = bq + r;
-      R* p = px0 + g;
-#pragma unroll
-      for (int vv = 0; vv < 3; vv++) {
-        p[(3 * vv + 0) * cap] = P[vv].x;
-        p[(3 * vv + 1) * cap] = P[vv].y;
-        p[(3 * vv + 2) * cap] = P[vv].z;
+        for (int q = 0; q < 8; q++) v[q] = 0.0;
       }
-      R* wp = m.wfn + g;
-      wp[0] = w.x;
-      wp[cap] = w.y;
-      wp[2 * cap] = w.z;
-      m.edges[g] = uint32_t(ent >> 18) | uint32_t(in) << 15;
-      m.cell[g] = cc;
+#pragma unroll
+      for (int q = 0; q < 8; q++) s_val[q][threadIdx.x] = v[q];
+      uint32_t code[kMaxThr];
+      const uint64_t packed = valid ? packed_counts(ts, v, code, s_ntri) : 0ull;
+      uint64_t tot;
+      const uint64_t ex = block_excl_scan64(packed, s_wave, tot);
+      uint32_t qoff[kMaxThr + 1];
+      qoff[0] = 0;
+#pragma unroll
+      for (int q = 0; q < kMaxThr; q++) {
+        qoff[q + 1] = qoff[q] + (uint32_t(tot >> (16 * q)) & 0xffffu);
+        s_pref[q][threadIdx.x] = uint16_t(ex >> (16 * q));
+        if (valid) s_code[q][threadIdx.x] = uint8_t(code[q]);
+      }
+      __syncthreads();
+      const uint32_t total = qoff[kMaxThr];
+      for (uint32_t it = threadIdx.x; it < total; it += kThreads) {
+        // threshold, then the last cell whose first triangle is at or before
+        // the item (cells without triangles share their successor's prefix)
+        int q = 0;
+#pragma unroll
+        for (int z = 1; z < kMaxThr; z++) q += it >= qoff[z] ? 1 : 0;
+        uint32_t qsel = 0;
+#pragma unroll
+        for (int z = 1; z < kMaxThr; z++) qsel = q == z ? qoff[z] : qsel;
+        const uint32_t r = it - qsel;
+        uint32_t l = 0;
+#pragma unroll
+        for (uint32_t step = kThreads / 2; step; step >>= 1) l += s_pref[q][l + step] <= r ? step : 0u;
+        const uint32_t sl = r - s_pref[q][l];
+        double thr = thrs[0];
+#pragma unroll
+        for (int z = 1; z < kMaxThr; z++) thr = q == z ? thrs[z] : thr;
+        const uint32_t cc = c0 + l;
+        const CellLoc L = locate(f, cc);
+        const uint64_t ent = c_cslot[uint32_t(s_code[q][l]) * 12u + sl];
+        V3<R> P[3];
+#pragma unroll
+        for (int vv = 0; vv < 3; vv++) {
+          const uint32_t tail = uint32_t(ent >> (6 * vv)) & 7u;
+          const uint32_t head = uint32_t(ent >> (6 * vv + 3)) & 7u;
+          V3<R> pt, ph;
+          if (f.cx) {
+            pt = corner_pos<R>(f, L, tail);
+            ph = corner_pos<R>(f, L, head);
+          } else {
+            pt = {R(s_grid[tail & 1u][l]), R(s_grid[2 + ((tail >> 1) & 1u)][l]), R(s_grid[4 + (tail >> 2)][l])};
+            ph = {R(s_grid[head & 1u][l]), R(s_grid[2 + ((head >> 1) & 1u)][l]), R(s_grid[4 + (head >> 2)][l])};
+          }
+          P[vv] = interp<R>(thr, pt, s_val[tail][l], ph, s_val[head][l]);
+        }
+        // face normal and area (triangulate.c:75-93); the smoothing weight
+        // area * unit normal is formed as normal.c:39-40 forms it
+        const V3<R> n = vcross(vsub(P[1], P[0]), vsub(P[2], P[0]));
+        const R area = sqrt(vdot(n, n));
+        const V3<R> fn = vscale(R(1) / area, n);
+        const V3<R> w = vscale(area, fn);
+        const bool in = L.i >= 1 && L.i + 2 <= f.ncx && L.j >= 1 && L.j + 2 <= f.ncy && L.k >= 1 &&
+                        L.k + 2 <= f.ncz;
+        uint64_t bq = base[0];
+#pragma unroll
+        for (int z = 1; z < kMaxThr; z++) bq = q == z ? base[z] : bq;
+        const uint64_t g = bq + r;
+        if (g >= cap) {
+          hdr->overflow = 1u;  // capacity reserved ahead of a replayed frame was too small
+          continue;
+        }
+        R* p = m.pos + g;
+#pragma unroll
+        for (int vv = 0; vv < 3; vv++) {
+          p[(3 * vv + 0) * cap] = P[vv].x;
+          p[(3 * vv + 1) * cap] = P[vv].y;
+          p[(3 * vv + 2) * cap] = P[vv].z;
+        }
+        R* wp = m.wfn + g;
+        wp[0] = w.x;
+        wp[cap] = w.y;
+        wp[2 * cap] = w.z;
+        m.edges[g] = uint32_t(ent >> 18) | uint32_t(in) << 15;
+        m.cell[g] = cc;
+      }
+#pragma unroll
+      for (int q = 0; q < kMaxThr; q++) base[q] += qoff[q + 1] - qoff[q];
+      __syncthreads();
     }
-#pragma unroll
-    for (int q = 0; q < kMaxThr; q++) base[q] += qoff[q + 1] - qoff[q];
-    __syncthreads();
   }
 }
 
 // ----------------------------------------------------------------------
-// vertex-normal smoothing: gather over the cells sharing each vertex's
-// lattice edge, in increasing cell order and slot order (normal.c:6-165)
+// vertex-normal smoothing
 
 __constant__ NbrTable c_nbr;
+
+
 
 __device__ __forceinline__ uint64_t lower_bound_cell(const uint32_t* __restrict__ cell, uint64_t lo, uint64_t hi,
                                                      uint32_t key) {
@@ -615,35 +686,57 @@ __device__ __forceinline__ R pixel_depth(const Projected<R>& P, R w0, R w1, R w2
   return R(1) / (w0 / P.q[0].z + w1 / P.q[1].z + w2 / P.q[2].z);
 }
 
-// FP64: pass 0 keeps the largest depth per pixel, pass 1 the latest serial
-// id among the fragments at that depth.  FP32: one pass over packed
-// (depth << 32 | id) keys.
+// the image owning raster work item u (the last job starting at or before
+// it: images with no triangles share their successor's start)
+template <typename R>
+__device__ __forceinline__ int find_rjob(const RasterJob<R>* __restrict__ jobs, int n, uint64_t u) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].wk0 <= u) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// FP32: one pass over packed (depth << 32 | id) keys.  FP64 pass 1: the
+// latest serial id among the fragments at each pixel's final depth, walking
+// the geometry; it only runs when the depth pass's fragment list overflowed.
 template <typename R, int PASS>
-__global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, uint64_t ntri, ViewDev<R> v,
-                                                     uint64_t* __restrict__ key, uint32_t* __restrict__ id) {
-  const uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
-  if (t >= ntri) return;
-  if (!((m.edges[t] >> 15) & 1u)) return;
-  Projected<R> P;
-  if (!setup_tri(m, t, v, P)) return;
-  for (int32_t j = P.j0; j <= P.j1; j++) {
-    for (int32_t i = P.i0; i <= P.i1; i++) {
-      R w0, w1, w2;
-      if (!pixel_weights(v, P, i, j, w0, w1, w2)) continue;
-      const R dep = pixel_depth(P, w0, w1, w2);
-      const uint64_t px = uint64_t(j) * uint64_t(v.W) + uint64_t(i);
-      // the stored keys only grow, so a key below any value read from the
-      // buffer can never change it and skips the atomic
-      if constexpr (sizeof(R) == 8) {
-        const uint64_t k = ord64(dep);
-        if (PASS == 0) {
+__global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, const RasterJob<R>* __restrict__ rjobs, int njobs,
+                                                     const FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
+                                                     uint32_t* __restrict__ id) {
+  if (sizeof(R) == 8 && PASS == 1 && !(hdr->overflow & 2u)) return;
+  const uint64_t nwork = hdr->nwork;
+  for (uint64_t u0 = uint64_t(blockIdx.x) * kThreads; u0 < nwork; u0 += uint64_t(gridDim.x) * kThreads) {
+    // one image per workgroup round: its constants are uniform
+    const int jb = njobs > 1 ? find_rjob(rjobs, njobs, u0) : 0;
+    const uint64_t t = rjobs[jb].tri0 + (u0 + threadIdx.x - rjobs[jb].wk0);
+    if (t >= rjobs[jb].tri1) continue;
+    if (!((m.edges[t] >> 15) & 1u)) continue;
+    const ViewDev<R> v = rjobs[jb].v;
+    const uint64_t px0 = rjobs[jb].px0;
+    Projected<R> P;
+    if (!setup_tri(m, t, v, P)) continue;
+    for (int32_t j = P.j0; j <= P.j1; j++) {
+      for (int32_t i = P.i0; i <= P.i1; i++) {
+        R w0, w1, w2;
+        if (!pixel_weights(v, P, i, j, w0, w1, w2)) continue;
+        const R dep = pixel_depth(P, w0, w1, w2);
+        const uint64_t px = px0 + uint64_t(j) * uint64_t(v.W) + uint64_t(i);
+        // the stored keys only grow, so a key below any value read from the
+        // buffer can never change it and skips the atomic
+        if constexpr (sizeof(R) == 8) {
+          const uint64_t k = ord64(dep);
+          if (PASS == 0) {
+            if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+          } else if (k == key[px]) {
+            atomicMax(id + px, uint32_t(t + 1));
+          }
+        } else {
+          const uint64_t k = uint64_t(ord32(dep)) << 32 | uint64_t(t + 1);
           if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
-        } else if (k == key[px]) {
-          atomicMax(id + px, uint32_t(t + 1));
         }
-      } else {
-        const uint64_t k = uint64_t(ord32(dep)) << 32 | uint64_t(t + 1);
-        if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
       }
     }
   }
@@ -654,83 +747,101 @@ __global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, uint64_t ntr
 // win (keys only grow), every other one goes through atomicMax and is kept
 // when the value it replaced was not larger.  The kept fragments collect in
 // a workgroup list in LDS and are flushed with one global reservation per
-// workgroup; *counter[1] is set when a list overflows (LDS or global), in
-// which case the id pass has to walk the geometry instead.
+// workgroup round; overflow (LDS or global) sets bit 2 of hdr->overflow, in
+// which case the id pass walks the geometry instead.
 constexpr int kFragLds = 2048;
 
-__global__ __launch_bounds__(kThreads) void k_raster_frags(MeshView<double> m, uint64_t ntri, ViewDev<double> v,
-                                                           uint64_t* __restrict__ key, Frag* __restrict__ frags,
-                                                           uint32_t cap, uint32_t* __restrict__ counter) {
+__global__ __launch_bounds__(kThreads) void k_raster_frags(MeshView<double> m,
+                                                           const RasterJob<double>* __restrict__ rjobs, int njobs,
+                                                           FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
+                                                           Frag* __restrict__ frags, uint32_t cap) {
   __shared__ Frag s_frag[kFragLds];
   __shared__ uint32_t s_n, s_base;
-  if (threadIdx.x == 0) s_n = 0;
-  __syncthreads();
-  const uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
-  Projected<double> P;
-  if (t < ntri && ((m.edges[t] >> 15) & 1u) && setup_tri(m, t, v, P)) {
-    for (int32_t j = P.j0; j <= P.j1; j++) {
-      for (int32_t i = P.i0; i <= P.i1; i++) {
-        double w0, w1, w2;
-        if (!pixel_weights(v, P, i, j, w0, w1, w2)) continue;
-        const uint64_t px = uint64_t(j) * uint64_t(v.W) + uint64_t(i);
-        const uint64_t k = ord64(pixel_depth(P, w0, w1, w2));
-        if (k < key[px]) continue;
-        const uint64_t old = atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
-        if (k < old) continue;
-        const uint32_t o = atomicAdd(&s_n, 1u);
-        if (o < uint32_t(kFragLds)) {
-          Frag fr;
-          fr.px = uint32_t(px);
-          fr.id = uint32_t(t + 1);
-          fr.key = k;
-          s_frag[o] = fr;
+  const uint64_t nwork = hdr->nwork;
+  const uint64_t stride = uint64_t(gridDim.x) * kThreads;
+  // every thread runs the same number of rounds so the barriers match
+  for (uint64_t u0 = uint64_t(blockIdx.x) * kThreads; u0 < nwork; u0 += stride) {
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    // one image per workgroup round: its constants are uniform
+    const int jb = njobs > 1 ? find_rjob(rjobs, njobs, u0) : 0;
+    const uint64_t t = rjobs[jb].tri0 + (u0 + threadIdx.x - rjobs[jb].wk0);
+    if (t < rjobs[jb].tri1) {
+      const ViewDev<double> v = rjobs[jb].v;
+      const uint64_t px0 = rjobs[jb].px0;
+      Projected<double> P;
+      if (((m.edges[t] >> 15) & 1u) && setup_tri(m, t, v, P)) {
+        for (int32_t j = P.j0; j <= P.j1; j++) {
+          for (int32_t i = P.i0; i <= P.i1; i++) {
+            double w0, w1, w2;
+            if (!pixel_weights(v, P, i, j, w0, w1, w2)) continue;
+            const uint64_t px = px0 + uint64_t(j) * uint64_t(v.W) + uint64_t(i);
+            const uint64_t k = ord64(pixel_depth(P, w0, w1, w2));
+            if (k < key[px]) continue;
+            const uint64_t old = atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+            if (k < old) continue;
+            const uint32_t o = atomicAdd(&s_n, 1u);
+            if (o < uint32_t(kFragLds)) {
+              Frag fr;
+              fr.px = uint32_t(px);
+              fr.id = uint32_t(t + 1);
+              fr.key = k;
+              s_frag[o] = fr;
+            }
+          }
         }
       }
     }
+    __syncthreads();
+    const uint32_t n = s_n < uint32_t(kFragLds) ? s_n : uint32_t(kFragLds);
+    if (threadIdx.x == 0) {
+      s_base = n ? atomicAdd(&hdr->nfrag, n) : 0u;
+      if (s_n > uint32_t(kFragLds) || (n && uint64_t(s_base) + n > cap)) atomicOr(&hdr->overflow, 2u);
+    }
+    __syncthreads();
+    const uint32_t b0 = s_base;
+    for (uint32_t q = threadIdx.x; q < n; q += kThreads) {
+      if (uint64_t(b0) + q < cap) frags[b0 + q] = s_frag[q];
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  const uint32_t n = s_n < uint32_t(kFragLds) ? s_n : uint32_t(kFragLds);
-  if (threadIdx.x == 0) {
-    s_base = n ? atomicAdd(counter, n) : 0u;
-    if (s_n > uint32_t(kFragLds)) atomicOr(counter + 1, 1u);
+}
+
+// the latest triangle among the kept fragments at each pixel's final depth
+__global__ void k_frag_ids(const Frag* __restrict__ frags, uint32_t cap, const FrameHdr* __restrict__ hdr,
+                           const uint64_t* __restrict__ key, uint32_t* __restrict__ id) {
+  if (hdr->overflow & 2u) return;
+  const uint32_t n = min(hdr->nfrag, cap);
+  for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < n; f += gridDim.x * blockDim.x) {
+    const Frag fr = frags[f];
+    if (fr.key == key[fr.px]) atomicMax(id + fr.px, fr.id);
   }
-  __syncthreads();
-  const uint32_t b0 = s_base;
-  if (threadIdx.x == 0 && n && uint64_t(b0) + n > cap) atomicOr(counter + 1, 1u);
-  for (uint32_t q = threadIdx.x; q < n; q += kThreads) {
-    if (uint64_t(b0) + q < cap) frags[b0 + q] = s_frag[q];
+}
+
+// also clears the frame header's raster state (fragment count, list overflow)
+__global__ void k_zinit(uint64_t* __restrict__ key, uint32_t* __restrict__ id, uint64_t npx, uint64_t k0,
+                        FrameHdr* __restrict__ hdr) {
+  if (hdr && blockIdx.x == 0 && threadIdx.x == 0) {
+    hdr->nfrag = 0;
+    hdr->overflow &= 1u;
+  }
+  for (uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < npx; p += uint64_t(gridDim.x) * blockDim.x) {
+    key[p] = k0;
+    if (id) id[p] = 0;
   }
 }
 
-// the latest triangle among the fragments at each pixel's final depth
-__global__ void k_frag_ids(const Frag* __restrict__ frags, uint32_t n, const uint64_t* __restrict__ key,
-                           uint32_t* __restrict__ id) {
-  const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= n) return;
-  const Frag fr = frags[f];
-  if (fr.key == key[fr.px]) atomicMax(id + fr.px, fr.id);
-}
-
-__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t val, uint64_t n) {
-  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = val;
-}
-
-__global__ void k_zinit(uint64_t* __restrict__ key, uint32_t* __restrict__ id, uint64_t npx, uint64_t k0) {
-  const uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= npx) return;
-  key[p] = k0;
-  if (id) id[p] = 0;
-}
-
-__global__ void k_line_frags(const uint32_t* __restrict__ pix, const double* __restrict__ depth, uint32_t nfrag,
-                             uint32_t id_base, int pass, int fp32, uint64_t* __restrict__ key,
+// Line fragments (pixels and depths computed on the host, in drawing
+// order) take part in the same z-test; their ids follow every triangle's.
+__global__ void k_line_frags(const uint32_t* __restrict__ pix, const double* __restrict__ depth,
+                             const FrameHdr* __restrict__ hdr, int pass, int fp32, uint64_t* __restrict__ key,
                              uint32_t* __restrict__ id) {
   const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nfrag) return;
+  if (f >= hdr->nline) return;
   const uint32_t px = pix[f];
+  const uint32_t lid = uint32_t(hdr->ntri + 1 + f);
   if (fp32) {
-    const uint64_t k = uint64_t(ord32(float(depth[f]))) << 32 | uint64_t(id_base + f);
+    const uint64_t k = uint64_t(ord32(float(depth[f]))) << 32 | uint64_t(lid);
     atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
     return;
   }
@@ -738,7 +849,7 @@ __global__ void k_line_frags(const uint32_t* __restrict__ pix, const double* __r
   if (pass == 0) {
     atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
   } else if (k == __hip_atomic_load(key + px, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-    atomicMax(id + px, id_base + f);
+    atomicMax(id + px, lid);
   }
 }
 
@@ -748,67 +859,211 @@ __global__ void k_line_frags(const uint32_t* __restrict__ pix, const double* __r
 // those triangles alone, with the same sums.  rgb rows are written top-down
 // (y index H-1 first) as the PPM writer packs them (output.c:113-123).
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_resolve(MeshView<R> m, uint64_t ntri, ViewDev<R> v,
-                                                      const ObjMeta* __restrict__ meta, int nobj,
-                                                      const uint8_t* __restrict__ frag_rgb, uchar4 bg,
+__global__ __launch_bounds__(kThreads) void k_resolve(MeshView<R> m, const RasterJob<R>* __restrict__ rjobs, int njobs,
+                                                      uint64_t npx_total, const ObjMeta* __restrict__ meta, int nobj,
+                                                      const uint8_t* __restrict__ frag_rgb,
+                                                      const FrameHdr* __restrict__ hdr,
                                                       const uint64_t* __restrict__ key,
-                                                      const uint32_t* __restrict__ idbuf, uint8_t* __restrict__ rgb,
-                                                      double* __restrict__ depth) {
-  const uint64_t p = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
-  const uint64_t npx = uint64_t(v.W) * uint64_t(v.H);
-  if (p >= npx) return;
-  const int32_t j = int32_t(p / uint64_t(v.W));
-  const int32_t i = int32_t(p - uint64_t(j) * uint64_t(v.W));
-  const uint64_t k = key[p];
-  uint32_t id;
-  double dep;
-  if constexpr (sizeof(R) == 8) {
-    id = idbuf[p];
-    dep = unord64(k);
-  } else {
-    id = uint32_t(k);
-    dep = id ? double(unord32(uint32_t(k >> 32))) : -DBL_MAX;
-  }
-  uint8_t r = bg.x, gg = bg.y, b = bg.z;
-  if (id != 0 && id <= ntri) {
-    const uint64_t t = id - 1;
-    int lo = 0, hi = nobj - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (meta[mid].base <= t) lo = mid;
-      else hi = mid - 1;
+                                                      const uint32_t* __restrict__ idbuf) {
+  const uint64_t ntri = hdr->ntri;
+  for (uint64_t p0 = uint64_t(blockIdx.x) * kThreads; p0 < npx_total; p0 += uint64_t(gridDim.x) * kThreads) {
+    // images start on workgroup boundaries of the pixel space, so the
+    // image of a workgroup round is uniform
+    int jb = 0;
+    if (njobs > 1) {
+      int lo = 0, hi = njobs - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (rjobs[mid].px0 <= p0) lo = mid;
+        else hi = mid - 1;
+      }
+      jb = lo;
     }
-    const ObjMeta om = meta[lo];
-    Projected<R> P;
-    setup_tri(m, t, v, P);
-    R w0, w1, w2;
-    pixel_weights(v, P, i, j, w0, w1, w2);
-    const uint32_t c = m.cell[t];
-    const uint32_t ed = m.edges[t];
-    V3<R> nv[3];
+    const RasterJob<R>& J = rjobs[jb];
+    const ViewDev<R> v = J.v;
+    const uint64_t p = p0 + threadIdx.x;
+    const uint64_t lp = p - J.px0;
+    if (lp >= uint64_t(v.W) * uint64_t(v.H)) continue;
+    const int32_t j = int32_t(lp / uint64_t(v.W));
+    const int32_t i = int32_t(lp - uint64_t(j) * uint64_t(v.W));
+    const uint64_t k = key[p];
+    uint32_t id;
+    double dep;
+    if constexpr (sizeof(R) == 8) {
+      id = idbuf[p];
+      dep = unord64(k);
+    } else {
+      id = uint32_t(k);
+      dep = id ? double(unord32(uint32_t(k >> 32))) : -DBL_MAX;
+    }
+    uint8_t r = J.bg[0], gg = J.bg[1], b = J.bg[2];
+    if (id != 0 && id <= ntri) {
+      const uint64_t t = id - 1;
+      int lo = 0, hi = nobj - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (meta[mid].base <= t) lo = mid;
+        else hi = mid - 1;
+      }
+      const ObjMeta om = meta[lo];
+      Projected<R> P;
+      setup_tri(m, t, v, P);
+      R w0, w1, w2;
+      pixel_weights(v, P, i, j, w0, w1, w2);
+      const uint32_t c = m.cell[t];
+      const uint32_t ed = m.edges[t];
+      V3<R> nv[3];
 #pragma unroll 1
-    for (int n = 0; n < 3; n++) nv[n] = vertex_normal<R>(m, om, c, (ed >> (5 * n)) & 31u);
-    const V3<R> n = vunit(vadd(vscale(w0, nv[0]), vadd(vscale(w1, nv[1]), vscale(w2, nv[2]))));
-    const V3<R> light = {v.light[0], v.light[1], v.light[2]};
-    const R fct = fmax(R(0.15), fabs(vdot(n, light)));
-    r = uint8_t(fct * R(om.rgb[0]));
-    gg = uint8_t(fct * R(om.rgb[1]));
-    b = uint8_t(fct * R(om.rgb[2]));
-  } else if (id > ntri) {
-    const uint64_t f = id - ntri - 1;
-    r = frag_rgb[3 * f + 0];
-    gg = frag_rgb[3 * f + 1];
-    b = frag_rgb[3 * f + 2];
+      for (int n = 0; n < 3; n++) nv[n] = vertex_normal<R>(m, om, c, (ed >> (5 * n)) & 31u);
+      const V3<R> n = vunit(vadd(vscale(w0, nv[0]), vadd(vscale(w1, nv[1]), vscale(w2, nv[2]))));
+      const V3<R> light = {v.light[0], v.light[1], v.light[2]};
+      const R fct = fmax(R(0.15), fabs(vdot(n, light)));
+      r = uint8_t(fct * R(om.rgb[0]));
+      gg = uint8_t(fct * R(om.rgb[1]));
+      b = uint8_t(fct * R(om.rgb[2]));
+    } else if (id > ntri) {
+      const uint64_t f = id - ntri - 1;
+      r = frag_rgb[3 * f + 0];
+      gg = frag_rgb[3 * f + 1];
+      b = frag_rgb[3 * f + 2];
+    }
+    const uint64_t o = (uint64_t(v.H - 1 - j) * uint64_t(v.W) + uint64_t(i)) * 3;
+    J.rgb[o + 0] = r;
+    J.rgb[o + 1] = gg;
+    J.rgb[o + 2] = b;
+    if (J.depth) J.depth[lp] = dep;
   }
-  const uint64_t o = (uint64_t(v.H - 1 - j) * uint64_t(v.W) + uint64_t(i)) * 3;
-  rgb[o + 0] = r;
-  rgb[o + 1] = gg;
-  rgb[o + 2] = b;
-  if (depth) depth[p] = dep;
 }
 
 // ----------------------------------------------------------------------
 // host launchers
+
+int persistent_blocks() {
+  static int n = 0;
+  if (n == 0) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess || cus <= 0)
+      cus = 256;
+    n = ((cus * 8 + 7) / 8) * 8;
+  }
+  return n;
+}
+
+static uint32_t grid8(uint64_t want, uint32_t cap) {
+  uint64_t g = want < cap ? want : cap;
+  if (g == 0) g = 1;
+  return uint32_t((g + 7) / 8 * 8);
+}
+
+hipError_t launch_count(const TessJob* jobs, int njobs, uint32_t ntiles, uint32_t* cnt, uint32_t grid,
+                        hipStream_t st) {
+  if (ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count, dim3(grid8(ntiles, grid)), dim3(kThreads), 0, st, jobs, njobs, ntiles, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(const uint32_t* cnt, uint64_t* start, uint64_t n, uint64_t* tile_tmp, hipStream_t st) {
+  const uint64_t ntile = (n + kScanTile - 1) / kScanTile;
+  if (ntile == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(uint32_t(ntile)), dim3(kThreads), 0, st, cnt, n, tile_tmp);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kThreads), 0, st, tile_tmp, ntile);
+  hipLaunchKernelGGL(k_scan_apply, dim3(uint32_t(ntile)), dim3(kThreads), 0, st, cnt, n,
+                     (const uint64_t*)tile_tmp, start);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_plan(const uint64_t* start, uint64_t ntiles, ObjMeta* meta, int nobj, RasterJob<R>* rjobs,
+                       int njobs, FrameHdr* hdr, uint64_t cap, hipStream_t st) {
+  hipLaunchKernelGGL(k_plan<R>, dim3(1), dim3(kThreads), 0, st, start, ntiles, meta, nobj, rjobs, njobs, hdr, cap);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_emit(const TessJob* jobs, int njobs, uint32_t ntiles, const uint64_t* start, const MeshView<R>& m,
+                       FrameHdr* hdr, uint32_t grid, hipStream_t st) {
+  if (ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_emit<R>, dim3(grid8(ntiles, grid)), dim3(kThreads), 0, st, jobs, njobs, ntiles, start, m, hdr);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, uint64_t count, double* out,
+                                 hipStream_t st) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vertex_normals<R>, dim3(uint32_t((count + kThreads - 1) / kThreads)), dim3(kThreads), 0, st,
+                     m, meta_dev, count, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_zinit(uint64_t* key, uint32_t* id, uint64_t npx, int precision, FrameHdr* hdr, hipStream_t st) {
+  const uint64_t k0 = precision == 0 ? 0x0010000000000000ull /* ord64(-DBL_MAX) */ : 0ull;
+  hipLaunchKernelGGL(k_zinit, dim3(grid8((npx + 255) / 256, 4096)), dim3(256), 0, st, key, id, npx, k0, hdr);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_raster(const MeshView<R>& m, const RasterJob<R>* rjobs, int njobs, const FrameHdr* hdr, int pass,
+                         uint64_t* key, uint32_t* id, uint32_t grid, hipStream_t st) {
+  if (pass == 0)
+    hipLaunchKernelGGL((k_raster<R, 0>), dim3(grid), dim3(kThreads), 0, st, m, rjobs, njobs, hdr, key, id);
+  else
+    hipLaunchKernelGGL((k_raster<R, 1>), dim3(grid), dim3(kThreads), 0, st, m, rjobs, njobs, hdr, key, id);
+  return hipGetLastError();
+}
+
+hipError_t launch_raster_frags(const MeshView<double>& m, const RasterJob<double>* rjobs, int njobs, FrameHdr* hdr,
+                               uint64_t* key, Frag* frags, uint32_t cap, uint32_t grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_raster_frags, dim3(grid), dim3(kThreads), 0, st, m, rjobs, njobs, hdr, key, frags, cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_frag_ids(const Frag* frags, uint32_t cap, const FrameHdr* hdr, const uint64_t* key, uint32_t* id,
+                           uint32_t grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_frag_ids, dim3(grid), dim3(256), 0, st, frags, cap, hdr, key, id);
+  return hipGetLastError();
+}
+
+hipError_t launch_line_frags(const uint32_t* pix, const double* depth, uint32_t cap, const FrameHdr* hdr, int pass,
+                             int precision, uint64_t* key, uint32_t* id, hipStream_t st) {
+  if (cap == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_line_frags, dim3((cap + 255) / 256), dim3(256), 0, st, pix, depth, hdr, pass, precision, key,
+                     id);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_resolve(const MeshView<R>& m, const RasterJob<R>* rjobs, int njobs, uint64_t npx_total,
+                          const ObjMeta* meta, int nobj, const uint8_t* frag_rgb, const FrameHdr* hdr,
+                          const uint64_t* key, const uint32_t* id, uint32_t grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_resolve<R>, dim3(grid8((npx_total + kThreads - 1) / kThreads, grid)), dim3(kThreads), 0, st,
+                     m, rjobs, njobs, npx_total, meta, nobj, frag_rgb, hdr, key, id);
+  return hipGetLastError();
+}
+
+template hipError_t launch_plan<double>(const uint64_t*, uint64_t, ObjMeta*, int, RasterJob<double>*, int, FrameHdr*,
+                                        uint64_t, hipStream_t);
+template hipError_t launch_plan<float>(const uint64_t*, uint64_t, ObjMeta*, int, RasterJob<float>*, int, FrameHdr*,
+                                       uint64_t, hipStream_t);
+template hipError_t launch_emit<double>(const TessJob*, int, uint32_t, const uint64_t*, const MeshView<double>&,
+                                        FrameHdr*, uint32_t, hipStream_t);
+template hipError_t launch_emit<float>(const TessJob*, int, uint32_t, const uint64_t*, const MeshView<float>&,
+                                       FrameHdr*, uint32_t, hipStream_t);
+template hipError_t launch_vertex_normals<double>(const MeshView<double>&, const ObjMeta*, uint64_t, double*,
+                                                  hipStream_t);
+template hipError_t launch_vertex_normals<float>(const MeshView<float>&, const ObjMeta*, uint64_t, double*,
+                                                 hipStream_t);
+template hipError_t launch_raster<double>(const MeshView<double>&, const RasterJob<double>*, int, const FrameHdr*,
+                                          int, uint64_t*, uint32_t*, uint32_t, hipStream_t);
+template hipError_t launch_raster<float>(const MeshView<float>&, const RasterJob<float>*, int, const FrameHdr*, int,
+                                         uint64_t*, uint32_t*, uint32_t, hipStream_t);
+template hipError_t launch_resolve<double>(const MeshView<double>&, const RasterJob<double>*, int, uint64_t,
+                                           const ObjMeta*, int, const uint8_t*, const FrameHdr*, const uint64_t*,
+                                           const uint32_t*, uint32_t, hipStream_t);
+template hipError_t launch_resolve<float>(const MeshView<float>&, const RasterJob<float>*, int, uint64_t,
+                                          const ObjMeta*, int, const uint8_t*, const FrameHdr*, const uint64_t*,
+                                          const uint32_t*, uint32_t, hipStream_t);
 
 hipError_t upload_tables(const NbrTable& nt) {
   // case table of triangulate.c:204-270: reduced mask -> triangles as
@@ -931,120 +1186,5 @@ NbrTable build_nbr_table() {
   }
   return nt;
 }
-
-static dim3 tile_grid(uint32_t nblocks) { return dim3(8 * ((nblocks + 7) / 8)); }
-
-hipError_t launch_count(const FieldDev& f, const ThrSet& ts, uint32_t* cnt, hipStream_t st) {
-  hipLaunchKernelGGL(k_count, tile_grid(f.nblocks), dim3(kThreads), 0, st, f, ts, cnt);
-  return hipGetLastError();
-}
-
-hipError_t launch_scan(const uint32_t* cnt, uint64_t* start, uint64_t n, uint64_t* tile_tmp, hipStream_t st) {
-  const uint64_t ntile = (n + kScanTile - 1) / kScanTile;
-  if (ntile == 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(uint32_t(ntile)), dim3(kThreads), 0, st, cnt, n, tile_tmp);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kThreads), 0, st, tile_tmp, ntile);
-  hipLaunchKernelGGL(k_scan_apply, dim3(uint32_t(ntile)), dim3(kThreads), 0, st, cnt, n,
-                     (const uint64_t*)tile_tmp, start);
-  return hipGetLastError();
-}
-
-hipError_t launch_gather_u64(const uint64_t* src, const uint32_t* idx, uint64_t* dst, int n, hipStream_t st) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gather_u64, dim3((n + 255) / 256), dim3(256), 0, st, src, idx, dst, n);
-  return hipGetLastError();
-}
-
-template <typename R>
-hipError_t launch_emit(const FieldDev& f, const ThrSet& ts, const uint64_t* start, const MeshView<R>& m,
-                       hipStream_t st) {
-  hipLaunchKernelGGL(k_emit<R>, tile_grid(f.nblocks), dim3(kThreads), 0, st, f, ts, start, m);
-  return hipGetLastError();
-}
-
-template <typename R>
-hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, uint64_t count, double* out,
-                                 hipStream_t st) {
-  if (count == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_vertex_normals<R>, dim3(uint32_t((count + kThreads - 1) / kThreads)), dim3(kThreads), 0, st,
-                     m, meta_dev, count, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_raster_frags(const MeshView<double>& m, uint64_t ntri, const ViewDev<double>& v, uint64_t* key,
-                               Frag* frags, uint32_t cap, uint32_t* counter, hipStream_t st) {
-  if (ntri == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_raster_frags, dim3(uint32_t((ntri + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, m,
-                     ntri, v, key, frags, cap, counter);
-  return hipGetLastError();
-}
-
-hipError_t launch_frag_ids(const Frag* frags, uint32_t n, const uint64_t* key, uint32_t* id, hipStream_t st) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_frag_ids, dim3((n + 255) / 256), dim3(256), 0, st, frags, n, key, id);
-  return hipGetLastError();
-}
-
-hipError_t launch_fill_u32(uint32_t* p, uint32_t val, uint64_t n, hipStream_t st) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fill_u32, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, st, p, val, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_zinit(uint64_t* key, uint32_t* id, uint64_t npx, int precision, hipStream_t st) {
-  const uint64_t k0 = precision == 0 ? 0x0010000000000000ull /* ord64(-DBL_MAX) */ : 0ull;
-  hipLaunchKernelGGL(k_zinit, dim3(uint32_t((npx + 255) / 256)), dim3(256), 0, st, key, id, npx, k0);
-  return hipGetLastError();
-}
-
-template <typename R>
-hipError_t launch_raster(const MeshView<R>& m, uint64_t ntri, const ViewDev<R>& v, int pass, uint64_t* key,
-                         uint32_t* id, hipStream_t st) {
-  if (ntri == 0) return hipSuccess;
-  const dim3 grid(uint32_t((ntri + kThreads - 1) / kThreads));
-  if (pass == 0)
-    hipLaunchKernelGGL((k_raster<R, 0>), grid, dim3(kThreads), 0, st, m, ntri, v, key, id);
-  else
-    hipLaunchKernelGGL((k_raster<R, 1>), grid, dim3(kThreads), 0, st, m, ntri, v, key, id);
-  return hipGetLastError();
-}
-
-hipError_t launch_line_frags(const uint32_t* pix, const double* depth, uint32_t nfrag, uint32_t id_base, int pass,
-                             int precision, uint64_t* key, uint32_t* id, hipStream_t st) {
-  if (nfrag == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_line_frags, dim3((nfrag + 255) / 256), dim3(256), 0, st, pix, depth, nfrag, id_base, pass,
-                     precision, key, id);
-  return hipGetLastError();
-}
-
-template <typename R>
-hipError_t launch_resolve(const MeshView<R>& m, uint64_t ntri, const ViewDev<R>& v, const ObjMeta* meta, int nobj,
-                          const uint8_t* frag_rgb, const uint8_t bg[3], const uint64_t* key, const uint32_t* id,
-                          uint8_t* rgb, double* depth, hipStream_t st) {
-  const uint64_t npx = uint64_t(v.W) * uint64_t(v.H);
-  const uchar4 b = make_uchar4(bg[0], bg[1], bg[2], 0);
-  hipLaunchKernelGGL(k_resolve<R>, dim3(uint32_t((npx + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, m, ntri,
-                     v, meta, nobj, frag_rgb, b, key, id, rgb, depth);
-  return hipGetLastError();
-}
-
-template hipError_t launch_emit<double>(const FieldDev&, const ThrSet&, const uint64_t*, const MeshView<double>&,
-                                        hipStream_t);
-template hipError_t launch_emit<float>(const FieldDev&, const ThrSet&, const uint64_t*, const MeshView<float>&,
-                                       hipStream_t);
-template hipError_t launch_vertex_normals<double>(const MeshView<double>&, const ObjMeta*, uint64_t, double*,
-                                                  hipStream_t);
-template hipError_t launch_vertex_normals<float>(const MeshView<float>&, const ObjMeta*, uint64_t, double*,
-                                                 hipStream_t);
-template hipError_t launch_raster<double>(const MeshView<double>&, uint64_t, const ViewDev<double>&, int, uint64_t*,
-                                          uint32_t*, hipStream_t);
-template hipError_t launch_raster<float>(const MeshView<float>&, uint64_t, const ViewDev<float>&, int, uint64_t*,
-                                         uint32_t*, hipStream_t);
-template hipError_t launch_resolve<double>(const MeshView<double>&, uint64_t, const ViewDev<double>&,
-                                           const ObjMeta*, int, const uint8_t*, const uint8_t*, const uint64_t*,
-                                           const uint32_t*, uint8_t*, double*, hipStream_t);
-template hipError_t launch_resolve<float>(const MeshView<float>&, uint64_t, const ViewDev<float>&, const ObjMeta*,
-                                          int, const uint8_t*, const uint8_t*, const uint64_t*, const uint32_t*,
-                                          uint8_t*, double*, hipStream_t);
 
 }  // namespace c3d

@@ -71,6 +71,36 @@ extern int contour3d_amd_execute_rgb(
     int precision
 );
 
+/* ---- batches ------------------------------------------------------------ */
+
+/* One image of a batch: the arguments of contour3d_amd_execute_rgb. */
+typedef struct {
+  contour3d_vector_t camera_position;
+  contour3d_vector_t light_direction;
+  size_t screen_sizes[2];
+  contour3d_vector_t screen_center;
+  contour3d_vector_t screen_local[2];
+  contour3d_color_t bg_color;
+  size_t num_contours;
+  const contour3d_contour_obj_t * contour_objs;
+  size_t num_lines;
+  const contour3d_line_obj_t * line_objs;
+  uint8_t * rgb;    /* W*H*3, host or device */
+  double * depth;   /* W*H or NULL, host or device */
+} contour3d_amd_image_t;
+
+/*
+ * Render several images as one frame: every contour object of every image
+ * is tessellated by one pass of each tessellation kernel, and every image is
+ * rasterised and resolved by one launch per stage.  Each image is identical
+ * to what contour3d_amd_execute_rgb returns for it alone.  Images whose
+ * contour lists are equal (same arrays, lattices, converters, thresholds,
+ * colours, in order) share one tessellation: several cameras on one scene
+ * cost one tessellation.  Limits: fewer than 2^31 pixels and 2^32 - 2^24
+ * triangles per batch.
+ */
+extern int contour3d_amd_execute_batch(size_t num_images, const contour3d_amd_image_t * images, int precision);
+
 /* ---- stage-level mesh access (tessellation + smoothing) --------------- */
 
 typedef struct contour3d_amd_mesh contour3d_amd_mesh_t;

@@ -43,11 +43,18 @@ def test_struct_layouts_match_c(tmp_path):
     src.write_text(r'''
 #include <stdio.h>
 #include <stddef.h>
-#include "contour3d.h"
+#include "contour3d_amd.h"
 #define P(T, m) printf(#T "." #m " %zu\n", offsetof(T, m))
 int main(void) {
-  printf("vector %zu\ncolor %zu\ncontour %zu\nline %zu\n", sizeof(contour3d_vector_t),
-         sizeof(contour3d_color_t), sizeof(contour3d_contour_obj_t), sizeof(contour3d_line_obj_t));
+  printf("vector %zu\ncolor %zu\ncontour %zu\nline %zu\nimage %zu\n", sizeof(contour3d_vector_t),
+         sizeof(contour3d_color_t), sizeof(contour3d_contour_obj_t), sizeof(contour3d_line_obj_t),
+         sizeof(contour3d_amd_image_t));
+  P(contour3d_amd_image_t, camera_position); P(contour3d_amd_image_t, light_direction);
+  P(contour3d_amd_image_t, screen_sizes); P(contour3d_amd_image_t, screen_center);
+  P(contour3d_amd_image_t, screen_local); P(contour3d_amd_image_t, bg_color);
+  P(contour3d_amd_image_t, num_contours); P(contour3d_amd_image_t, contour_objs);
+  P(contour3d_amd_image_t, num_lines); P(contour3d_amd_image_t, line_objs);
+  P(contour3d_amd_image_t, rgb); P(contour3d_amd_image_t, depth);
   P(contour3d_contour_obj_t, glsizes); P(contour3d_contour_obj_t, grids);
   P(contour3d_contour_obj_t, converter); P(contour3d_contour_obj_t, threshold);
   P(contour3d_contour_obj_t, color); P(contour3d_contour_obj_t, array);
@@ -71,6 +78,9 @@ int main(void) {
         assert got[f"contour3d_contour_obj_t.{f}"] == getattr(_abi.ContourObj, f).offset, f
     for f in ("nitems", "converter", "color", "width"):
         assert got[f"contour3d_line_obj_t.{f}"] == getattr(_abi.LineObj, f).offset, f
+    assert got["image"] == C.sizeof(_abi.Image)
+    for f, _ in _abi.Image._fields_:
+        assert got[f"contour3d_amd_image_t.{f}"] == getattr(_abi.Image, f).offset, f
 
 
 def test_entry_points_reject_bad_arguments_without_touching_memory():

@@ -1,0 +1,124 @@
+"""GPU: contour3d_amd_execute_batch.  Every image of a batch must be
+bit-identical (pixels and z-buffer) to the same image rendered alone by
+contour3d_amd_execute_rgb, in both arithmetic forms, whatever else shares
+its frame; FP64 images must also equal the CPU oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from contour3d_amd import api, fields
+
+from test_gpu_parity import EDGE_LINES, IMAGES, same_bits, view_for
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def alone(im, precision):
+    return api.execute_rgb(im["camera"], im["light"], im["size"], im["center"], im["local"], im["bg"],
+                           im["contours"], im.get("lines", ()), precision=precision, want_depth=True)
+
+
+def uneven_batch():
+    out = []
+    for name, contours, lines, size in IMAGES:
+        im = view_for(*size)
+        im.update(contours=contours, lines=lines, want_depth=True)
+        out.append(im)
+    sv = fields.sphere_view(0.9, 1.8, 72, 64)
+    out.append(dict(camera=sv["camera"], light=sv["light"], size=sv["size"], center=sv["center"],
+                    local=sv["local"], bg=(9, 9, 9), want_depth=True,
+                    contours=[dict(field=fields.blobs((25, 21, 23), seed=7), threshold=0.45, color=(30, 60, 90))],
+                    lines=[]))
+    # an image with lines only
+    im = view_for(64, 48)
+    im.update(contours=[], lines=EDGE_LINES, want_depth=True)
+    out.append(im)
+    return out
+
+
+@pytest.mark.parametrize("precision", [api.FP64, api.FP32], ids=["fp64", "fp32"])
+def test_uneven_batch_equals_images_alone(orc, precision):
+    banner(f"batch uneven precision={precision}")
+    batch = uneven_batch()
+    got = api.execute_batch(batch, precision=precision)
+    for q, im in enumerate(batch):
+        rgb, dep = alone(im, precision)
+        assert np.array_equal(got[q][0], rgb), q
+        assert same_bits(got[q][1], dep), q
+        if precision == api.FP64:
+            ref_rgb, ref_dep = orc.execute(im["camera"], im["light"], im["size"], im["center"], im["local"],
+                                           im["bg"], im["contours"], im.get("lines", ()))
+            assert np.array_equal(got[q][0], ref_rgb), q
+            assert same_bits(got[q][1], ref_dep), q
+
+
+@pytest.mark.parametrize("precision", [api.FP64, api.FP32], ids=["fp64", "fp32"])
+def test_cameras_sharing_one_scene_tessellate_once(orc, precision):
+    banner(f"batch shared scene precision={precision}")
+    f = fields.turbulence((30, 30, 30), kmax=8, seed=21)
+    scene = [dict(field=f, threshold=t, color=c) for t, c in ((-0.8, (255, 40, 40)), (0.6, (40, 40, 255)))]
+    batch = []
+    for q, (th, ph) in enumerate(((0.3, 0.4), (1.1, 2.0), (2.2, 4.1), (0.7, 5.5))):
+        sv = fields.sphere_view(th, ph, 80 + 16 * q, 60 + 8 * q)
+        batch.append(dict(camera=sv["camera"], light=sv["light"], size=sv["size"], center=sv["center"],
+                          local=sv["local"], bg=(0, 0, 0), contours=scene, want_depth=True))
+    api.profile_enable(True)
+    api.profile_reset()
+    got = api.execute_batch(batch, precision=precision)
+    _, _, ntri_batch = api.profile_read()
+    api.profile_reset()
+    rgb0, dep0 = alone(batch[0], precision)
+    _, _, ntri_one = api.profile_read()
+    api.profile_enable(False)
+    assert ntri_batch == ntri_one > 0
+    for q, im in enumerate(batch):
+        rgb, dep = (rgb0, dep0) if q == 0 else alone(im, precision)
+        assert np.array_equal(got[q][0], rgb), q
+        assert same_bits(got[q][1], dep), q
+
+
+def test_batch_device_outputs_and_capacity_changes():
+    banner("batch device outputs, growing and shrinking frames")
+    import torch
+
+    big = uneven_batch()
+    host = api.execute_batch(big, precision=api.FP64)
+    dev = []
+    for im in big:
+        W, H = im["size"]
+        d = dict(im)
+        d["rgb"] = torch.zeros((H, W, 3), dtype=torch.uint8, device="cuda")
+        d["depth"] = torch.zeros((H, W), dtype=torch.float64, device="cuda")
+        dev.append(d)
+    api.execute_batch(dev, precision=api.FP64)
+    torch.cuda.synchronize()
+    for q, d in enumerate(dev):
+        assert np.array_equal(d["rgb"].cpu().numpy(), host[q][0]), q
+        assert same_bits(d["depth"].cpu().numpy(), host[q][1]), q
+    # a smaller frame after a larger one, then the larger one again
+    small = api.execute_batch(big[:1], precision=api.FP64)
+    assert np.array_equal(small[0][0], host[0][0])
+    again = api.execute_batch(big, precision=api.FP64)
+    for q in range(len(big)):
+        assert np.array_equal(again[q][0], host[q][0]), q
+        assert same_bits(again[q][1], host[q][1]), q
+
+
+def test_batch_errors():
+    banner("batch errors")
+    v = view_for(16, 16)
+    bad = dict(v, contours=[dict(field=np.zeros((1, 4, 4)), threshold=0.0, color=(1, 1, 1))])
+    good = dict(v, contours=[dict(field=fields.blobs((10, 10, 10), seed=3), threshold=0.5, color=(1, 1, 1))])
+    with pytest.raises(RuntimeError):
+        api.execute_batch([good, bad])
+    with pytest.raises(RuntimeError):
+        api.execute_batch([good, dict(v, size=(0, 16), contours=[])])
+    assert api.execute_batch([]) == []
+    out = api.execute_batch([good])
+    assert out[0][0].shape == (16, 16, 3)
