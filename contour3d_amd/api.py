@@ -35,6 +35,7 @@ EXPORTS = (
     "contour3d_amd_mesh_copy",
     "contour3d_amd_mesh_render",
     "contour3d_amd_mesh_free",
+    "contour3d_amd_tune",
     "contour3d_amd_profile_enable",
     "contour3d_amd_profile_read",
     "contour3d_amd_profile_reset",
@@ -68,6 +69,7 @@ def lib():
                                                 C.POINTER(_abi.LineObj), C.c_void_p, C.c_void_p, C.c_int]
         L.contour3d_amd_execute_batch.restype = C.c_int
         L.contour3d_amd_execute_batch.argtypes = [C.c_size_t, C.POINTER(_abi.Image), C.c_int]
+        L.contour3d_amd_tune.argtypes = [C.c_int, C.c_int64]
         L.contour3d_amd_init.argtypes = [C.c_int]
         L.contour3d_amd_set_precision.argtypes = [C.c_int]
         L.contour3d_amd_mesh_build.argtypes = [C.POINTER(_abi.ContourObj), C.c_size_t, C.c_int,
@@ -319,6 +321,18 @@ def profile_read():
     nt = C.c_uint64()
     lib().contour3d_amd_profile_read(ms.ctypes.data, calls.ctypes.data, C.byref(nt))
     return dict(zip(STAGES, ms)), dict(zip(STAGES, calls)), nt.value
+
+
+TUNE_TESS_GRID = 0
+TUNE_RASTER_GRID = 1
+TUNE_FRAG_CAPACITY = 2
+
+
+def tune(param, value):
+    """contour3d_amd_tune: launch-shape and capacity parameters (results do
+    not depend on them)."""
+    if lib().contour3d_amd_tune(int(param), int(value)) != 0:
+        raise RuntimeError("contour3d_amd_tune failed")
 
 
 def set_precision(p):

@@ -20,7 +20,7 @@ OBJ = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libcontour3d.so")
 SOURCES = ["c3d_kernels.hip", "c3d_api.hip"]
-HEADERS = ["c3d_internal.h"]
+HEADERS = ["c3d_internal.h", "c3d_geom.h"]
 
 
 def hipcc():

@@ -178,8 +178,10 @@ struct Lib {
   std::vector<RasterJob<float>> h_rjobs32;
   uint32_t h_nline = 0;
   Frags h_frags;
-  // workgroups of the tessellation kernels: one per tile unless capped
+  // workgroups of the tessellation and raster kernels: one per tile or
+  // per 256 triangles unless capped
   uint32_t tess_cap = 0;
+  uint32_t raster_cap = 0;
   uint32_t tess_grid(uint32_t ntiles) const {
     const uint32_t g = tess_cap && tess_cap < ntiles ? tess_cap : ntiles;
     return g < 8 ? 8 : g;
@@ -818,7 +820,12 @@ int raster_frame(Lib& L, const contour3d_amd_mesh& M, std::vector<Image>& imgs, 
   C3D_HIP(hipMemcpyAsync(reinterpret_cast<unsigned char*>(hdr) + offsetof(FrameHdr, nline), &L.h_nline,
                          sizeof(uint32_t), hipMemcpyHostToDevice, L.st));
   const MeshView<R> mv = view_of<R>(M);
-  const uint32_t rgrid = uint32_t(std::min<uint64_t>((hh.nwork + kThreads - 1) / kThreads + 8, 1u << 30)) / 8 * 8;
+  // FP64 rasterises with persistent workgroups (64 per CU) that overlap the
+  // next round's loads with drawing; FP32 with one workgroup per round
+  uint64_t rg = (hh.nwork + kThreads - 1) / kThreads;
+  const uint64_t rcap = L.raster_cap ? L.raster_cap : (fp64 ? uint64_t(persistent_blocks()) * 8 : 0);
+  if (rcap && rg > rcap) rg = rcap;
+  const uint32_t rgrid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(rg, 1u << 30)));
   if (prof_begin(L, CONTOUR3D_AMD_STAGE_RASTER)) return 1;
   C3D_HIP(launch_zinit(key, id, npx, fp64 ? 0 : 1, hdr, L.st));
   if constexpr (sizeof(R) == 8) {
@@ -828,14 +835,16 @@ int raster_frame(Lib& L, const contour3d_amd_mesh& M, std::vector<Image>& imgs, 
     // next frame
     if (L.frag_cap == 0) L.frag_cap = 1u << 22;
     C3D_HIP(L.frags.ensure(size_t(L.frag_cap) * sizeof(Frag)));
-    C3D_HIP(launch_raster_frags(mv, rjobs, njobs, hdr, key, L.frags.as<Frag>(), L.frag_cap, rgrid, L.st));
+    C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, kRasterDepth64, key, id, L.frags.as<Frag>(), L.frag_cap, rgrid,
+                             L.st));
     C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, hdr, 0, 0, key, id, L.st));
     C3D_HIP(launch_frag_ids(L.frags.as<Frag>(), L.frag_cap, hdr, key, id,
                             std::min<uint32_t>((L.frag_cap + 255) / 256, 4096), L.st));
-    C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, 1, key, id, uint32_t(persistent_blocks()), L.st));
+    C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, kRasterId64, key, id, nullptr, 0, uint32_t(persistent_blocks()),
+                             L.st));
     C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, hdr, 1, 0, key, id, L.st));
   } else {
-    C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, 0, key, id, rgrid, L.st));
+    C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, kRasterFp32, key, id, nullptr, 0, rgrid, L.st));
     C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, hdr, 0, 1, key, id, L.st));
   }
   if (prof_end(L)) return 1;
@@ -910,6 +919,7 @@ int frame_setup(Lib& L, std::vector<Image>& imgs, uint64_t& npx) {
     return 1;
   }
   C3D_HIP(L.rgb.ensure(3 * npx + 1));
+
   bool any_dep = false;
   for (const Image& im : imgs) any_dep = any_dep || (im.depth && !im.dep_dev);
   if (any_dep) C3D_HIP(L.depth.ensure(npx * sizeof(double)));
@@ -1101,6 +1111,29 @@ int contour3d_amd_finalize(void) {
   L.st = nullptr;
   L.ready = false;
   return 0;
+}
+
+int contour3d_amd_tune(int param, int64_t value) {
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  if (value < 0 || value > int64_t(0xffffffffu)) {
+    c3d::log_error("tune: value %lld out of range", (long long)value);
+    return 1;
+  }
+  switch (param) {
+    case CONTOUR3D_AMD_TUNE_TESS_GRID:
+      L.tess_cap = uint32_t(value);
+      return 0;
+    case CONTOUR3D_AMD_TUNE_RASTER_GRID:
+      L.raster_cap = uint32_t(value);
+      return 0;
+    case CONTOUR3D_AMD_TUNE_FRAG_CAPACITY:
+      L.frag_cap = uint32_t(value ? value : 1);
+      return 0;
+    default:
+      c3d::log_error("tune: unknown parameter %d", param);
+      return 1;
+  }
 }
 
 int contour3d_amd_set_precision(int precision) {

@@ -1,0 +1,201 @@
+// Vector, projection and pixel-coverage arithmetic of the render stage,
+// shared by the device kernels and host-side checks (compiled by hipcc for
+// the device and by a plain C++ compiler for the CPU tests).  Every
+// expression keeps the reference's operation order; builds use
+// -ffp-contract=off, so no multiply-add is fused.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#define C3D_HD __host__ __device__ __forceinline__
+#else
+#define C3D_HD inline
+#endif
+
+namespace c3d {
+
+using std::fabs;
+using std::fmax;
+using std::fmin;
+using std::sqrt;
+
+template <typename R>
+struct V3 {
+  R x, y, z;
+};
+
+template <typename R>
+C3D_HD V3<R> vadd(V3<R> a, V3<R> b) {
+  return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+template <typename R>
+C3D_HD V3<R> vsub(V3<R> a, V3<R> b) {
+  return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+template <typename R>
+C3D_HD V3<R> vscale(R w, V3<R> a) {
+  return {w * a.x, w * a.y, w * a.z};
+}
+template <typename R>
+C3D_HD R vdot(V3<R> a, V3<R> b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <typename R>
+C3D_HD V3<R> vcross(V3<R> a, V3<R> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// reciprocal of the norm, then one multiply per component (vector.c:66-76)
+template <typename R>
+C3D_HD V3<R> vunit(V3<R> a) {
+  const R s = R(1) / sqrt(vdot(a, a));
+  return {a.x * s, a.y * s, a.z * s};
+}
+
+// Camera/screen constants; nc = n.camera, d = n.center, lxx = lx.lx,
+// lyy = ly.ly are formed on the host in FP64 exactly as project.c:34-36,
+// 62-63 form them per call.
+template <typename R>
+struct ViewDev {
+  R cam[3], ctr[3], lx[3], ly[3], nrm[3], light[3];
+  R nc, d, lxx, lyy;
+  R fw, fh;  // width, height as reals
+  int32_t W, H;
+};
+
+// perspective projection (project.c:9-67); false when rejected
+template <typename R>
+C3D_HD bool project(const ViewDev<R>& v, V3<R> p, V3<R>& q) {
+  const V3<R> cam = {v.cam[0], v.cam[1], v.cam[2]};
+  const V3<R> nrm = {v.nrm[0], v.nrm[1], v.nrm[2]};
+  const V3<R> ray = vsub(p, cam);
+  const R nr = vdot(nrm, ray);
+  if (R(-1.e-8) < nr && nr < R(1.e-8)) return false;
+  const R t = (v.d - v.nc) / nr;
+  if (t <= R(0) || R(1) <= t) return false;
+  const V3<R> delta = {v.cam[0] + t * ray.x - v.ctr[0], v.cam[1] + t * ray.y - v.ctr[1],
+                       v.cam[2] + t * ray.z - v.ctr[2]};
+  const V3<R> lx = {v.lx[0], v.lx[1], v.lx[2]};
+  const V3<R> ly = {v.ly[0], v.ly[1], v.ly[2]};
+  q.x = vdot(delta, lx) / v.lxx;
+  q.y = vdot(delta, ly) / v.lyy;
+  q.z = nr;
+  return true;
+}
+
+// 2-D edge function (render.c:38-46)
+template <typename R>
+C3D_HD R edge_fn(const V3<R>& a, const V3<R>& b, R cx, R cy) {
+  return (cx - a.x) * (b.y - a.y) - (cy - a.y) * (b.x - a.x);
+}
+
+// truncation toward zero of a bbox bound; the bound only matters after
+// clamping to [0, W-1], so it is first clamped into int range
+template <typename R>
+C3D_HD int32_t trunc_i(R x) {
+  x = fmin(fmax(x, R(-1.e9)), R(1.e9));
+  return int32_t(x);
+}
+
+C3D_HD int32_t clampi(int32_t lo, int32_t x, int32_t hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// pixel-centre coordinate of column i (render.c:91-94)
+template <typename R>
+C3D_HD R pixel_centre(int32_t i, R extent) {
+  return R(1) * (R(i) + R(0.5)) / extent - R(0.5);
+}
+
+// A projected triangle: vertices, 1 / (clamped) doubled area and the pixel
+// box its coverage test runs over.
+template <typename R>
+struct Projected {
+  V3<R> q[3];
+  R ainv;
+  int32_t i0, i1, j0, j1;
+};
+
+// The reference's bounding box and area of projected vertices P.q
+// (render.c:60-89); returns the unclamped doubled area.
+template <typename R>
+C3D_HD R raster_box(const ViewDev<R>& v, Projected<R>& P) {
+  const V3<R>* q = P.q;
+  const int32_t xmin = trunc_i<R>((R(0.5) + fmin(q[0].x, fmin(q[1].x, q[2].x))) * v.fw - R(1));
+  const int32_t xmax = trunc_i<R>((R(0.5) + fmax(q[0].x, fmax(q[1].x, q[2].x))) * v.fw + R(1));
+  const int32_t ymin = trunc_i<R>((R(0.5) + fmin(q[0].y, fmin(q[1].y, q[2].y))) * v.fh - R(1));
+  const int32_t ymax = trunc_i<R>((R(0.5) + fmax(q[0].y, fmax(q[1].y, q[2].y))) * v.fh + R(1));
+  P.i0 = clampi(0, xmin, v.W - 1);
+  P.i1 = clampi(0, xmax, v.W - 1);
+  P.j0 = clampi(0, ymin, v.H - 1);
+  P.j1 = clampi(0, ymax, v.H - 1);
+  const R area = edge_fn(q[0], q[1], q[2].x, q[2].y);
+  P.ainv = R(1) / (area < R(0) ? fmin(area, R(-1.e-16)) : fmax(area, R(1.e-16)));
+  return area;
+}
+
+// barycentric weights at the pixel centre (cx, cy); false when outside
+// (render.c:91-115)
+template <typename R>
+C3D_HD bool pixel_weights(const Projected<R>& P, R cx, R cy, R& w0, R& w1, R& w2) {
+  w0 = P.ainv * edge_fn(P.q[1], P.q[2], cx, cy);
+  w1 = P.ainv * edge_fn(P.q[2], P.q[0], cx, cy);
+  w2 = P.ainv * edge_fn(P.q[0], P.q[1], cx, cy);
+  return !(w0 < R(-1.e-8) || w1 < R(-1.e-8) || w2 < R(-1.e-8));
+}
+
+// harmonic depth (render.c:120-124)
+template <typename R>
+C3D_HD R pixel_depth(const Projected<R>& P, R w0, R w1, R w2) {
+  return R(1) / (w0 / P.q[0].z + w1 / P.q[1].z + w2 / P.q[2].z);
+}
+
+// Narrows P's box to the pixels whose centres pixel_weights can accept,
+// so the coverage loop skips the reference box's margin.  Pixel centres are
+// evaluated with pixel_centre, the expression the coverage loop uses.
+//
+// Bound: with exact weights w_i = E_i / A (sum 1, sum w_i q_i = p), a pixel
+// centre p farther than M to the left of every vertex has, for
+// d_i = q_i.x - p.x in (M, D], sum over negative w_i of |w_i| d_i equal to
+// the positive part, which is at least M; so some w_i < -M / (2D).  The
+// computed weights differ from the exact ones by at most 4 eps D / h (h a
+// lower bound of the triangle's heights) plus a relative 1e-3, so with
+// M = 2D (2e-8 + 32 eps D / h) (+ the rounding of the comparison bounds)
+// the computed test rejects p.  Same for the other three sides.  Thin or
+// tiny triangles (error bound above 1e-3, area near the 1e-16 clamp) and
+// non-finite values keep the reference box.
+template <typename R>
+C3D_HD void tight_box(const ViewDev<R>& v, R area, Projected<R>& P) {
+  const R eps = sizeof(R) == 8 ? R(2.220446049250313e-16) : R(1.1920929e-7);
+  const V3<R>* q = P.q;
+  const R minx = fmin(q[0].x, fmin(q[1].x, q[2].x)), maxx = fmax(q[0].x, fmax(q[1].x, q[2].x));
+  const R miny = fmin(q[0].y, fmin(q[1].y, q[2].y)), maxy = fmax(q[0].y, fmax(q[1].y, q[2].y));
+  const R pitch = fmax(R(1) / v.fw, R(1) / v.fh);
+  const R D = fmax(maxx - minx, maxy - miny) + R(4) * pitch;
+  const R emax = fmax(fmax(fmax(fabs(q[1].x - q[0].x), fabs(q[1].y - q[0].y)),
+                           fmax(fabs(q[2].x - q[1].x), fabs(q[2].y - q[1].y))),
+                      fmax(fabs(q[0].x - q[2].x), fabs(q[0].y - q[2].y)));
+  const R A = fabs(area);
+  if (!(A > R(1.e-14)) || !(emax > R(0))) return;
+  const R h = A / (R(1.5) * emax);
+  const R rel = R(16) * eps * D / h;
+  if (!(rel < R(1.e-3))) return;
+  const R qmax = fmax(fmax(fmax(fabs(minx), fabs(maxx)), fmax(fabs(miny), fabs(maxy))), R(1));
+  const R M = R(2) * D * (R(2.e-8) + R(2) * rel) + R(16) * eps * qmax;
+  if (!(M < pitch)) return;
+  const R xlo = minx - M, xhi = maxx + M, ylo = miny - M, yhi = maxy + M;
+  // first column with centre >= xlo: start at or below it and walk up
+  int32_t a = clampi(P.i0, trunc_i<R>((xlo + R(0.5)) * v.fw - R(0.5)) - 1, P.i1 + 1);
+  while (a <= P.i1 && pixel_centre<R>(a, v.fw) < xlo) a++;
+  int32_t b = clampi(a - 1, trunc_i<R>((xhi + R(0.5)) * v.fw - R(0.5)) + 2, P.i1);
+  while (b >= a && pixel_centre<R>(b, v.fw) > xhi) b--;
+  int32_t c = clampi(P.j0, trunc_i<R>((ylo + R(0.5)) * v.fh - R(0.5)) - 1, P.j1 + 1);
+  while (c <= P.j1 && pixel_centre<R>(c, v.fh) < ylo) c++;
+  int32_t d = clampi(c - 1, trunc_i<R>((yhi + R(0.5)) * v.fh - R(0.5)) + 2, P.j1);
+  while (d >= c && pixel_centre<R>(d, v.fh) > yhi) d--;
+  P.i0 = a;
+  P.i1 = b;
+  P.j0 = c;
+  P.j1 = d;
+}
+
+}  // namespace c3d

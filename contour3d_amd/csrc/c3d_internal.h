@@ -7,6 +7,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "c3d_geom.h"
+
 namespace c3d {
 
 constexpr int kThreads = 256;           // one workgroup = 4 wave64
@@ -92,17 +94,6 @@ struct NbrTable {
 };
 NbrTable build_nbr_table();
 
-// Camera/screen constants; nc = n.camera, d = n.center, lxx = lx.lx,
-// lyy = ly.ly are formed on the host in FP64 exactly as project.c:34-36,
-// 62-63 form them per call.
-template <typename R>
-struct ViewDev {
-  R cam[3], ctr[3], lx[3], ly[3], nrm[3], light[3];
-  R nc, d, lxx, lyy;
-  R fw, fh;          // width, height as reals
-  int32_t W, H;
-};
-
 // One image of a frame: its camera, its contour objects [obj0, obj1) (their
 // triangles are contiguous in the mesh; images showing the same objects
 // share them), its pixels in the frame's z-buffer from px0, and where its
@@ -157,18 +148,15 @@ hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, 
                                  double* out, hipStream_t st);
 hipError_t launch_zinit(uint64_t* key, uint32_t* id, uint64_t npx, int precision, FrameHdr* hdr,
                         hipStream_t st);
-// FP32: one pass over packed (depth << 32 | id) keys.  FP64 pass 1 over the
-// geometry (used when the fragment list of the depth pass overflowed).
+// raster modes: FP32 single pass over packed (depth << 32 | id) keys; FP64
+// depth pass keeping the fragments that may end up nearest; FP64 id pass
+// over the geometry (runs only when the fragment list overflowed)
+enum { kRasterFp32 = 0, kRasterDepth64 = 1, kRasterId64 = 2 };
 template <typename R>
-hipError_t launch_raster(const MeshView<R>& m, const RasterJob<R>* rjobs, int njobs, const FrameHdr* hdr,
-                         int pass, uint64_t* key, uint32_t* id, uint32_t grid, hipStream_t st);
-// FP64 depth pass that keeps the fragments that may end up nearest
-hipError_t launch_raster_frags(const MeshView<double>& m, const RasterJob<double>* rjobs, int njobs,
-                               FrameHdr* hdr, uint64_t* key, Frag* frags, uint32_t cap, uint32_t grid,
-                               hipStream_t st);
+hipError_t launch_raster(const MeshView<R>& m, const RasterJob<R>* rjobs, int njobs, FrameHdr* hdr, int mode,
+                         uint64_t* key, uint32_t* id, Frag* frags, uint32_t cap, uint32_t grid, hipStream_t st);
 // FP64 id pass over the fragment list (reads the count from hdr; does
-// nothing when the list overflowed, and the geometry id pass of
-// launch_raster(pass 1) then does the work)
+// nothing when the list overflowed)
 hipError_t launch_frag_ids(const Frag* frags, uint32_t cap, const FrameHdr* hdr, const uint64_t* key,
                            uint32_t* id, uint32_t grid, hipStream_t st);
 // line fragments [0, hdr->nline) of the frame; `cap` sizes the grid

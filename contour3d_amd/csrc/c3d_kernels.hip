@@ -47,44 +47,6 @@ __device__ __forceinline__ XcdRange xcd_range(uint32_t n) {
   return r;
 }
 
-template <typename R>
-struct V3 {
-  R x, y, z;
-};
-
-template <typename R>
-__device__ __forceinline__ V3<R> vadd(V3<R> a, V3<R> b) {
-  return {a.x + b.x, a.y + b.y, a.z + b.z};
-}
-template <typename R>
-__device__ __forceinline__ V3<R> vsub(V3<R> a, V3<R> b) {
-  return {a.x - b.x, a.y - b.y, a.z - b.z};
-}
-template <typename R>
-__device__ __forceinline__ V3<R> vscale(R w, V3<R> a) {
-  return {w * a.x, w * a.y, w * a.z};
-}
-template <typename R>
-__device__ __forceinline__ R vdot(V3<R> a, V3<R> b) {
-  return a.x * b.x + a.y * b.y + a.z * b.z;
-}
-template <typename R>
-__device__ __forceinline__ V3<R> vcross(V3<R> a, V3<R> b) {
-  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-// reciprocal of the norm, then one multiply per component (vector.c:66-76)
-template <typename R>
-__device__ __forceinline__ V3<R> vunit(V3<R> a) {
-  const R s = R(1) / sqrt(vdot(a, a));
-  return {a.x * s, a.y * s, a.z * s};
-}
-
-// ----------------------------------------------------------------------
-// tessellation tables
-//
-// Cube corner c = i + 2j + 4k.  The six tetrahedra share corners 0 and 7;
-// their other two corners, in tetra-vertex order 2 and 3, are (3,1) (1,5)
-// (5,4) (4,6) (6,2) (2,3) (triangulate.c:145-152), packed 6 bits per tetra.
 // ----------------------------------------------------------------------
 // block-wide exclusive scan of one value per thread (256 threads)
 
@@ -603,50 +565,11 @@ __device__ __forceinline__ float unord32(uint32_t k) {
   return __uint_as_float(u);
 }
 
-// perspective projection (project.c:9-67); false when rejected
+// loads triangle t, projects it and sets up its reference box; false when
+// a vertex is rejected by the projection
 template <typename R>
-__device__ __forceinline__ bool project(const ViewDev<R>& v, V3<R> p, V3<R>& q) {
-  const V3<R> cam = {v.cam[0], v.cam[1], v.cam[2]};
-  const V3<R> nrm = {v.nrm[0], v.nrm[1], v.nrm[2]};
-  const V3<R> ray = vsub(p, cam);
-  const R nr = vdot(nrm, ray);
-  if (R(-1.e-8) < nr && nr < R(1.e-8)) return false;
-  const R t = (v.d - v.nc) / nr;
-  if (t <= R(0) || R(1) <= t) return false;
-  const V3<R> delta = {v.cam[0] + t * ray.x - v.ctr[0], v.cam[1] + t * ray.y - v.ctr[1],
-                       v.cam[2] + t * ray.z - v.ctr[2]};
-  const V3<R> lx = {v.lx[0], v.lx[1], v.lx[2]};
-  const V3<R> ly = {v.ly[0], v.ly[1], v.ly[2]};
-  q.x = vdot(delta, lx) / v.lxx;
-  q.y = vdot(delta, ly) / v.lyy;
-  q.z = nr;
-  return true;
-}
-
-// 2-D edge function (render.c:38-46)
-template <typename R>
-__device__ __forceinline__ R edge_fn(const V3<R>& a, const V3<R>& b, R cx, R cy) {
-  return (cx - a.x) * (b.y - a.y) - (cy - a.y) * (b.x - a.x);
-}
-
-// truncation toward zero of a bbox bound; the bound only matters after
-// clamping to [0, W-1], so it is first clamped into int range
-template <typename R>
-__device__ __forceinline__ int32_t trunc_i(R x) {
-  x = fmin(fmax(x, R(-1.e9)), R(1.e9));
-  return int32_t(x);
-}
-
-template <typename R>
-struct Projected {
-  V3<R> q[3];
-  R ainv;
-  int32_t i0, i1, j0, j1;
-};
-
-template <typename R>
-__device__ __forceinline__ bool setup_tri(const MeshView<R>& m, uint64_t t, const ViewDev<R>& v,
-                                          Projected<R>& P) {
+__device__ __forceinline__ bool setup_tri(const MeshView<R>& m, uint64_t t, const ViewDev<R>& v, Projected<R>& P,
+                                          R& area) {
   const uint64_t cap = m.cap;
 #pragma unroll
   for (int n = 0; n < 3; n++) {
@@ -654,36 +577,8 @@ __device__ __forceinline__ bool setup_tri(const MeshView<R>& m, uint64_t t, cons
                      m.pos[(3 * n + 2) * cap + t]};
     if (!project(v, p, P.q[n])) return false;
   }
-  const V3<R>* q = P.q;
-  const int32_t xmin = trunc_i<R>((R(0.5) + fmin(q[0].x, fmin(q[1].x, q[2].x))) * v.fw - R(1));
-  const int32_t xmax = trunc_i<R>((R(0.5) + fmax(q[0].x, fmax(q[1].x, q[2].x))) * v.fw + R(1));
-  const int32_t ymin = trunc_i<R>((R(0.5) + fmin(q[0].y, fmin(q[1].y, q[2].y))) * v.fh - R(1));
-  const int32_t ymax = trunc_i<R>((R(0.5) + fmax(q[0].y, fmax(q[1].y, q[2].y))) * v.fh + R(1));
-  P.i0 = min(v.W - 1, max(0, xmin));
-  P.i1 = min(v.W - 1, max(0, xmax));
-  P.j0 = min(v.H - 1, max(0, ymin));
-  P.j1 = min(v.H - 1, max(0, ymax));
-  const R area = edge_fn(q[0], q[1], q[2].x, q[2].y);
-  P.ainv = R(1) / (area < R(0) ? fmin(area, R(-1.e-16)) : fmax(area, R(1.e-16)));
+  area = raster_box(v, P);
   return true;
-}
-
-// barycentric weights at pixel (i, j); false when outside (render.c:91-115)
-template <typename R>
-__device__ __forceinline__ bool pixel_weights(const ViewDev<R>& v, const Projected<R>& P, int32_t i,
-                                              int32_t j, R& w0, R& w1, R& w2) {
-  const R cx = R(1) * (R(i) + R(0.5)) / v.fw - R(0.5);
-  const R cy = R(1) * (R(j) + R(0.5)) / v.fh - R(0.5);
-  w0 = P.ainv * edge_fn(P.q[1], P.q[2], cx, cy);
-  w1 = P.ainv * edge_fn(P.q[2], P.q[0], cx, cy);
-  w2 = P.ainv * edge_fn(P.q[0], P.q[1], cx, cy);
-  return !(w0 < R(-1.e-8) || w1 < R(-1.e-8) || w2 < R(-1.e-8));
-}
-
-// harmonic depth (render.c:120-124)
-template <typename R>
-__device__ __forceinline__ R pixel_depth(const Projected<R>& P, R w0, R w1, R w2) {
-  return R(1) / (w0 / P.q[0].z + w1 / P.q[1].z + w2 / P.q[2].z);
 }
 
 // the image owning raster work item u (the last job starting at or before
@@ -699,111 +594,135 @@ __device__ __forceinline__ int find_rjob(const RasterJob<R>* __restrict__ jobs, 
   return lo;
 }
 
-// FP32: one pass over packed (depth << 32 | id) keys.  FP64 pass 1: the
-// latest serial id among the fragments at each pixel's final depth, walking
-// the geometry; it only runs when the depth pass's fragment list overflowed.
-template <typename R, int PASS>
-__global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, const RasterJob<R>* __restrict__ rjobs, int njobs,
-                                                     const FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
-                                                     uint32_t* __restrict__ id) {
-  if (sizeof(R) == 8 && PASS == 1 && !(hdr->overflow & 2u)) return;
-  const uint64_t nwork = hdr->nwork;
-  for (uint64_t u0 = uint64_t(blockIdx.x) * kThreads; u0 < nwork; u0 += uint64_t(gridDim.x) * kThreads) {
-    // one image per workgroup round: its constants are uniform
-    const int jb = njobs > 1 ? find_rjob(rjobs, njobs, u0) : 0;
-    const uint64_t t = rjobs[jb].tri0 + (u0 + threadIdx.x - rjobs[jb].wk0);
-    if (t >= rjobs[jb].tri1) continue;
-    if (!((m.edges[t] >> 15) & 1u)) continue;
-    const ViewDev<R> v = rjobs[jb].v;
-    const uint64_t px0 = rjobs[jb].px0;
-    Projected<R> P;
-    if (!setup_tri(m, t, v, P)) continue;
-    for (int32_t j = P.j0; j <= P.j1; j++) {
-      for (int32_t i = P.i0; i <= P.i1; i++) {
-        R w0, w1, w2;
-        if (!pixel_weights(v, P, i, j, w0, w1, w2)) continue;
-        const R dep = pixel_depth(P, w0, w1, w2);
-        const uint64_t px = px0 + uint64_t(j) * uint64_t(v.W) + uint64_t(i);
-        // the stored keys only grow, so a key below any value read from the
-        // buffer can never change it and skips the atomic
-        if constexpr (sizeof(R) == 8) {
-          const uint64_t k = ord64(dep);
-          if (PASS == 0) {
-            if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
-          } else if (k == key[px]) {
-            atomicMax(id + px, uint32_t(t + 1));
-          }
-        } else {
-          const uint64_t k = uint64_t(ord32(dep)) << 32 | uint64_t(t + 1);
-          if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
-        }
-      }
-    }
+// ----------------------------------------------------------------------
+// raster
+//
+// One triangle per thread; a workgroup round takes 256 consecutive work
+// items of one image, so the image's constants are uniform.  Workgroups
+// stride over the rounds and load the next round's triangles while drawing
+// the current ones.  In FP64 the coverage loop runs over the narrowed box
+// of tight_box.
+// Modes:
+//   kRasterFp32     FP32: packed (depth << 32 | id) keys, one pass
+//   kRasterDepth64  FP64 depth pass that keeps the fragments that may end
+//                   up nearest: a key below the value read from the z-buffer
+//                   can never win (keys only grow); every other one goes
+//                   through atomicMax and into a workgroup list in LDS, which
+//                   is flushed with one global reservation per round.  The
+//                   list holds every fragment whose key equals the final
+//                   value.  Overflow (LDS or global) sets bit 2 of
+//                   hdr->overflow.
+//   kRasterId64     FP64 id pass over the geometry: the latest serial id at
+//                   each pixel's final depth; runs only when the fragment
+//                   list overflowed
+constexpr int kFragLds = 1024;
+
+// registers of one raster work item, loaded a round ahead
+template <typename R>
+struct TriRegs {
+  R p[9];
+  uint32_t ed;
+  uint64_t t;
+  int jb;
+  bool ok;
+};
+
+template <typename R>
+__device__ __forceinline__ void fetch_tri(const MeshView<R>& m, const RasterJob<R>* __restrict__ rjobs, int njobs,
+                                          uint64_t u0, TriRegs<R>& T) {
+  T.jb = njobs > 1 ? find_rjob(rjobs, njobs, u0) : 0;
+  T.t = rjobs[T.jb].tri0 + (u0 + threadIdx.x - rjobs[T.jb].wk0);
+  T.ok = T.t < rjobs[T.jb].tri1;
+  T.ed = 0;
+  if (T.ok) {
+    T.ed = m.edges[T.t];
+#pragma unroll
+    for (int n = 0; n < 9; n++) T.p[n] = m.pos[uint64_t(n) * m.cap + T.t];
   }
 }
 
-// FP64 depth pass that also keeps the fragments that may end up nearest: a
-// fragment whose key is below a value already in the z-buffer can never
-// win (keys only grow), every other one goes through atomicMax and is kept
-// when the value it replaced was not larger.  The kept fragments collect in
-// a workgroup list in LDS and are flushed with one global reservation per
-// workgroup round; overflow (LDS or global) sets bit 2 of hdr->overflow, in
-// which case the id pass walks the geometry instead.
-constexpr int kFragLds = 2048;
-
-__global__ __launch_bounds__(kThreads) void k_raster_frags(MeshView<double> m,
-                                                           const RasterJob<double>* __restrict__ rjobs, int njobs,
-                                                           FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
-                                                           Frag* __restrict__ frags, uint32_t cap) {
-  __shared__ Frag s_frag[kFragLds];
+template <typename R, int MODE>
+__global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, const RasterJob<R>* __restrict__ rjobs, int njobs,
+                                                     FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
+                                                     uint32_t* __restrict__ id, Frag* __restrict__ frags,
+                                                     uint32_t cap) {
+  __shared__ Frag s_frag[MODE == kRasterDepth64 ? kFragLds : 1];
   __shared__ uint32_t s_n, s_base;
+  if (MODE == kRasterId64 && !(hdr->overflow & 2u)) return;
   const uint64_t nwork = hdr->nwork;
   const uint64_t stride = uint64_t(gridDim.x) * kThreads;
-  // every thread runs the same number of rounds so the barriers match
-  for (uint64_t u0 = uint64_t(blockIdx.x) * kThreads; u0 < nwork; u0 += stride) {
-    if (threadIdx.x == 0) s_n = 0;
-    __syncthreads();
-    // one image per workgroup round: its constants are uniform
-    const int jb = njobs > 1 ? find_rjob(rjobs, njobs, u0) : 0;
-    const uint64_t t = rjobs[jb].tri0 + (u0 + threadIdx.x - rjobs[jb].wk0);
-    if (t < rjobs[jb].tri1) {
-      const ViewDev<double> v = rjobs[jb].v;
-      const uint64_t px0 = rjobs[jb].px0;
-      Projected<double> P;
-      if (((m.edges[t] >> 15) & 1u) && setup_tri(m, t, v, P)) {
+  uint64_t u0 = uint64_t(blockIdx.x) * kThreads;
+  TriRegs<R> cur;
+  cur.ok = false;
+  if (u0 < nwork) fetch_tri(m, rjobs, njobs, u0, cur);
+  for (; u0 < nwork; u0 += stride) {
+    // the next round's triangle is loaded while this one is drawn
+    TriRegs<R> nxt;
+    nxt.ok = false;
+    if (u0 + stride < nwork) fetch_tri(m, rjobs, njobs, u0 + stride, nxt);
+    if (MODE == kRasterDepth64) {
+      if (threadIdx.x == 0) s_n = 0;
+      __syncthreads();
+    }
+    if (cur.ok && ((cur.ed >> 15) & 1u)) {
+      const ViewDev<R> v = rjobs[cur.jb].v;
+      Projected<R> P;
+      bool vis = true;
+#pragma unroll
+      for (int n = 0; n < 3; n++) vis = vis && project(v, V3<R>{cur.p[3 * n], cur.p[3 * n + 1], cur.p[3 * n + 2]}, P.q[n]);
+      if (vis) {
+        const R area = raster_box(v, P);
+        // FP64 pays for narrowing the box; in FP32 the box arithmetic
+        // costs more than the candidates it removes
+        if constexpr (sizeof(R) == 8) tight_box(v, area, P);
+        (void)area;
+        const uint64_t row0 = rjobs[cur.jb].px0;
+        const uint32_t tid1 = uint32_t(cur.t + 1);
         for (int32_t j = P.j0; j <= P.j1; j++) {
+          const R cy = pixel_centre<R>(j, v.fh);
+          const uint64_t prow = row0 + uint64_t(j) * uint64_t(v.W);
           for (int32_t i = P.i0; i <= P.i1; i++) {
-            double w0, w1, w2;
-            if (!pixel_weights(v, P, i, j, w0, w1, w2)) continue;
-            const uint64_t px = px0 + uint64_t(j) * uint64_t(v.W) + uint64_t(i);
-            const uint64_t k = ord64(pixel_depth(P, w0, w1, w2));
-            if (k < key[px]) continue;
-            const uint64_t old = atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
-            if (k < old) continue;
-            const uint32_t o = atomicAdd(&s_n, 1u);
-            if (o < uint32_t(kFragLds)) {
-              Frag fr;
-              fr.px = uint32_t(px);
-              fr.id = uint32_t(t + 1);
-              fr.key = k;
-              s_frag[o] = fr;
+            R w0, w1, w2;
+            if (!pixel_weights(P, pixel_centre<R>(i, v.fw), cy, w0, w1, w2)) continue;
+            const R dep = pixel_depth(P, w0, w1, w2);
+            const uint64_t px = prow + uint64_t(i);
+            if constexpr (MODE == kRasterFp32) {
+              const uint64_t k = uint64_t(ord32(float(dep))) << 32 | uint64_t(tid1);
+              if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+            } else if constexpr (MODE == kRasterDepth64) {
+              const uint64_t k = ord64(double(dep));
+              if (k < key[px]) continue;
+              atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+              const uint32_t o = atomicAdd(&s_n, 1u);
+              if (o < uint32_t(kFragLds)) {
+                Frag fr;
+                fr.px = uint32_t(px);
+                fr.id = tid1;
+                fr.key = k;
+                s_frag[o] = fr;
+              }
+            } else {
+              if (ord64(double(dep)) == key[px]) atomicMax(id + px, tid1);
             }
           }
         }
       }
     }
-    __syncthreads();
-    const uint32_t n = s_n < uint32_t(kFragLds) ? s_n : uint32_t(kFragLds);
-    if (threadIdx.x == 0) {
-      s_base = n ? atomicAdd(&hdr->nfrag, n) : 0u;
-      if (s_n > uint32_t(kFragLds) || (n && uint64_t(s_base) + n > cap)) atomicOr(&hdr->overflow, 2u);
+    if constexpr (MODE == kRasterDepth64) {
+      __syncthreads();
+      const uint32_t n = s_n < uint32_t(kFragLds) ? s_n : uint32_t(kFragLds);
+      if (threadIdx.x == 0) {
+        s_base = n ? atomicAdd(&hdr->nfrag, n) : 0u;
+        if (s_n > uint32_t(kFragLds) || (n && uint64_t(s_base) + n > cap)) atomicOr(&hdr->overflow, 2u);
+      }
+      __syncthreads();
+      const uint32_t b0 = s_base;
+      for (uint32_t e = threadIdx.x; e < n; e += kThreads) {
+        if (uint64_t(b0) + e < cap) frags[b0 + e] = s_frag[e];
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    const uint32_t b0 = s_base;
-    for (uint32_t q = threadIdx.x; q < n; q += kThreads) {
-      if (uint64_t(b0) + q < cap) frags[b0 + q] = s_frag[q];
-    }
-    __syncthreads();
+    cur = nxt;
   }
 }
 
@@ -907,9 +826,10 @@ __global__ __launch_bounds__(kThreads) void k_resolve(MeshView<R> m, const Raste
       }
       const ObjMeta om = meta[lo];
       Projected<R> P;
-      setup_tri(m, t, v, P);
+      R area;
+      setup_tri(m, t, v, P, area);
       R w0, w1, w2;
-      pixel_weights(v, P, i, j, w0, w1, w2);
+      pixel_weights(P, pixel_centre<R>(i, v.fw), pixel_centre<R>(j, v.fh), w0, w1, w2);
       const uint32_t c = m.cell[t];
       const uint32_t ed = m.edges[t];
       V3<R> nv[3];
@@ -1004,18 +924,19 @@ hipError_t launch_zinit(uint64_t* key, uint32_t* id, uint64_t npx, int precision
 }
 
 template <typename R>
-hipError_t launch_raster(const MeshView<R>& m, const RasterJob<R>* rjobs, int njobs, const FrameHdr* hdr, int pass,
-                         uint64_t* key, uint32_t* id, uint32_t grid, hipStream_t st) {
-  if (pass == 0)
-    hipLaunchKernelGGL((k_raster<R, 0>), dim3(grid), dim3(kThreads), 0, st, m, rjobs, njobs, hdr, key, id);
+hipError_t launch_raster(const MeshView<R>& m, const RasterJob<R>* rjobs, int njobs, FrameHdr* hdr, int mode,
+                         uint64_t* key, uint32_t* id, Frag* frags, uint32_t cap, uint32_t grid, hipStream_t st) {
+  if (sizeof(R) == 4 && mode != kRasterFp32) return hipErrorInvalidValue;
+  if (sizeof(R) == 8 && mode == kRasterFp32) return hipErrorInvalidValue;
+  if (mode == kRasterFp32)
+    hipLaunchKernelGGL((k_raster<R, kRasterFp32>), dim3(grid), dim3(kThreads), 0, st, m, rjobs, njobs, hdr, key, id,
+                       frags, cap);
+  else if (mode == kRasterDepth64)
+    hipLaunchKernelGGL((k_raster<R, kRasterDepth64>), dim3(grid), dim3(kThreads), 0, st, m, rjobs, njobs, hdr, key,
+                       id, frags, cap);
   else
-    hipLaunchKernelGGL((k_raster<R, 1>), dim3(grid), dim3(kThreads), 0, st, m, rjobs, njobs, hdr, key, id);
-  return hipGetLastError();
-}
-
-hipError_t launch_raster_frags(const MeshView<double>& m, const RasterJob<double>* rjobs, int njobs, FrameHdr* hdr,
-                               uint64_t* key, Frag* frags, uint32_t cap, uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_raster_frags, dim3(grid), dim3(kThreads), 0, st, m, rjobs, njobs, hdr, key, frags, cap);
+    hipLaunchKernelGGL((k_raster<R, kRasterId64>), dim3(grid), dim3(kThreads), 0, st, m, rjobs, njobs, hdr, key, id,
+                       frags, cap);
   return hipGetLastError();
 }
 
@@ -1054,10 +975,10 @@ template hipError_t launch_vertex_normals<double>(const MeshView<double>&, const
                                                   hipStream_t);
 template hipError_t launch_vertex_normals<float>(const MeshView<float>&, const ObjMeta*, uint64_t, double*,
                                                  hipStream_t);
-template hipError_t launch_raster<double>(const MeshView<double>&, const RasterJob<double>*, int, const FrameHdr*,
-                                          int, uint64_t*, uint32_t*, uint32_t, hipStream_t);
-template hipError_t launch_raster<float>(const MeshView<float>&, const RasterJob<float>*, int, const FrameHdr*, int,
-                                         uint64_t*, uint32_t*, uint32_t, hipStream_t);
+template hipError_t launch_raster<double>(const MeshView<double>&, const RasterJob<double>*, int, FrameHdr*, int,
+                                          uint64_t*, uint32_t*, Frag*, uint32_t, uint32_t, hipStream_t);
+template hipError_t launch_raster<float>(const MeshView<float>&, const RasterJob<float>*, int, FrameHdr*, int,
+                                         uint64_t*, uint32_t*, Frag*, uint32_t, uint32_t, hipStream_t);
 template hipError_t launch_resolve<double>(const MeshView<double>&, const RasterJob<double>*, int, uint64_t,
                                            const ObjMeta*, int, const uint8_t*, const FrameHdr*, const uint64_t*,
                                            const uint32_t*, uint32_t, hipStream_t);

@@ -152,6 +152,19 @@ extern int contour3d_amd_mesh_render(
 
 extern int contour3d_amd_mesh_free(contour3d_amd_mesh_t * mesh);
 
+/* ---- tuning ------------------------------------------------------------- */
+
+/* Launch-shape and capacity parameters.  None changes a result: every
+ * setting renders the same bits. */
+enum {
+  CONTOUR3D_AMD_TUNE_TESS_GRID = 0,     /* max workgroups of count/emit (0 = one per tile) */
+  CONTOUR3D_AMD_TUNE_RASTER_GRID = 1,   /* max workgroups of the raster passes (0 = one per 256 triangles) */
+  CONTOUR3D_AMD_TUNE_FRAG_CAPACITY = 2  /* FP64 fragment list entries (grows by itself after an overflow;
+                                           an overflowing frame takes the slower geometry id pass) */
+};
+
+extern int contour3d_amd_tune(int param, int64_t value);
+
 /* ---- profiling --------------------------------------------------------- */
 
 /* Stages timed with HIP events on the library stream when profiling is on.

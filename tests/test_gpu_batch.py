@@ -122,3 +122,38 @@ def test_batch_errors():
     assert api.execute_batch([]) == []
     out = api.execute_batch([good])
     assert out[0][0].shape == (16, 16, 3)
+
+
+def test_launch_shapes_and_fallback_pass_do_not_change_results(orc):
+    """Persistent loops with few workgroups (several tiles and raster
+    rounds per workgroup) and a fragment list too small for the frame (the
+    FP64 geometry id pass) render the same bits."""
+    banner("tuning: tiny grids, fragment-list overflow")
+    batch = uneven_batch()
+    ref64 = api.execute_batch(batch, precision=api.FP64)
+    ref32 = api.execute_batch(batch, precision=api.FP32)
+    try:
+        for tess, ras, frag in ((8, 8, 0), (24, 3, 16), (0, 0, 16)):
+            api.tune(api.TUNE_TESS_GRID, tess)
+            api.tune(api.TUNE_RASTER_GRID, ras)
+            if frag:
+                api.tune(api.TUNE_FRAG_CAPACITY, frag)
+            got = api.execute_batch(batch, precision=api.FP64)
+            for q in range(len(batch)):
+                assert np.array_equal(got[q][0], ref64[q][0]), (tess, ras, frag, q)
+                assert same_bits(got[q][1], ref64[q][1]), (tess, ras, frag, q)
+            got = api.execute_batch(batch, precision=api.FP32)
+            for q in range(len(batch)):
+                assert np.array_equal(got[q][0], ref32[q][0]), (tess, ras, frag, q)
+                assert same_bits(got[q][1], ref32[q][1]), (tess, ras, frag, q)
+        # a single image with the fallback pass, against the oracle
+        api.tune(api.TUNE_FRAG_CAPACITY, 1)
+        im = batch[2]
+        ref_rgb, ref_dep = orc.execute(im["camera"], im["light"], im["size"], im["center"], im["local"], im["bg"],
+                                       im["contours"], im["lines"])
+        rgb, dep = alone(im, api.FP64)
+        assert np.array_equal(rgb, ref_rgb)
+        assert same_bits(dep, ref_dep)
+    finally:
+        api.tune(api.TUNE_TESS_GRID, 0)
+        api.tune(api.TUNE_RASTER_GRID, 0)
