@@ -73,6 +73,28 @@ class Image(C.Structure):
     ]
 
 
+class ServerOpts(C.Structure):
+    """contour3d_amd_server_opts_t (no reference counterpart)."""
+
+    _fields_ = [("max_images", C.c_size_t), ("hbm_budget", C.c_uint64), ("max_wait_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    """contour3d_amd_server_stats_t (no reference counterpart)."""
+
+    _fields_ = [
+        ("completed", C.c_uint64),
+        ("failed", C.c_uint64),
+        ("batches", C.c_uint64),
+        ("mean_batch", C.c_double),
+        ("p50_ms", C.c_double),
+        ("p99_ms", C.c_double),
+        ("max_ms", C.c_double),
+        ("frames_per_s", C.c_double),
+        ("peak_admitted_bytes", C.c_uint64),
+    ]
+
+
 def vec(v):
     return Vector(float(v[0]), float(v[1]), float(v[2]))
 

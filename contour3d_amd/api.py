@@ -30,6 +30,11 @@ EXPORTS = (
     "contour3d_amd_identity",
     "contour3d_amd_execute_rgb",
     "contour3d_amd_execute_batch",
+    "contour3d_amd_server_start",
+    "contour3d_amd_server_submit",
+    "contour3d_amd_server_wait",
+    "contour3d_amd_server_stats",
+    "contour3d_amd_server_stop",
     "contour3d_amd_mesh_build",
     "contour3d_amd_mesh_count",
     "contour3d_amd_mesh_copy",
@@ -70,6 +75,10 @@ def lib():
         L.contour3d_amd_execute_batch.restype = C.c_int
         L.contour3d_amd_execute_batch.argtypes = [C.c_size_t, C.POINTER(_abi.Image), C.c_int]
         L.contour3d_amd_tune.argtypes = [C.c_int, C.c_int64]
+        L.contour3d_amd_server_start.argtypes = [C.POINTER(_abi.ServerOpts)]
+        L.contour3d_amd_server_submit.argtypes = [C.POINTER(_abi.Image), C.c_int, C.POINTER(C.c_uint64)]
+        L.contour3d_amd_server_wait.argtypes = [C.c_uint64, C.POINTER(C.c_double)]
+        L.contour3d_amd_server_stats.argtypes = [C.POINTER(_abi.ServerStats)]
         L.contour3d_amd_init.argtypes = [C.c_int]
         L.contour3d_amd_set_precision.argtypes = [C.c_int]
         L.contour3d_amd_mesh_build.argtypes = [C.POINTER(_abi.ContourObj), C.c_size_t, C.c_int,
@@ -211,43 +220,99 @@ def execute_batch(images, precision=FP64):
     tessellates them once.  Returns a list of (rgb, depth)."""
     keep = _Keep()
     arr = (_abi.Image * max(1, len(images)))()
-    out = []
     made = {}
-    for n, im in enumerate(images):
-        W, H = int(im["size"][0]), int(im["size"][1])
-        cs = im.get("contours", ())
-        key = id(cs)
-        if key not in made:
-            made[key] = (contour_objs(cs, keep), len(cs))
-        objs, nc = made[key]
-        lines = im.get("lines", ())
-        lobjs = line_objs(lines, keep)
-        keep.items += [objs, lobjs]
-        rgb = im.get("rgb")
-        if rgb is None:
-            rgb = np.zeros((H, W, 3), np.uint8)
-        depth = im.get("depth")
-        if depth is None and im.get("want_depth"):
-            depth = np.zeros((H, W), np.float64)
-        e = arr[n]
-        e.camera_position = _abi.vec(im["camera"])
-        e.light_direction = _abi.vec(im["light"])
-        e.screen_sizes[:] = [W, H]
-        e.screen_center = _abi.vec(im["center"])
-        e.screen_local[0] = _abi.vec(im["local"][0])
-        e.screen_local[1] = _abi.vec(im["local"][1])
-        e.bg_color = _abi.color(im["bg"])
-        e.num_contours = nc
-        e.contour_objs = C.cast(objs, C.POINTER(_abi.ContourObj))
-        e.num_lines = len(lines)
-        e.line_objs = C.cast(lobjs, C.POINTER(_abi.LineObj))
-        e.rgb = _addr(rgb)
-        e.depth = None if depth is None else _addr(depth)
-        keep.items += [rgb, depth]
-        out.append((rgb, depth))
+    out = [_fill_image(arr[n], im, keep, made) for n, im in enumerate(images)]
     if lib().contour3d_amd_execute_batch(len(images), arr, int(precision)) != 0:
         raise RuntimeError("contour3d_amd_execute_batch failed")
     return out
+
+
+def _fill_image(e, im, keep, made):
+    """Fills contour3d_amd_image_t `e` from an execute_batch dict; returns
+    its (rgb, depth) outputs.  `made` maps id(contours list) to the ctypes
+    array already built for it."""
+    W, H = int(im["size"][0]), int(im["size"][1])
+    cs = im.get("contours", ())
+    key = id(cs)
+    if key not in made:
+        made[key] = (contour_objs(cs, keep), len(cs))
+    objs, nc = made[key]
+    lines = im.get("lines", ())
+    lobjs = line_objs(lines, keep)
+    keep.items += [objs, lobjs, cs]
+    rgb = im.get("rgb")
+    if rgb is None:
+        rgb = np.zeros((H, W, 3), np.uint8)
+    depth = im.get("depth")
+    if depth is None and im.get("want_depth"):
+        depth = np.zeros((H, W), np.float64)
+    e.camera_position = _abi.vec(im["camera"])
+    e.light_direction = _abi.vec(im["light"])
+    e.screen_sizes[:] = [W, H]
+    e.screen_center = _abi.vec(im["center"])
+    e.screen_local[0] = _abi.vec(im["local"][0])
+    e.screen_local[1] = _abi.vec(im["local"][1])
+    e.bg_color = _abi.color(im["bg"])
+    e.num_contours = nc
+    e.contour_objs = C.cast(objs, C.POINTER(_abi.ContourObj))
+    e.num_lines = len(lines)
+    e.line_objs = C.cast(lobjs, C.POINTER(_abi.LineObj))
+    e.rgb = _addr(rgb)
+    e.depth = None if depth is None else _addr(depth)
+    keep.items += [rgb, depth]
+    return rgb, depth
+
+
+class Server:
+    """The request loop of contour3d_amd_server_*: submit() queues one image
+    (an execute_batch dict) and returns a ticket; wait(ticket) returns its
+    (rgb, depth, latency_ms).  The buffers behind a request are held here
+    until it has been waited for."""
+
+    def __init__(self, max_images=16, hbm_budget=200e9, max_wait_ms=0.0):
+        o = _abi.ServerOpts(int(max_images), int(hbm_budget), float(max_wait_ms))
+        if lib().contour3d_amd_server_start(C.byref(o)) != 0:
+            raise RuntimeError("contour3d_amd_server_start failed")
+        self._pending = {}
+        self._made = {}
+        self._keep = _Keep()
+
+    def submit(self, image, precision=FP64):
+        keep = _Keep()
+        e = _abi.Image()
+        out = _fill_image(e, image, keep, self._made)
+        self._keep.items += keep.items
+        t = C.c_uint64()
+        if lib().contour3d_amd_server_submit(C.byref(e), int(precision), C.byref(t)) != 0:
+            raise RuntimeError("contour3d_amd_server_submit failed")
+        self._pending[t.value] = (e, keep, out)
+        return t.value
+
+    def wait(self, ticket):
+        lat = C.c_double()
+        rc = lib().contour3d_amd_server_wait(int(ticket), C.byref(lat))
+        e, keep, out = self._pending.pop(ticket)
+        if rc != 0:
+            raise RuntimeError(f"request {ticket} failed")
+        return out[0], out[1], lat.value
+
+    def stats(self):
+        st = _abi.ServerStats()
+        if lib().contour3d_amd_server_stats(C.byref(st)) != 0:
+            raise RuntimeError("contour3d_amd_server_stats failed")
+        return {f: getattr(st, f) for f, _ in _abi.ServerStats._fields_}
+
+    def stop(self):
+        lib().contour3d_amd_server_stop()
+        self._pending.clear()
+        self._made.clear()
+        self._keep = _Keep()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.stop()
 
 
 class Mesh:

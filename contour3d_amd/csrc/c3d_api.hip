@@ -11,8 +11,15 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
 #include <memory>
 #include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 namespace c3d {
@@ -106,6 +113,7 @@ struct Geom {
   DevBuf grids;  // nx + ny + nz doubles
   DevBuf table;  // 3 * npts doubles when not the identity
   uint64_t stamp = 0;
+  std::atomic<int> busy{0};  // frames in flight that read it (never evicted then)
 };
 
 struct Frags {
@@ -157,49 +165,84 @@ struct contour3d_amd_mesh {
 namespace c3d {
 namespace {
 
-struct Lib {
-  std::mutex mu;
-  bool ready = false;
-  int device = 0;
+// Stream, buffers and host staging of one frame.  The synchronous entry
+// points use the library's main frame; the serving loop alternates two, so
+// that one frame's host work and device work overlap the other's.
+struct Frame {
   hipStream_t st = nullptr;
-  int precision = CONTOUR3D_AMD_FP64;
-  NbrTable nbr{};
-  uint64_t stamp = 0;
-  std::vector<std::unique_ptr<Geom>> geoms;
+  hipEvent_t done = nullptr;
   std::vector<std::unique_ptr<DevBuf>> uploads;
   DevBuf cnt, tiles, tjobs, rjobs, hdr;
   DevBuf key, id, rgb, depth, fpix, fdep, frgb, frags, vntmp;
   uint32_t frag_cap = 0;
   HostBuf hrgb, hstage;
-  // host sources of asynchronous uploads stay alive until the next call
+  // host sources of asynchronous uploads stay alive until the frame ends
   std::vector<TessJob> h_tjobs;
   std::vector<ObjMeta> h_meta;
   std::vector<RasterJob<double>> h_rjobs64;
   std::vector<RasterJob<float>> h_rjobs32;
   uint32_t h_nline = 0;
   Frags h_frags;
-  // workgroups of the tessellation and raster kernels: one per tile or
-  // per 256 triangles unless capped
+  contour3d_amd_mesh mesh;  // the frame's triangles, kept between frames
+  std::vector<Geom*> geoms_used;  // lattice tables the frame in flight reads
+  Prof prof;
+  bool fp64 = true;         // form of the frame in flight
+  template <typename R>
+  std::vector<RasterJob<R>>& rjobs_host();
+  hipError_t create() {
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+    return e;
+  }
+  void release() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (auto& u : uploads) u->release();
+    uploads.clear();
+    DevBuf* bufs[] = {&cnt, &tiles, &tjobs, &rjobs, &hdr, &key, &id, &rgb, &depth, &fpix, &fdep, &frgb, &frags, &vntmp};
+    for (auto* b : bufs) b->release();
+    mesh.release();
+    frag_cap = 0;
+    hrgb.release();
+    hstage.release();
+    for (auto e : prof.pool) (void)hipEventDestroy(e);
+    prof.pool.clear();
+    prof.pending.clear();
+    if (done) (void)hipEventDestroy(done);
+    done = nullptr;
+    if (st) (void)hipStreamDestroy(st);
+    st = nullptr;
+  }
+};
+
+template <>
+std::vector<RasterJob<double>>& Frame::rjobs_host<double>() {
+  return h_rjobs64;
+}
+template <>
+std::vector<RasterJob<float>>& Frame::rjobs_host<float>() {
+  return h_rjobs32;
+}
+
+struct Server;
+
+struct Lib {
+  std::mutex mu;
+  bool ready = false;
+  int device = 0;
+  int precision = CONTOUR3D_AMD_FP64;
+  NbrTable nbr{};
+  uint64_t stamp = 0;
+  std::vector<std::unique_ptr<Geom>> geoms;
+  // workgroups of the tessellation and raster kernels (0: built-in choice)
   uint32_t tess_cap = 0;
   uint32_t raster_cap = 0;
   uint32_t tess_grid(uint32_t ntiles) const {
     const uint32_t g = tess_cap && tess_cap < ntiles ? tess_cap : ntiles;
     return g < 8 ? 8 : g;
   }
-  template <typename R>
-  std::vector<RasterJob<R>>& rjobs_host();
-  contour3d_amd_mesh frame_mesh;  // reused by contour3d_execute
-  Prof prof;
+  Frame main;
+  Server* server = nullptr;
 };
-
-template <>
-std::vector<RasterJob<double>>& Lib::rjobs_host<double>() {
-  return h_rjobs64;
-}
-template <>
-std::vector<RasterJob<float>>& Lib::rjobs_host<float>() {
-  return h_rjobs32;
-}
 
 Lib& lib() {
   static Lib L;
@@ -215,7 +258,7 @@ int ensure_init(Lib& L, int device) {
     return 1;
   }
   C3D_HIP(hipSetDevice(device));
-  C3D_HIP(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
+  C3D_HIP(L.main.create());
   L.nbr = build_nbr_table();
   C3D_HIP(upload_tables(L.nbr));
   L.device = device;
@@ -226,8 +269,8 @@ int ensure_init(Lib& L, int device) {
 // ----------------------------------------------------------------------
 // profiling
 
-int prof_begin(Lib& L, int stage) {
-  Prof& P = L.prof;
+int prof_begin(Frame& F, int stage) {
+  Prof& P = F.prof;
   if (!P.on) return 0;
   if (P.pool.size() < 2) {
     for (int q = 0; q < 8; q++) {
@@ -239,24 +282,24 @@ int prof_begin(Lib& L, int stage) {
   P.cur_a = P.pool.back();
   P.pool.pop_back();
   P.cur_stage = stage;
-  C3D_HIP(hipEventRecord(P.cur_a, L.st));
+  C3D_HIP(hipEventRecord(P.cur_a, F.st));
   return 0;
 }
 
-int prof_end(Lib& L) {
-  Prof& P = L.prof;
+int prof_end(Frame& F) {
+  Prof& P = F.prof;
   if (!P.on || P.cur_stage < 0) return 0;
   hipEvent_t b = P.pool.back();
   P.pool.pop_back();
-  C3D_HIP(hipEventRecord(b, L.st));
+  C3D_HIP(hipEventRecord(b, F.st));
   P.pending.push_back({P.cur_stage, P.cur_a, b});
   P.cur_stage = -1;
   return 0;
 }
 
 // after a stream synchronisation
-int prof_collect(Lib& L) {
-  Prof& P = L.prof;
+int prof_collect(Frame& F) {
+  Prof& P = F.prof;
   for (auto& q : P.pending) {
     float ms = 0;
     C3D_HIP(hipEventElapsedTime(&ms, q.a, q.b));
@@ -369,14 +412,18 @@ int get_geometry(Lib& L, const contour3d_contour_obj_t& o, Geom** out) {
     }
   }
   g->stamp = ++L.stamp;
-  // keep a handful of geometries; evict the least recently used
+  // keep a handful of geometries; evict the least recently used one that
+  // no frame in flight reads
   if (L.geoms.size() >= 16) {
-    size_t victim = 0;
-    for (size_t q = 1; q < L.geoms.size(); q++)
-      if (L.geoms[q]->stamp < L.geoms[victim]->stamp) victim = q;
-    L.geoms[victim]->grids.release();
-    L.geoms[victim]->table.release();
-    L.geoms.erase(L.geoms.begin() + victim);
+    size_t victim = L.geoms.size();
+    for (size_t q = 0; q < L.geoms.size(); q++)
+      if (L.geoms[q]->busy.load() == 0 && (victim == L.geoms.size() || L.geoms[q]->stamp < L.geoms[victim]->stamp))
+        victim = q;
+    if (victim < L.geoms.size()) {
+      L.geoms[victim]->grids.release();
+      L.geoms[victim]->table.release();
+      L.geoms.erase(L.geoms.begin() + victim);
+    }
   }
   *out = g.get();
   L.geoms.push_back(std::move(g));
@@ -605,7 +652,7 @@ int ensure_capacity(contour3d_amd_mesh& M) {
 // from the plan kernel.  Synchronises once, after the plan, to size the
 // mesh; on return the emission is enqueued.
 template <typename R>
-int tessellate(Lib& L, contour3d_amd_mesh& M, const std::vector<const contour3d_contour_obj_t*>& objs,
+int tessellate(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<const contour3d_contour_obj_t*>& objs,
                RasterJob<R>* rjobs, int njobs, FrameHdr& hh) {
   const size_t n = objs.size();
   std::vector<FieldDev> fd(n);
@@ -635,10 +682,10 @@ int tessellate(Lib& L, contour3d_amd_mesh& M, const std::vector<const contour3d_
       for (size_t q = 0; q < upl_src.size(); q++)
         if (upl_src[q] == ob.array) dst = upl_dst[q];
       if (!dst) {
-        if (L.uploads.size() <= upl_src.size()) L.uploads.push_back(std::make_unique<DevBuf>());
-        DevBuf& b = *L.uploads[upl_src.size()];
+        if (F.uploads.size() <= upl_src.size()) F.uploads.push_back(std::make_unique<DevBuf>());
+        DevBuf& b = *F.uploads[upl_src.size()];
         C3D_HIP(b.ensure(bytes));
-        C3D_HIP(hipMemcpyAsync(b.p, ob.array, bytes, hipMemcpyHostToDevice, L.st));
+        C3D_HIP(hipMemcpyAsync(b.p, ob.array, bytes, hipMemcpyHostToDevice, F.st));
         upl_src.push_back(ob.array);
         upl_dst.push_back(b.p);
         dst = b.p;
@@ -647,6 +694,8 @@ int tessellate(Lib& L, contour3d_amd_mesh& M, const std::vector<const contour3d_
     }
     Geom* g = nullptr;
     if (get_geometry(L, ob, &g)) return 1;
+    g->busy++;
+    F.geoms_used.push_back(g);
     f.gx = g->grids.as<double>();
     f.gy = f.gx + f.nx;
     f.gz = f.gy + f.ny;
@@ -680,7 +729,7 @@ int tessellate(Lib& L, contour3d_amd_mesh& M, const std::vector<const contour3d_
 
   // field passes: thresholds of objects sharing a field are classified in
   // one read of it, up to kMaxThr at a time
-  std::vector<TessJob>& tj = L.h_tjobs;
+  std::vector<TessJob>& tj = F.h_tjobs;
   tj.clear();
   uint64_t jt = 0;
   for (size_t o = 0; o < n; o++) {
@@ -711,14 +760,14 @@ int tessellate(Lib& L, contour3d_amd_mesh& M, const std::vector<const contour3d_
   }
   const uint32_t ntj = uint32_t(jt);
 
-  C3D_HIP(L.cnt.ensure((nbt + 1) * sizeof(uint32_t)));
+  C3D_HIP(F.cnt.ensure((nbt + 1) * sizeof(uint32_t)));
   C3D_HIP(M.bstart.ensure((nbt + 1) * sizeof(uint64_t)));
-  C3D_HIP(L.tiles.ensure(((nbt + kScanTile - 1) / kScanTile + 1) * sizeof(uint64_t)));
-  C3D_HIP(L.tjobs.ensure((tj.size() + 1) * sizeof(TessJob)));
+  C3D_HIP(F.tiles.ensure(((nbt + kScanTile - 1) / kScanTile + 1) * sizeof(uint64_t)));
+  C3D_HIP(F.tjobs.ensure((tj.size() + 1) * sizeof(TessJob)));
   C3D_HIP(M.meta.ensure((n + 1) * sizeof(ObjMeta)));
-  C3D_HIP(L.hdr.ensure(sizeof(FrameHdr)));
-  C3D_HIP(L.hstage.ensure(sizeof(FrameHdr) + (n + 1) * sizeof(ObjMeta)));
-  std::vector<ObjMeta>& hm = L.h_meta;
+  C3D_HIP(F.hdr.ensure(sizeof(FrameHdr)));
+  C3D_HIP(F.hstage.ensure(sizeof(FrameHdr) + (n + 1) * sizeof(ObjMeta)));
+  std::vector<ObjMeta>& hm = F.h_meta;
   hm.assign(n, ObjMeta{});
   for (size_t o = 0; o < n; o++) {
     hm[o].bstart = M.bstart.as<uint64_t>() + boff[o];
@@ -731,25 +780,25 @@ int tessellate(Lib& L, contour3d_amd_mesh& M, const std::vector<const contour3d_
     hm[o].rgb[2] = objs[o]->color.b;
   }
   if (n) {
-    C3D_HIP(hipMemcpyAsync(M.meta.p, hm.data(), n * sizeof(ObjMeta), hipMemcpyHostToDevice, L.st));
-    C3D_HIP(hipMemcpyAsync(L.tjobs.p, tj.data(), tj.size() * sizeof(TessJob), hipMemcpyHostToDevice, L.st));
+    C3D_HIP(hipMemcpyAsync(M.meta.p, hm.data(), n * sizeof(ObjMeta), hipMemcpyHostToDevice, F.st));
+    C3D_HIP(hipMemcpyAsync(F.tjobs.p, tj.data(), tj.size() * sizeof(TessJob), hipMemcpyHostToDevice, F.st));
   }
-  const TessJob* tjd = L.tjobs.as<TessJob>();
-  FrameHdr* hdr = L.hdr.as<FrameHdr>();
+  const TessJob* tjd = F.tjobs.as<TessJob>();
+  FrameHdr* hdr = F.hdr.as<FrameHdr>();
 
-  if (prof_begin(L, CONTOUR3D_AMD_STAGE_COUNT)) return 1;
-  C3D_HIP(launch_count(tjd, int(tj.size()), ntj, L.cnt.as<uint32_t>(), L.tess_grid(ntj), L.st));
-  if (prof_end(L)) return 1;
-  if (prof_begin(L, CONTOUR3D_AMD_STAGE_SCAN)) return 1;
-  if (nbt) C3D_HIP(launch_scan(L.cnt.as<uint32_t>(), M.bstart.as<uint64_t>(), nbt, L.tiles.as<uint64_t>(), L.st));
+  if (prof_begin(F, CONTOUR3D_AMD_STAGE_COUNT)) return 1;
+  C3D_HIP(launch_count(tjd, int(tj.size()), ntj, F.cnt.as<uint32_t>(), L.tess_grid(ntj), F.st));
+  if (prof_end(F)) return 1;
+  if (prof_begin(F, CONTOUR3D_AMD_STAGE_SCAN)) return 1;
+  if (nbt) C3D_HIP(launch_scan(F.cnt.as<uint32_t>(), M.bstart.as<uint64_t>(), nbt, F.tiles.as<uint64_t>(), F.st));
   C3D_HIP(launch_plan<R>(M.bstart.as<uint64_t>(), nbt, M.meta.as<ObjMeta>(), int(n), rjobs, njobs, hdr,
-                         ~uint64_t(0), L.st));
-  unsigned char* hs = static_cast<unsigned char*>(L.hstage.p);
-  C3D_HIP(hipMemcpyAsync(hs, hdr, sizeof(FrameHdr), hipMemcpyDeviceToHost, L.st));
+                         ~uint64_t(0), F.st));
+  unsigned char* hs = static_cast<unsigned char*>(F.hstage.p);
+  C3D_HIP(hipMemcpyAsync(hs, hdr, sizeof(FrameHdr), hipMemcpyDeviceToHost, F.st));
   if (n)
-    C3D_HIP(hipMemcpyAsync(hs + sizeof(FrameHdr), M.meta.p, n * sizeof(ObjMeta), hipMemcpyDeviceToHost, L.st));
-  if (prof_end(L)) return 1;
-  C3D_HIP(hipStreamSynchronize(L.st));
+    C3D_HIP(hipMemcpyAsync(hs + sizeof(FrameHdr), M.meta.p, n * sizeof(ObjMeta), hipMemcpyDeviceToHost, F.st));
+  if (prof_end(F)) return 1;
+  C3D_HIP(hipStreamSynchronize(F.st));
   std::memcpy(&hh, hs, sizeof(FrameHdr));
   for (size_t o = 0; o < n; o++) {
     ObjMeta om;
@@ -758,36 +807,36 @@ int tessellate(Lib& L, contour3d_amd_mesh& M, const std::vector<const contour3d_
     M.count[o] = om.count;
   }
   M.total = hh.ntri;
-  L.prof.ntri += M.total;
+  F.prof.ntri += M.total;
   if (M.total >= (uint64_t(1) << 32) - (uint64_t(1) << 24)) {
     log_error("%llu triangles exceed the 2^32 triangle ids of one frame", (unsigned long long)M.total);
     return 1;
   }
   if (ensure_capacity<R>(M)) return 1;
-  if (prof_begin(L, CONTOUR3D_AMD_STAGE_EMIT)) return 1;
+  if (prof_begin(F, CONTOUR3D_AMD_STAGE_EMIT)) return 1;
   C3D_HIP(launch_emit<R>(tjd, int(tj.size()), ntj, M.bstart.as<uint64_t>(), view_of<R>(M), hdr, L.tess_grid(ntj),
-                         L.st));
-  if (prof_end(L)) return 1;
+                         F.st));
+  if (prof_end(F)) return 1;
   return 0;
 }
 
-// Line fragments of every image, appended to L.h_frags with frame pixel
+// Line fragments of every image, appended to F.h_frags with frame pixel
 // indices.
-int frame_lines(Lib& L, const std::vector<Image>& imgs) {
-  Frags& F = L.h_frags;
-  F.pix.clear();
-  F.depth.clear();
-  F.rgb.clear();
+int frame_lines(Frame& F, const std::vector<Image>& imgs) {
+  Frags& LF = F.h_frags;
+  LF.pix.clear();
+  LF.depth.clear();
+  LF.rgb.clear();
   for (const Image& im : imgs) {
     if (im.nlines && !im.lines) {
       log_error("line objects pointer is NULL");
       return 1;
     }
-    const size_t f0 = F.pix.size();
-    if (line_fragments(im.hv, im.nlines, im.lines, F)) return 1;
-    for (size_t q = f0; q < F.pix.size(); q++) F.pix[q] += uint32_t(im.px0);
+    const size_t f0 = LF.pix.size();
+    if (line_fragments(im.hv, im.nlines, im.lines, LF)) return 1;
+    for (size_t q = f0; q < LF.pix.size(); q++) LF.pix[q] += uint32_t(im.px0);
   }
-  if (F.pix.size() >= (size_t(1) << 31)) {
+  if (LF.pix.size() >= (size_t(1) << 31)) {
     log_error("too many line fragments in one frame");
     return 1;
   }
@@ -795,30 +844,30 @@ int frame_lines(Lib& L, const std::vector<Image>& imgs) {
 }
 
 // z-buffer passes, lines and resolve of every image of a frame over the
-// mesh M; then the host copies and one synchronisation.
+// mesh M, then the copies to host outputs; frame_finish waits for them.
 template <typename R>
-int raster_frame(Lib& L, const contour3d_amd_mesh& M, std::vector<Image>& imgs, const RasterJob<R>* rjobs,
+int raster_frame(Lib& L, Frame& F, const contour3d_amd_mesh& M, std::vector<Image>& imgs, const RasterJob<R>* rjobs,
                  uint64_t npx, const FrameHdr& hh) {
   const bool fp64 = sizeof(R) == 8;
   const int njobs = int(imgs.size());
-  const Frags& F = L.h_frags;
-  const uint32_t nf = uint32_t(F.pix.size());
-  FrameHdr* hdr = L.hdr.as<FrameHdr>();
-  C3D_HIP(L.key.ensure(npx * sizeof(uint64_t)));
-  if (fp64) C3D_HIP(L.id.ensure(npx * sizeof(uint32_t)));
-  uint32_t* id = fp64 ? L.id.as<uint32_t>() : nullptr;
-  uint64_t* key = L.key.as<uint64_t>();
-  C3D_HIP(L.fpix.ensure((nf + 1) * sizeof(uint32_t)));
-  C3D_HIP(L.fdep.ensure((nf + 1) * sizeof(double)));
-  C3D_HIP(L.frgb.ensure(3 * size_t(nf + 1)));
+  const Frags& LF = F.h_frags;
+  const uint32_t nf = uint32_t(LF.pix.size());
+  FrameHdr* hdr = F.hdr.as<FrameHdr>();
+  C3D_HIP(F.key.ensure(npx * sizeof(uint64_t)));
+  if (fp64) C3D_HIP(F.id.ensure(npx * sizeof(uint32_t)));
+  uint32_t* id = fp64 ? F.id.as<uint32_t>() : nullptr;
+  uint64_t* key = F.key.as<uint64_t>();
+  C3D_HIP(F.fpix.ensure((nf + 1) * sizeof(uint32_t)));
+  C3D_HIP(F.fdep.ensure((nf + 1) * sizeof(double)));
+  C3D_HIP(F.frgb.ensure(3 * size_t(nf + 1)));
   if (nf) {
-    C3D_HIP(hipMemcpyAsync(L.fpix.p, F.pix.data(), nf * sizeof(uint32_t), hipMemcpyHostToDevice, L.st));
-    C3D_HIP(hipMemcpyAsync(L.fdep.p, F.depth.data(), nf * sizeof(double), hipMemcpyHostToDevice, L.st));
-    C3D_HIP(hipMemcpyAsync(L.frgb.p, F.rgb.data(), 3 * size_t(nf), hipMemcpyHostToDevice, L.st));
+    C3D_HIP(hipMemcpyAsync(F.fpix.p, LF.pix.data(), nf * sizeof(uint32_t), hipMemcpyHostToDevice, F.st));
+    C3D_HIP(hipMemcpyAsync(F.fdep.p, LF.depth.data(), nf * sizeof(double), hipMemcpyHostToDevice, F.st));
+    C3D_HIP(hipMemcpyAsync(F.frgb.p, LF.rgb.data(), 3 * size_t(nf), hipMemcpyHostToDevice, F.st));
   }
-  L.h_nline = nf;
-  C3D_HIP(hipMemcpyAsync(reinterpret_cast<unsigned char*>(hdr) + offsetof(FrameHdr, nline), &L.h_nline,
-                         sizeof(uint32_t), hipMemcpyHostToDevice, L.st));
+  F.h_nline = nf;
+  C3D_HIP(hipMemcpyAsync(reinterpret_cast<unsigned char*>(hdr) + offsetof(FrameHdr, nline), &F.h_nline,
+                         sizeof(uint32_t), hipMemcpyHostToDevice, F.st));
   const MeshView<R> mv = view_of<R>(M);
   // FP64 rasterises with persistent workgroups (64 per CU) that overlap the
   // next round's loads with drawing; FP32 with one workgroup per round
@@ -826,73 +875,88 @@ int raster_frame(Lib& L, const contour3d_amd_mesh& M, std::vector<Image>& imgs, 
   const uint64_t rcap = L.raster_cap ? L.raster_cap : (fp64 ? uint64_t(persistent_blocks()) * 8 : 0);
   if (rcap && rg > rcap) rg = rcap;
   const uint32_t rgrid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(rg, 1u << 30)));
-  if (prof_begin(L, CONTOUR3D_AMD_STAGE_RASTER)) return 1;
-  C3D_HIP(launch_zinit(key, id, npx, fp64 ? 0 : 1, hdr, L.st));
+  if (prof_begin(F, CONTOUR3D_AMD_STAGE_RASTER)) return 1;
+  C3D_HIP(launch_zinit(key, id, npx, fp64 ? 0 : 1, hdr, F.st));
   if constexpr (sizeof(R) == 8) {
     // depth pass keeping the fragments that may end up nearest, then the id
     // pass over that list; when the list overflows, the geometry id pass
     // runs instead (both decide on the device) and the list grows for the
     // next frame
-    if (L.frag_cap == 0) L.frag_cap = 1u << 22;
-    C3D_HIP(L.frags.ensure(size_t(L.frag_cap) * sizeof(Frag)));
-    C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, kRasterDepth64, key, id, L.frags.as<Frag>(), L.frag_cap, rgrid,
-                             L.st));
-    C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, hdr, 0, 0, key, id, L.st));
-    C3D_HIP(launch_frag_ids(L.frags.as<Frag>(), L.frag_cap, hdr, key, id,
-                            std::min<uint32_t>((L.frag_cap + 255) / 256, 4096), L.st));
+    if (F.frag_cap == 0) F.frag_cap = 1u << 22;
+    C3D_HIP(F.frags.ensure(size_t(F.frag_cap) * sizeof(Frag)));
+    C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, kRasterDepth64, key, id, F.frags.as<Frag>(), F.frag_cap, rgrid,
+                             F.st));
+    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), nf, hdr, 0, 0, key, id, F.st));
+    C3D_HIP(launch_frag_ids(F.frags.as<Frag>(), F.frag_cap, hdr, key, id,
+                            std::min<uint32_t>((F.frag_cap + 255) / 256, 4096), F.st));
     C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, kRasterId64, key, id, nullptr, 0, uint32_t(persistent_blocks()),
-                             L.st));
-    C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, hdr, 1, 0, key, id, L.st));
+                             F.st));
+    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), nf, hdr, 1, 0, key, id, F.st));
   } else {
-    C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, kRasterFp32, key, id, nullptr, 0, rgrid, L.st));
-    C3D_HIP(launch_line_frags(L.fpix.as<uint32_t>(), L.fdep.as<double>(), nf, hdr, 0, 1, key, id, L.st));
+    C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, kRasterFp32, key, id, nullptr, 0, rgrid, F.st));
+    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), nf, hdr, 0, 1, key, id, F.st));
   }
-  if (prof_end(L)) return 1;
-  if (prof_begin(L, CONTOUR3D_AMD_STAGE_RESOLVE)) return 1;
-  C3D_HIP(launch_resolve<R>(mv, rjobs, njobs, npx, M.meta.as<ObjMeta>(), int(M.base.size()), L.frgb.as<uint8_t>(),
-                            hdr, key, id, 1u << 30, L.st));
-  if (prof_end(L)) return 1;
+  if (prof_end(F)) return 1;
+  if (prof_begin(F, CONTOUR3D_AMD_STAGE_RESOLVE)) return 1;
+  C3D_HIP(launch_resolve<R>(mv, rjobs, njobs, npx, M.meta.as<ObjMeta>(), int(M.base.size()), F.frgb.as<uint8_t>(),
+                            hdr, key, id, 1u << 30, F.st));
+  if (prof_end(F)) return 1;
   for (const Image& im : imgs) {
     const uint64_t n = uint64_t(im.hv.W) * uint64_t(im.hv.H);
     if (!im.rgb_dev)
-      C3D_HIP(hipMemcpyAsync(im.rgb, L.rgb.as<uint8_t>() + 3 * im.px0, 3 * n, hipMemcpyDeviceToHost, L.st));
+      C3D_HIP(hipMemcpyAsync(im.rgb, F.rgb.as<uint8_t>() + 3 * im.px0, 3 * n, hipMemcpyDeviceToHost, F.st));
     if (im.depth && !im.dep_dev)
-      C3D_HIP(hipMemcpyAsync(im.depth, L.depth.as<double>() + im.px0, n * sizeof(double), hipMemcpyDeviceToHost,
-                             L.st));
+      C3D_HIP(hipMemcpyAsync(im.depth, F.depth.as<double>() + im.px0, n * sizeof(double), hipMemcpyDeviceToHost,
+                             F.st));
   }
-  C3D_HIP(hipMemcpyAsync(L.hstage.p, hdr, sizeof(FrameHdr), hipMemcpyDeviceToHost, L.st));
-  C3D_HIP(hipStreamSynchronize(L.st));
+  C3D_HIP(hipMemcpyAsync(F.hstage.p, hdr, sizeof(FrameHdr), hipMemcpyDeviceToHost, F.st));
+  C3D_HIP(hipEventRecord(F.done, F.st));
+  F.fp64 = fp64;
+  return 0;
+}
+
+void release_geoms(Frame& F) {
+  for (Geom* g : F.geoms_used) g->busy--;
+  F.geoms_used.clear();
+}
+
+// Waits for a frame's device work and host copies; grows the FP64 fragment
+// list after an overflow.
+int frame_finish(Frame& F) {
+  const hipError_t e = hipEventSynchronize(F.done);
+  release_geoms(F);
+  C3D_HIP(e);
   FrameHdr fin;
-  std::memcpy(&fin, L.hstage.p, sizeof(FrameHdr));
-  if (fp64 && (fin.overflow & 2u) && fin.nfrag > L.frag_cap) {
+  std::memcpy(&fin, F.hstage.p, sizeof(FrameHdr));
+  if (F.fp64 && (fin.overflow & 2u) && fin.nfrag > F.frag_cap) {
     const uint64_t grow = uint64_t(fin.nfrag) + fin.nfrag / 4;
-    L.frag_cap = uint32_t(grow > 0xffffffffull ? 0xffffffffull : grow);
+    F.frag_cap = uint32_t(grow > 0xffffffffull ? 0xffffffffull : grow);
   }
-  return prof_collect(L);
+  return prof_collect(F);
 }
 
 template <typename R>
-int build_mesh_t(Lib& L, contour3d_amd_mesh& M, const contour3d_contour_obj_t* objs, size_t n) {
+int build_mesh_t(Lib& L, Frame& F, contour3d_amd_mesh& M, const contour3d_contour_obj_t* objs, size_t n) {
   std::vector<const contour3d_contour_obj_t*> v(n);
   for (size_t o = 0; o < n; o++) v[o] = objs + o;
   FrameHdr hh{};
-  return tessellate<R>(L, M, v, nullptr, 0, hh);
+  return tessellate<R>(L, F, M, v, nullptr, 0, hh);
 }
 
-int build_mesh(Lib& L, contour3d_amd_mesh& M, const contour3d_contour_obj_t* objs, size_t n, int precision) {
+int build_mesh(Lib& L, Frame& F, contour3d_amd_mesh& M, const contour3d_contour_obj_t* objs, size_t n, int precision) {
   if (n && !objs) {
     log_error("contour objects pointer is NULL");
     return 1;
   }
-  if (precision == CONTOUR3D_AMD_FP64) return build_mesh_t<double>(L, M, objs, n);
-  if (precision == CONTOUR3D_AMD_FP32) return build_mesh_t<float>(L, M, objs, n);
+  if (precision == CONTOUR3D_AMD_FP64) return build_mesh_t<double>(L, F, M, objs, n);
+  if (precision == CONTOUR3D_AMD_FP32) return build_mesh_t<float>(L, F, M, objs, n);
   log_error("unknown precision %d", precision);
   return 1;
 }
 
 // Validates the images, places them in the frame's pixel space and sets up
 // their output targets.
-int frame_setup(Lib& L, std::vector<Image>& imgs, uint64_t& npx) {
+int frame_setup(Frame& F, std::vector<Image>& imgs, uint64_t& npx) {
   npx = 0;
   for (size_t q = 0; q < imgs.size(); q++) {
     Image& im = imgs[q];
@@ -918,17 +982,17 @@ int frame_setup(Lib& L, std::vector<Image>& imgs, uint64_t& npx) {
     log_error("%llu pixels in one frame exceed 2^31", (unsigned long long)npx);
     return 1;
   }
-  C3D_HIP(L.rgb.ensure(3 * npx + 1));
+  C3D_HIP(F.rgb.ensure(3 * npx + 1));
 
   bool any_dep = false;
   for (const Image& im : imgs) any_dep = any_dep || (im.depth && !im.dep_dev);
-  if (any_dep) C3D_HIP(L.depth.ensure(npx * sizeof(double)));
+  if (any_dep) C3D_HIP(F.depth.ensure(npx * sizeof(double)));
   return 0;
 }
 
 template <typename R>
-void fill_rjobs(Lib& L, const std::vector<Image>& imgs) {
-  std::vector<RasterJob<R>>& rj = L.rjobs_host<R>();
+void fill_rjobs(Frame& F, const std::vector<Image>& imgs) {
+  std::vector<RasterJob<R>>& rj = F.rjobs_host<R>();
   rj.assign(imgs.size(), RasterJob<R>{});
   for (size_t q = 0; q < imgs.size(); q++) {
     const Image& im = imgs[q];
@@ -937,8 +1001,8 @@ void fill_rjobs(Lib& L, const std::vector<Image>& imgs) {
     j.px0 = im.px0;
     j.obj0 = im.obj0;
     j.obj1 = im.obj1;
-    j.rgb = im.rgb_dev ? im.rgb : L.rgb.as<uint8_t>() + 3 * im.px0;
-    j.depth = im.depth ? (im.dep_dev ? im.depth : L.depth.as<double>() + im.px0) : nullptr;
+    j.rgb = im.rgb_dev ? im.rgb : F.rgb.as<uint8_t>() + 3 * im.px0;
+    j.depth = im.depth ? (im.dep_dev ? im.depth : F.depth.as<double>() + im.px0) : nullptr;
     j.bg[0] = im.hv.bg.r;
     j.bg[1] = im.hv.bg.g;
     j.bg[2] = im.hv.bg.b;
@@ -947,9 +1011,9 @@ void fill_rjobs(Lib& L, const std::vector<Image>& imgs) {
 
 // Tessellate the objects of every image and render them all.
 template <typename R>
-int run_frame_t(Lib& L, contour3d_amd_mesh& M, std::vector<Image>& imgs) {
+int run_frame_t(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& imgs) {
   uint64_t npx = 0;
-  if (frame_setup(L, imgs, npx)) return 1;
+  if (frame_setup(F, imgs, npx)) return 1;
   // the frame's object list: each image's objects, unless an earlier image
   // shows exactly the same ones
   std::vector<const contour3d_contour_obj_t*> objs;
@@ -972,52 +1036,71 @@ int run_frame_t(Lib& L, contour3d_amd_mesh& M, std::vector<Image>& imgs) {
     for (size_t o = 0; o < im.nobj; o++) objs.push_back(im.objs + o);
     im.obj1 = uint32_t(objs.size());
   }
-  if (frame_lines(L, imgs)) return 1;
-  fill_rjobs<R>(L, imgs);
-  C3D_HIP(L.rjobs.ensure(imgs.size() * sizeof(RasterJob<R>)));
-  C3D_HIP(hipMemcpyAsync(L.rjobs.p, L.rjobs_host<R>().data(), imgs.size() * sizeof(RasterJob<R>),
-                         hipMemcpyHostToDevice, L.st));
-  RasterJob<R>* rjd = L.rjobs.as<RasterJob<R>>();
+  if (frame_lines(F, imgs)) return 1;
+  fill_rjobs<R>(F, imgs);
+  C3D_HIP(F.rjobs.ensure(imgs.size() * sizeof(RasterJob<R>)));
+  C3D_HIP(hipMemcpyAsync(F.rjobs.p, F.rjobs_host<R>().data(), imgs.size() * sizeof(RasterJob<R>),
+                         hipMemcpyHostToDevice, F.st));
+  RasterJob<R>* rjd = F.rjobs.as<RasterJob<R>>();
   FrameHdr hh{};
-  if (tessellate<R>(L, M, objs, rjd, int(imgs.size()), hh)) return 1;
-  return raster_frame<R>(L, M, imgs, rjd, npx, hh);
+  if (tessellate<R>(L, F, M, objs, rjd, int(imgs.size()), hh)) return 1;
+  return raster_frame<R>(L, F, M, imgs, rjd, npx, hh);
 }
 
-int run_frame(Lib& L, contour3d_amd_mesh& M, std::vector<Image>& imgs, int precision) {
-  if (precision == CONTOUR3D_AMD_FP64) return run_frame_t<double>(L, M, imgs);
-  if (precision == CONTOUR3D_AMD_FP32) return run_frame_t<float>(L, M, imgs);
-  log_error("unknown precision %d", precision);
-  return 1;
+// Renders the images as one frame into F; with wait, also waits for the
+// frame (otherwise frame_finish must be called before F is reused).
+int run_frame(Lib& L, Frame& F, std::vector<Image>& imgs, int precision, bool wait = true) {
+  int rc;
+  if (precision == CONTOUR3D_AMD_FP64) {
+    rc = run_frame_t<double>(L, F, F.mesh, imgs);
+  } else if (precision == CONTOUR3D_AMD_FP32) {
+    rc = run_frame_t<float>(L, F, F.mesh, imgs);
+  } else {
+    log_error("unknown precision %d", precision);
+    return 1;
+  }
+  if (rc) {
+    (void)hipStreamSynchronize(F.st);
+    release_geoms(F);
+    return rc;
+  }
+  return wait ? frame_finish(F) : 0;
 }
 
 // Render an already built mesh (all its objects) as one image.
 template <typename R>
-int render_mesh_t(Lib& L, const contour3d_amd_mesh& M, std::vector<Image>& imgs) {
+int render_mesh_t(Lib& L, Frame& F, const contour3d_amd_mesh& M, std::vector<Image>& imgs) {
   uint64_t npx = 0;
-  if (frame_setup(L, imgs, npx)) return 1;
+  if (frame_setup(F, imgs, npx)) return 1;
   for (Image& im : imgs) {
     im.obj0 = 0;
     im.obj1 = uint32_t(M.base.size());
   }
-  if (frame_lines(L, imgs)) return 1;
-  fill_rjobs<R>(L, imgs);
-  C3D_HIP(L.rjobs.ensure(imgs.size() * sizeof(RasterJob<R>)));
-  C3D_HIP(hipMemcpyAsync(L.rjobs.p, L.rjobs_host<R>().data(), imgs.size() * sizeof(RasterJob<R>),
-                         hipMemcpyHostToDevice, L.st));
-  RasterJob<R>* rjd = L.rjobs.as<RasterJob<R>>();
-  C3D_HIP(L.hdr.ensure(sizeof(FrameHdr)));
-  C3D_HIP(L.hstage.ensure(sizeof(FrameHdr)));
+  if (frame_lines(F, imgs)) return 1;
+  fill_rjobs<R>(F, imgs);
+  C3D_HIP(F.rjobs.ensure(imgs.size() * sizeof(RasterJob<R>)));
+  C3D_HIP(hipMemcpyAsync(F.rjobs.p, F.rjobs_host<R>().data(), imgs.size() * sizeof(RasterJob<R>),
+                         hipMemcpyHostToDevice, F.st));
+  RasterJob<R>* rjd = F.rjobs.as<RasterJob<R>>();
+  C3D_HIP(F.hdr.ensure(sizeof(FrameHdr)));
+  C3D_HIP(F.hstage.ensure(sizeof(FrameHdr)));
   // the plan kernel recomputes the ranges from the mesh's tile starts
   C3D_HIP(launch_plan<R>(M.bstart.as<uint64_t>(), M.ntiles, M.meta.as<ObjMeta>(), int(M.base.size()), rjd,
-                         int(imgs.size()), L.hdr.as<FrameHdr>(), ~uint64_t(0), L.st));
+                         int(imgs.size()), F.hdr.as<FrameHdr>(), ~uint64_t(0), F.st));
   FrameHdr hh{};
   hh.ntri = M.total;
   hh.nwork = (M.total + kThreads) * imgs.size();  // bound on the aligned work space
-  return raster_frame<R>(L, M, imgs, rjd, npx, hh);
+  return raster_frame<R>(L, F, M, imgs, rjd, npx, hh);
 }
 
-int render_mesh(Lib& L, const contour3d_amd_mesh& M, std::vector<Image>& imgs) {
-  return M.precision == CONTOUR3D_AMD_FP64 ? render_mesh_t<double>(L, M, imgs) : render_mesh_t<float>(L, M, imgs);
+int render_mesh(Lib& L, Frame& F, const contour3d_amd_mesh& M, std::vector<Image>& imgs) {
+  const int rc = M.precision == CONTOUR3D_AMD_FP64 ? render_mesh_t<double>(L, F, M, imgs)
+                                                   : render_mesh_t<float>(L, F, M, imgs);
+  if (rc) {
+    (void)hipStreamSynchronize(F.st);
+    return rc;
+  }
+  return frame_finish(F);
 }
 
 Image make_image(const V* cam, const V* light, const size_t sizes[2], const V* ctr, const V local[2],
@@ -1034,6 +1117,213 @@ Image make_image(const V* cam, const V* light, const size_t sizes[2], const V* c
   im.rgb_dev = !rgb_host_only && rgb && is_device_ptr(rgb);
   im.dep_dev = depth && is_device_ptr(depth);
   return im;
+}
+
+// ----------------------------------------------------------------------
+// serving loop
+//
+// Requests (one image each) queue in arrival order.  A worker thread forms
+// a batch from the head of the queue: requests of the head's arithmetic
+// form, up to max_images, while the estimated device bytes of the frames
+// in flight stay within the HBM budget.  Two frames alternate, each with
+// its own stream and buffers: while one frame runs on the device, the next
+// batch is prepared and enqueued on the other, so requests that arrive
+// during a frame start as soon as a frame slot and budget free up.  A host
+// function at the end of each frame's stream wakes the worker.
+
+double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+struct Request {
+  uint64_t ticket = 0;
+  contour3d_amd_image_t img{};
+  int precision = 0;
+  uint64_t bytes = 0;  // admission estimate
+  double t_submit = 0, t_done = 0;
+  int status = -1;  // -1 pending, 0 done, 1 failed
+};
+
+struct Flight {
+  Server* srv = nullptr;
+  int slot = 0;
+  std::vector<std::shared_ptr<Request>> reqs;
+  uint64_t bytes = 0;
+  double cells = 0;
+  bool done = false;  // set by the stream's host function
+};
+
+struct Server {
+  contour3d_amd_server_opts_t opts{};
+  std::thread worker;
+  std::mutex m;
+  std::condition_variable cv_work, cv_done;
+  std::deque<std::shared_ptr<Request>> queue;
+  std::unordered_map<uint64_t, std::shared_ptr<Request>> tickets;
+  std::deque<std::unique_ptr<Flight>> flights;
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  Frame frames[2];
+  int next_slot = 0;
+  double tri_per_cell = 0.5;  // running estimate of triangles per cell and threshold
+  uint64_t admitted = 0;      // estimated bytes of the frames in flight
+  std::vector<double> lat_ms;
+  uint64_t completed = 0, failed = 0, batches = 0, images = 0, peak = 0;
+  double t_first = 0, t_last = 0;
+};
+
+std::mutex g_server_mu;  // server start/stop against submit/wait/stats
+
+double obj_cells(const contour3d_contour_obj_t& c) {
+  double n = 1;
+  for (int d = 0; d < 3; d++) n *= c.glsizes[d] > 1 ? double(c.glsizes[d] - 1) : 0.0;
+  return n;
+}
+
+// device bytes a request adds to a frame: triangles, z-buffer and image
+// buffers, FP64 fragment list share, and host fields to upload
+uint64_t estimate_bytes(const Server& S, const contour3d_amd_image_t& im, int precision) {
+  double tri = 0, upload = 0;
+  for (size_t o = 0; o < im.num_contours; o++) {
+    const contour3d_contour_obj_t& c = im.contour_objs[o];
+    tri += obj_cells(c) * S.tri_per_cell;
+    if (!is_device_ptr(c.array)) upload += 8.0 * double(c.glsizes[0]) * double(c.glsizes[1]) * double(c.glsizes[2]);
+  }
+  const double per_tri = precision == CONTOUR3D_AMD_FP64 ? 9 * 8 + 3 * 8 + 8 : 9 * 4 + 3 * 4 + 8;
+  const double npx = double(im.screen_sizes[0]) * double(im.screen_sizes[1]);
+  const double per_px = 8 + (precision == CONTOUR3D_AMD_FP64 ? 4 + 16 : 0) + 3 + (im.depth ? 8 : 0);
+  return uint64_t(1.25 * tri * per_tri + upload + npx * per_px) + (uint64_t(1) << 20);
+}
+
+// runs on a runtime thread when the frame's stream reaches it; the flight
+// stays queued until the worker has seen `done`
+void on_frame_done(void* p) {
+  Flight* f = static_cast<Flight*>(p);
+  std::lock_guard<std::mutex> g(f->srv->m);
+  f->done = true;
+  f->srv->cv_work.notify_all();
+}
+
+void complete(Server& S, Flight& f, int rc, double tri_total) {
+  const double t = now_s();
+  std::lock_guard<std::mutex> g(S.m);
+  for (auto& r : f.reqs) {
+    r->status = rc ? 1 : 0;
+    r->t_done = t;
+    if (rc) {
+      S.failed++;
+    } else {
+      S.completed++;
+      S.lat_ms.push_back((t - r->t_submit) * 1e3);
+    }
+  }
+  if (!rc && f.cells > 0) {
+    const double seen = tri_total / f.cells;
+    S.tri_per_cell = std::max(seen, 0.8 * S.tri_per_cell + 0.2 * seen);
+  }
+  S.admitted -= f.bytes;
+  S.batches++;
+  S.images += f.reqs.size();
+  S.t_last = t;
+  S.cv_done.notify_all();
+}
+
+void server_loop(Lib& L, Server& S) {
+  (void)hipSetDevice(L.device);
+  std::unique_lock<std::mutex> lk(S.m);
+  for (;;) {
+    const double wait_s = S.opts.max_wait_ms * 1e-3;
+    auto head_fits = [&] {
+      return !S.queue.empty() && (S.flights.empty() || S.admitted + S.queue.front()->bytes <= S.opts.hbm_budget);
+    };
+    auto can_start = [&] {
+      if (S.flights.size() >= 2 || !head_fits()) return false;
+      // a lone request may wait a little for company while the device is idle
+      return !S.flights.empty() || S.queue.size() >= S.opts.max_images || wait_s <= 0 ||
+             now_s() >= S.queue.front()->t_submit + wait_s;
+    };
+    auto pred = [&] {
+      return (!S.flights.empty() && S.flights.front()->done) || can_start() ||
+             (S.stop && S.queue.empty() && S.flights.empty());
+    };
+    if (!S.queue.empty() && wait_s > 0 && S.flights.empty()) {
+      const auto until = std::chrono::steady_clock::now() +
+                         std::chrono::duration_cast<std::chrono::steady_clock::duration>(std::chrono::duration<double>(
+                             std::max(0.0, S.queue.front()->t_submit + wait_s - now_s())));
+      S.cv_work.wait_until(lk, until, pred);
+    } else {
+      S.cv_work.wait(lk, pred);
+    }
+    if (!S.flights.empty() && S.flights.front()->done) {
+      std::unique_ptr<Flight> f = std::move(S.flights.front());
+      S.flights.pop_front();
+      lk.unlock();
+      Frame& F = S.frames[f->slot];
+      const int rc = frame_finish(F);
+      complete(S, *f, rc, double(F.mesh.total));
+      lk.lock();
+      continue;
+    }
+    if (can_start()) {
+      auto f = std::make_unique<Flight>();
+      f->srv = &S;
+      const int prec = S.queue.front()->precision;
+      uint64_t bytes = 0;
+      for (auto it = S.queue.begin(); it != S.queue.end() && f->reqs.size() < S.opts.max_images;) {
+        Request& r = **it;
+        if (r.precision != prec) {
+          ++it;
+          continue;
+        }
+        // first come, first served: stop at the first request that does not fit
+        if (!(f->reqs.empty() && S.flights.empty()) && S.admitted + bytes + r.bytes > S.opts.hbm_budget) break;
+        bytes += r.bytes;
+        f->reqs.push_back(*it);
+        it = S.queue.erase(it);
+      }
+      f->bytes = bytes;
+      S.admitted += bytes;
+      S.peak = std::max(S.peak, S.admitted);
+      f->slot = S.next_slot;
+      S.next_slot ^= 1;
+      Flight* fp = f.get();
+      S.flights.push_back(std::move(f));
+      lk.unlock();
+      Frame& F = S.frames[fp->slot];
+      std::vector<Image> imgs;
+      imgs.reserve(fp->reqs.size());
+      for (auto& r : fp->reqs) {
+        const contour3d_amd_image_t& im = r->img;
+        for (size_t o = 0; o < im.num_contours; o++) fp->cells += obj_cells(im.contour_objs[o]);
+        imgs.push_back(make_image(&im.camera_position, &im.light_direction, im.screen_sizes, &im.screen_center,
+                                  im.screen_local, &im.bg_color, im.num_contours, im.contour_objs, im.num_lines,
+                                  im.line_objs, im.rgb, im.depth, false));
+      }
+      int rc;
+      {
+        std::lock_guard<std::mutex> g(L.mu);
+        rc = run_frame(L, F, imgs, prec, false);
+      }
+      if (!rc && hipLaunchHostFunc(F.st, on_frame_done, fp) != hipSuccess) {
+        log_error("server: cannot schedule the completion of a frame");
+        rc = 1;
+        (void)frame_finish(F);
+      }
+      if (rc) {
+        lk.lock();
+        std::unique_ptr<Flight> f2;
+        for (auto it = S.flights.begin(); it != S.flights.end(); ++it)
+          if (it->get() == fp) {
+            f2 = std::move(*it);
+            S.flights.erase(it);
+            break;
+          }
+        lk.unlock();
+        if (f2) complete(S, *f2, 1, 0);
+      }
+      lk.lock();
+      continue;
+    }
+    if (S.stop && S.queue.empty() && S.flights.empty()) break;
+  }
 }
 
 // output.c:125-140
@@ -1089,26 +1379,17 @@ int contour3d_amd_finalize(void) {
   auto& L = lib();
   std::lock_guard<std::mutex> g(L.mu);
   if (!L.ready) return 0;
-  (void)hipStreamSynchronize(L.st);
+  if (L.server) {
+    c3d::log_error("finalize: stop the server first");
+    return 1;
+  }
+  (void)hipStreamSynchronize(L.main.st);
   for (auto& gm : L.geoms) {
     gm->grids.release();
     gm->table.release();
   }
   L.geoms.clear();
-  for (auto& u : L.uploads) u->release();
-  L.uploads.clear();
-  c3d::DevBuf* bufs[] = {&L.cnt, &L.tiles, &L.tjobs, &L.rjobs, &L.hdr, &L.key, &L.id, &L.rgb,
-                         &L.depth, &L.fpix, &L.fdep, &L.frgb, &L.frags, &L.vntmp};
-  for (auto* b : bufs) b->release();
-  L.frame_mesh.release();
-  L.frag_cap = 0;
-  L.hrgb.release();
-  L.hstage.release();
-  for (auto e : L.prof.pool) (void)hipEventDestroy(e);
-  L.prof.pool.clear();
-  L.prof.pending.clear();
-  (void)hipStreamDestroy(L.st);
-  L.st = nullptr;
+  L.main.release();
   L.ready = false;
   return 0;
 }
@@ -1128,7 +1409,7 @@ int contour3d_amd_tune(int param, int64_t value) {
       L.raster_cap = uint32_t(value);
       return 0;
     case CONTOUR3D_AMD_TUNE_FRAG_CAPACITY:
-      L.frag_cap = uint32_t(value ? value : 1);
+      L.main.frag_cap = uint32_t(value ? value : 1);
       return 0;
     default:
       c3d::log_error("tune: unknown parameter %d", param);
@@ -1164,7 +1445,7 @@ int contour3d_amd_execute_rgb(const contour3d_vector_t* camera_position, const c
   std::vector<c3d::Image> imgs{c3d::make_image(camera_position, light_direction, screen_sizes, screen_center,
                                                screen_local, bg_color, num_contours, contour_objs, num_lines,
                                                line_objs, rgb, depth, false)};
-  return c3d::run_frame(L, L.frame_mesh, imgs, precision);
+  return c3d::run_frame(L, L.main, imgs, precision);
 }
 
 int contour3d_amd_execute_batch(size_t num_images, const contour3d_amd_image_t* images, int precision) {
@@ -1184,7 +1465,7 @@ int contour3d_amd_execute_batch(size_t num_images, const contour3d_amd_image_t* 
                                    im.screen_local, &im.bg_color, im.num_contours, im.contour_objs, im.num_lines,
                                    im.line_objs, im.rgb, im.depth, false));
   }
-  return c3d::run_frame(L, L.frame_mesh, imgs, precision);
+  return c3d::run_frame(L, L.main, imgs, precision);
 }
 
 int contour3d_execute(const sdecomp_info_t* const sdecomp_info, const contour3d_vector_t* const camera_position,
@@ -1210,15 +1491,15 @@ int contour3d_execute(const sdecomp_info_t* const sdecomp_info, const contour3d_
     c3d::log_error("screen size %zu x %zu out of range", W, H);
     return 1;
   }
-  if (L.hrgb.ensure(W * H * 3) != hipSuccess) {
+  if (L.main.hrgb.ensure(W * H * 3) != hipSuccess) {
     c3d::log_error("failed to allocate image buffer");
     return 1;
   }
-  uint8_t* host = static_cast<uint8_t*>(L.hrgb.p);
+  uint8_t* host = static_cast<uint8_t*>(L.main.hrgb.p);
   std::vector<c3d::Image> imgs{c3d::make_image(camera_position, light_direction, screen_sizes, screen_center,
                                                screen_local, bg_color, num_contours, contour3d_contour_objs,
                                                num_lines, contour3d_line_objs, host, nullptr, true)};
-  if (c3d::run_frame(L, L.frame_mesh, imgs, L.precision)) {
+  if (c3d::run_frame(L, L.main, imgs, L.precision)) {
     c3d::log_error("image output failed");
     return 1;
   }
@@ -1236,12 +1517,15 @@ int contour3d_amd_mesh_build(const contour3d_contour_obj_t* contour_objs, size_t
   std::lock_guard<std::mutex> g(L.mu);
   if (c3d::ensure_init(L, 0)) return 1;
   auto M = std::make_unique<contour3d_amd_mesh>();
-  if (c3d::build_mesh(L, *M, contour_objs, num_contours, precision)) return 1;
-  if (hipStreamSynchronize(L.st) != hipSuccess) {
+  const int rc = c3d::build_mesh(L, L.main, *M, contour_objs, num_contours, precision);
+  const hipError_t e = hipStreamSynchronize(L.main.st);
+  c3d::release_geoms(L.main);
+  if (rc) return 1;
+  if (e != hipSuccess) {
     c3d::log_error("mesh build failed on the device");
     return 1;
   }
-  if (c3d::prof_collect(L)) return 1;
+  if (c3d::prof_collect(L.main)) return 1;
   *mesh = M.release();
   return 0;
 }
@@ -1296,18 +1580,18 @@ int contour3d_amd_mesh_copy(const contour3d_amd_mesh_t* mesh, size_t obj, double
     // the mesh keeps no per-triangle vertex normals; they are gathered now
     // with the same sums the resolve pass uses for nearest triangles
     if (c3d::ensure_init(L, 0)) return 1;
-    if (L.vntmp.ensure(n * 9 * sizeof(double)) != hipSuccess) {
+    if (L.main.vntmp.ensure(n * 9 * sizeof(double)) != hipSuccess) {
       c3d::log_error("mesh_copy: allocation failed");
       return 1;
     }
     const c3d::ObjMeta* md = mesh->meta.as<c3d::ObjMeta>() + obj;
     hipError_t e = mesh->precision == CONTOUR3D_AMD_FP64
                        ? c3d::launch_vertex_normals<double>(c3d::view_of<double>(*mesh), md, n,
-                                                            L.vntmp.as<double>(), L.st)
+                                                            L.main.vntmp.as<double>(), L.main.st)
                        : c3d::launch_vertex_normals<float>(c3d::view_of<float>(*mesh), md, n,
-                                                           L.vntmp.as<double>(), L.st);
-    if (e == hipSuccess) e = hipStreamSynchronize(L.st);
-    if (e == hipSuccess) e = hipMemcpy(vn, L.vntmp.p, n * 9 * sizeof(double), hipMemcpyDeviceToHost);
+                                                           L.main.vntmp.as<double>(), L.main.st);
+    if (e == hipSuccess) e = hipStreamSynchronize(L.main.st);
+    if (e == hipSuccess) e = hipMemcpy(vn, L.main.vntmp.p, n * 9 * sizeof(double), hipMemcpyDeviceToHost);
     if (e != hipSuccess) {
       c3d::log_error("mesh_copy: vertex normals failed: %s", hipGetErrorString(e));
       return 1;
@@ -1334,23 +1618,165 @@ int contour3d_amd_mesh_render(const contour3d_amd_mesh_t* mesh, const contour3d_
   std::vector<c3d::Image> imgs{c3d::make_image(camera_position, light_direction, screen_sizes, screen_center,
                                                screen_local, bg_color, 0, nullptr, num_lines, line_objs, rgb,
                                                depth, false)};
-  return c3d::render_mesh(L, *mesh, imgs);
+  return c3d::render_mesh(L, L.main, *mesh, imgs);
 }
 
 int contour3d_amd_mesh_free(contour3d_amd_mesh_t* mesh) {
   if (!mesh) return 0;
   auto& L = lib();
   std::lock_guard<std::mutex> g(L.mu);
-  if (L.ready) (void)hipStreamSynchronize(L.st);
+  if (L.ready) (void)hipStreamSynchronize(L.main.st);
   mesh->release();
   delete mesh;
+  return 0;
+}
+
+int contour3d_amd_server_start(const contour3d_amd_server_opts_t* opts) {
+  auto& L = lib();
+  std::lock_guard<std::mutex> gs(c3d::g_server_mu);
+  {
+    std::lock_guard<std::mutex> g(L.mu);
+    if (c3d::ensure_init(L, 0)) return 1;
+    if (L.server) {
+      c3d::log_error("server already running");
+      return 1;
+    }
+  }
+  auto* S = new c3d::Server();
+  if (opts) S->opts = *opts;
+  if (S->opts.max_images == 0) S->opts.max_images = 16;
+  if (S->opts.hbm_budget == 0) S->opts.hbm_budget = uint64_t(200) * 1000 * 1000 * 1000;
+  if (S->opts.max_wait_ms < 0) S->opts.max_wait_ms = 0;
+  for (auto& F : S->frames) {
+    const hipError_t e = F.create();
+    if (e != hipSuccess) {
+      c3d::log_error("server: stream creation failed: %s", hipGetErrorString(e));
+      for (auto& G : S->frames) G.release();
+      delete S;
+      return 1;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> g(L.mu);
+    L.server = S;
+  }
+  S->worker = std::thread(c3d::server_loop, std::ref(L), std::ref(*S));
+  return 0;
+}
+
+int contour3d_amd_server_submit(const contour3d_amd_image_t* image, int precision, uint64_t* ticket) {
+  auto& L = lib();
+  std::lock_guard<std::mutex> gs(c3d::g_server_mu);
+  c3d::Server* S = L.server;
+  if (!S || !image || !ticket) {
+    c3d::log_error("server_submit: server not running or NULL argument");
+    return 1;
+  }
+  if (precision != CONTOUR3D_AMD_FP64 && precision != CONTOUR3D_AMD_FP32) {
+    c3d::log_error("unknown precision %d", precision);
+    return 1;
+  }
+  if (image->num_contours && !image->contour_objs) {
+    c3d::log_error("server_submit: contour objects pointer is NULL");
+    return 1;
+  }
+  auto r = std::make_shared<c3d::Request>();
+  r->img = *image;
+  r->precision = precision;
+  r->bytes = c3d::estimate_bytes(*S, *image, precision);
+  r->t_submit = c3d::now_s();
+  std::lock_guard<std::mutex> g(S->m);
+  if (S->stop) {
+    c3d::log_error("server_submit: server stopping");
+    return 1;
+  }
+  r->ticket = S->next_ticket++;
+  if (S->t_first == 0) S->t_first = r->t_submit;
+  S->queue.push_back(r);
+  S->tickets[r->ticket] = r;
+  *ticket = r->ticket;
+  S->cv_work.notify_all();
+  return 0;
+}
+
+int contour3d_amd_server_wait(uint64_t ticket, double* latency_ms) {
+  auto& L = lib();
+  c3d::Server* S;
+  {
+    std::lock_guard<std::mutex> gs(c3d::g_server_mu);
+    S = L.server;
+  }
+  if (!S) {
+    c3d::log_error("server_wait: server not running");
+    return 1;
+  }
+  std::unique_lock<std::mutex> lk(S->m);
+  auto it = S->tickets.find(ticket);
+  if (it == S->tickets.end()) {
+    c3d::log_error("server_wait: unknown ticket %llu", (unsigned long long)ticket);
+    return 1;
+  }
+  std::shared_ptr<c3d::Request> r = it->second;
+  S->cv_done.wait(lk, [&] { return r->status >= 0; });
+  S->tickets.erase(ticket);
+  if (latency_ms) *latency_ms = (r->t_done - r->t_submit) * 1e3;
+  return r->status;
+}
+
+int contour3d_amd_server_stats(contour3d_amd_server_stats_t* st) {
+  auto& L = lib();
+  std::lock_guard<std::mutex> gs(c3d::g_server_mu);
+  c3d::Server* S = L.server;
+  if (!S || !st) {
+    c3d::log_error("server_stats: server not running or NULL argument");
+    return 1;
+  }
+  std::lock_guard<std::mutex> g(S->m);
+  std::vector<double> v = S->lat_ms;
+  std::sort(v.begin(), v.end());
+  auto pct = [&](double q) {
+    if (v.empty()) return 0.0;
+    const size_t k = size_t(q * double(v.size() - 1) + 0.5);
+    return v[std::min(k, v.size() - 1)];
+  };
+  st->completed = S->completed;
+  st->failed = S->failed;
+  st->batches = S->batches;
+  st->mean_batch = S->batches ? double(S->images) / double(S->batches) : 0.0;
+  st->p50_ms = pct(0.50);
+  st->p99_ms = pct(0.99);
+  st->max_ms = v.empty() ? 0.0 : v.back();
+  st->frames_per_s = S->t_last > S->t_first ? double(S->completed) / (S->t_last - S->t_first) : 0.0;
+  st->peak_admitted_bytes = S->peak;
+  return 0;
+}
+
+int contour3d_amd_server_stop(void) {
+  auto& L = lib();
+  c3d::Server* S;
+  {
+    std::lock_guard<std::mutex> gs(c3d::g_server_mu);
+    S = L.server;
+    if (!S) return 0;
+    std::lock_guard<std::mutex> g(S->m);
+    S->stop = true;
+    S->cv_work.notify_all();
+  }
+  S->worker.join();
+  {
+    std::lock_guard<std::mutex> gs(c3d::g_server_mu);
+    std::lock_guard<std::mutex> g(L.mu);
+    L.server = nullptr;
+  }
+  for (auto& F : S->frames) F.release();
+  delete S;
   return 0;
 }
 
 int contour3d_amd_profile_enable(int enable) {
   auto& L = lib();
   std::lock_guard<std::mutex> g(L.mu);
-  L.prof.on = enable != 0;
+  L.main.prof.on = enable != 0;
   return 0;
 }
 
@@ -1359,10 +1785,10 @@ int contour3d_amd_profile_read(double ms[CONTOUR3D_AMD_NSTAGES], uint64_t calls[
   auto& L = lib();
   std::lock_guard<std::mutex> g(L.mu);
   for (int s = 0; s < CONTOUR3D_AMD_NSTAGES; s++) {
-    if (ms) ms[s] = L.prof.ms[s];
-    if (calls) calls[s] = L.prof.calls[s];
+    if (ms) ms[s] = L.main.prof.ms[s];
+    if (calls) calls[s] = L.main.prof.calls[s];
   }
-  if (ntri_total) *ntri_total = L.prof.ntri;
+  if (ntri_total) *ntri_total = L.main.prof.ntri;
   return 0;
 }
 
@@ -1370,10 +1796,10 @@ int contour3d_amd_profile_reset(void) {
   auto& L = lib();
   std::lock_guard<std::mutex> g(L.mu);
   for (int s = 0; s < CONTOUR3D_AMD_NSTAGES; s++) {
-    L.prof.ms[s] = 0;
-    L.prof.calls[s] = 0;
+    L.main.prof.ms[s] = 0;
+    L.main.prof.calls[s] = 0;
   }
-  L.prof.ntri = 0;
+  L.main.prof.ntri = 0;
   return 0;
 }
 

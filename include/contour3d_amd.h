@@ -101,6 +101,45 @@ typedef struct {
  */
 extern int contour3d_amd_execute_batch(size_t num_images, const contour3d_amd_image_t * images, int precision);
 
+/* ---- serving ------------------------------------------------------------ */
+
+typedef struct {
+  size_t max_images;    /* images per batch at most (0: 16) */
+  uint64_t hbm_budget;  /* estimated device bytes of the frames in flight at most (0: 200e9) */
+  double max_wait_ms;   /* while the device is idle, a request waits this long for others (0: none) */
+} contour3d_amd_server_opts_t;
+
+typedef struct {
+  uint64_t completed, failed, batches;
+  double mean_batch;                /* images per batch */
+  double p50_ms, p99_ms, max_ms;    /* from submit to the image in the caller's buffer */
+  double frames_per_s;              /* completed images / (last completion - first submit) */
+  uint64_t peak_admitted_bytes;
+} contour3d_amd_server_stats_t;
+
+/*
+ * A request loop with continuous batching.  Submitted images queue in
+ * arrival order; a worker thread batches them (same arithmetic form, up to
+ * max_images, within the HBM budget) into frames rendered exactly as
+ * contour3d_amd_execute_batch renders them, so each result is the same
+ * alone and in any mix.  Two frames are in flight at once: requests that
+ * arrive while one frame runs start on the other as soon as it is free.
+ *
+ * Every buffer an image points to (contour object array, fields, grids,
+ * line objects, rgb, depth) must stay valid and unchanged from submit until
+ * its wait returns.  Fields updated in place on the device must be complete
+ * (the updating stream synchronised) before the submit that reads them.
+ * start/stop must not run concurrently with submit, wait or stats.
+ */
+extern int contour3d_amd_server_start(const contour3d_amd_server_opts_t * opts);
+extern int contour3d_amd_server_submit(const contour3d_amd_image_t * image, int precision, uint64_t * ticket);
+/* Blocks until the request is done; 0 = rendered, 1 = failed.  Each ticket
+ * is waited for once. */
+extern int contour3d_amd_server_wait(uint64_t ticket, double * latency_ms);
+extern int contour3d_amd_server_stats(contour3d_amd_server_stats_t * stats);
+/* Renders everything still queued, then stops the worker. */
+extern int contour3d_amd_server_stop(void);
+
 /* ---- stage-level mesh access (tessellation + smoothing) --------------- */
 
 typedef struct contour3d_amd_mesh contour3d_amd_mesh_t;

@@ -55,6 +55,10 @@ int main(void) {
   P(contour3d_amd_image_t, num_contours); P(contour3d_amd_image_t, contour_objs);
   P(contour3d_amd_image_t, num_lines); P(contour3d_amd_image_t, line_objs);
   P(contour3d_amd_image_t, rgb); P(contour3d_amd_image_t, depth);
+  printf("sopts %zu\nsstats %zu\n", sizeof(contour3d_amd_server_opts_t), sizeof(contour3d_amd_server_stats_t));
+  P(contour3d_amd_server_opts_t, hbm_budget); P(contour3d_amd_server_opts_t, max_wait_ms);
+  P(contour3d_amd_server_stats_t, mean_batch); P(contour3d_amd_server_stats_t, p99_ms);
+  P(contour3d_amd_server_stats_t, frames_per_s); P(contour3d_amd_server_stats_t, peak_admitted_bytes);
   P(contour3d_contour_obj_t, glsizes); P(contour3d_contour_obj_t, grids);
   P(contour3d_contour_obj_t, converter); P(contour3d_contour_obj_t, threshold);
   P(contour3d_contour_obj_t, color); P(contour3d_contour_obj_t, array);
@@ -78,6 +82,12 @@ int main(void) {
         assert got[f"contour3d_contour_obj_t.{f}"] == getattr(_abi.ContourObj, f).offset, f
     for f in ("nitems", "converter", "color", "width"):
         assert got[f"contour3d_line_obj_t.{f}"] == getattr(_abi.LineObj, f).offset, f
+    assert got["sopts"] == C.sizeof(_abi.ServerOpts)
+    assert got["sstats"] == C.sizeof(_abi.ServerStats)
+    for f in ("hbm_budget", "max_wait_ms"):
+        assert got[f"contour3d_amd_server_opts_t.{f}"] == getattr(_abi.ServerOpts, f).offset, f
+    for f in ("mean_batch", "p99_ms", "frames_per_s", "peak_admitted_bytes"):
+        assert got[f"contour3d_amd_server_stats_t.{f}"] == getattr(_abi.ServerStats, f).offset, f
     assert got["image"] == C.sizeof(_abi.Image)
     for f, _ in _abi.Image._fields_:
         assert got[f"contour3d_amd_image_t.{f}"] == getattr(_abi.Image, f).offset, f
@@ -88,3 +98,9 @@ def test_entry_points_reject_bad_arguments_without_touching_memory():
     # NULL output pointer for a mesh build is refused before any device work
     assert L.contour3d_amd_mesh_build(None, 0, 0, None) == 1
     assert L.contour3d_amd_set_precision(7) == 1
+    # the serving entry points refuse work when no server runs
+    import ctypes as C
+    t = C.c_uint64()
+    assert L.contour3d_amd_server_submit(None, 0, C.byref(t)) == 1
+    assert L.contour3d_amd_server_wait(1, None) == 1
+    assert L.contour3d_amd_server_stop() == 0
