@@ -391,6 +391,7 @@ def profile_read():
 TUNE_TESS_GRID = 0
 TUNE_RASTER_GRID = 1
 TUNE_FRAG_CAPACITY = 2
+TUNE_GRAPHS = 3
 
 
 def tune(param, value):

@@ -165,6 +165,17 @@ struct contour3d_amd_mesh {
 namespace c3d {
 namespace {
 
+// triangle ids are u32 and line fragments take ids after the triangles
+constexpr uint64_t kMaxFrameTriangles = (uint64_t(1) << 32) - (uint64_t(1) << 24);
+
+// Host-side layout of a frame's tessellation.
+struct TessLayout {
+  uint64_t nbt = 0;  // tile-count entries (the objects' tiles)
+  uint32_t ntj = 0;  // tiles of the field passes
+  int njobs = 0;     // field passes
+  int nobj = 0;
+};
+
 // Stream, buffers and host staging of one frame.  The synchronous entry
 // points use the library's main frame; the serving loop alternates two, so
 // that one frame's host work and device work overlap the other's.
@@ -185,8 +196,19 @@ struct Frame {
   Frags h_frags;
   contour3d_amd_mesh mesh;  // the frame's triangles, kept between frames
   std::vector<Geom*> geoms_used;  // lattice tables the frame in flight reads
+  TessLayout tl;
+  // the captured launch sequence of the last frame layout, replayed while
+  // the layout (sizes, buffers, capacities) stays the same
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint64_t> gsig;
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    gsig.clear();
+  }
   Prof prof;
   bool fp64 = true;         // form of the frame in flight
+  FrameHdr last{};          // header of the last finished frame
   template <typename R>
   std::vector<RasterJob<R>>& rjobs_host();
   hipError_t create() {
@@ -207,6 +229,7 @@ struct Frame {
     for (auto e : prof.pool) (void)hipEventDestroy(e);
     prof.pool.clear();
     prof.pending.clear();
+    drop_graph();
     if (done) (void)hipEventDestroy(done);
     done = nullptr;
     if (st) (void)hipStreamDestroy(st);
@@ -236,6 +259,7 @@ struct Lib {
   // workgroups of the tessellation and raster kernels (0: built-in choice)
   uint32_t tess_cap = 0;
   uint32_t raster_cap = 0;
+  bool graphs = false;  // replay captured launch sequences of repeated frame layouts
   uint32_t tess_grid(uint32_t ntiles) const {
     const uint32_t g = tess_cap && tess_cap < ntiles ? tess_cap : ntiles;
     return g < 8 ? 8 : g;
@@ -647,13 +671,12 @@ int ensure_capacity(contour3d_amd_mesh& M) {
   return 0;
 }
 
-// Count, scan, plan and emit every object of `objs` into M.  `rjobs`
-// (device, njobs entries) receive their triangle ranges and work offsets
-// from the plan kernel.  Synchronises once, after the plan, to size the
-// mesh; on return the emission is enqueued.
+// Host half of a frame's tessellation: validates the objects, resolves
+// fields (host fields are uploaded) and lattices, groups thresholds of a
+// shared field into field passes, sizes the buffers and enqueues the uploads
+// of the pass and object tables.  The layout goes to F.tl.
 template <typename R>
-int tessellate(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<const contour3d_contour_obj_t*>& objs,
-               RasterJob<R>* rjobs, int njobs, FrameHdr& hh) {
+int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<const contour3d_contour_obj_t*>& objs) {
   const size_t n = objs.size();
   std::vector<FieldDev> fd(n);
   std::vector<int> group(n, -1);
@@ -783,21 +806,47 @@ int tessellate(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<const 
     C3D_HIP(hipMemcpyAsync(M.meta.p, hm.data(), n * sizeof(ObjMeta), hipMemcpyHostToDevice, F.st));
     C3D_HIP(hipMemcpyAsync(F.tjobs.p, tj.data(), tj.size() * sizeof(TessJob), hipMemcpyHostToDevice, F.st));
   }
-  const TessJob* tjd = F.tjobs.as<TessJob>();
-  FrameHdr* hdr = F.hdr.as<FrameHdr>();
+  F.tl.nbt = nbt;
+  F.tl.ntj = ntj;
+  F.tl.njobs = int(tj.size());
+  F.tl.nobj = int(n);
+  return 0;
+}
 
+// count, scan and plan; the plan flags a frame whose triangles exceed cap
+template <typename R>
+int tess_head(Lib& L, Frame& F, contour3d_amd_mesh& M, RasterJob<R>* rjobs, int njobs, uint64_t cap) {
+  const TessLayout& tl = F.tl;
   if (prof_begin(F, CONTOUR3D_AMD_STAGE_COUNT)) return 1;
-  C3D_HIP(launch_count(tjd, int(tj.size()), ntj, F.cnt.as<uint32_t>(), L.tess_grid(ntj), F.st));
+  C3D_HIP(launch_count(F.tjobs.as<TessJob>(), tl.njobs, tl.ntj, F.cnt.as<uint32_t>(), L.tess_grid(tl.ntj), F.st));
   if (prof_end(F)) return 1;
   if (prof_begin(F, CONTOUR3D_AMD_STAGE_SCAN)) return 1;
-  if (nbt) C3D_HIP(launch_scan(F.cnt.as<uint32_t>(), M.bstart.as<uint64_t>(), nbt, F.tiles.as<uint64_t>(), F.st));
-  C3D_HIP(launch_plan<R>(M.bstart.as<uint64_t>(), nbt, M.meta.as<ObjMeta>(), int(n), rjobs, njobs, hdr,
-                         ~uint64_t(0), F.st));
+  if (tl.nbt)
+    C3D_HIP(launch_scan(F.cnt.as<uint32_t>(), M.bstart.as<uint64_t>(), tl.nbt, F.tiles.as<uint64_t>(), F.st));
+  C3D_HIP(launch_plan<R>(M.bstart.as<uint64_t>(), tl.nbt, M.meta.as<ObjMeta>(), tl.nobj, rjobs, njobs,
+                         F.hdr.as<FrameHdr>(), cap, F.st));
+  if (prof_end(F)) return 1;
+  return 0;
+}
+
+template <typename R>
+int tess_emit(Lib& L, Frame& F, contour3d_amd_mesh& M) {
+  const TessLayout& tl = F.tl;
+  if (prof_begin(F, CONTOUR3D_AMD_STAGE_EMIT)) return 1;
+  C3D_HIP(launch_emit<R>(F.tjobs.as<TessJob>(), tl.njobs, tl.ntj, M.bstart.as<uint64_t>(), view_of<R>(M),
+                         F.hdr.as<FrameHdr>(), L.tess_grid(tl.ntj), F.st));
+  if (prof_end(F)) return 1;
+  return 0;
+}
+
+// Reads the plan's totals back (one synchronisation) and sizes the mesh.
+template <typename R>
+int tess_readback(Frame& F, contour3d_amd_mesh& M, FrameHdr& hh) {
+  const size_t n = size_t(F.tl.nobj);
   unsigned char* hs = static_cast<unsigned char*>(F.hstage.p);
-  C3D_HIP(hipMemcpyAsync(hs, hdr, sizeof(FrameHdr), hipMemcpyDeviceToHost, F.st));
+  C3D_HIP(hipMemcpyAsync(hs, F.hdr.p, sizeof(FrameHdr), hipMemcpyDeviceToHost, F.st));
   if (n)
     C3D_HIP(hipMemcpyAsync(hs + sizeof(FrameHdr), M.meta.p, n * sizeof(ObjMeta), hipMemcpyDeviceToHost, F.st));
-  if (prof_end(F)) return 1;
   C3D_HIP(hipStreamSynchronize(F.st));
   std::memcpy(&hh, hs, sizeof(FrameHdr));
   for (size_t o = 0; o < n; o++) {
@@ -808,16 +857,24 @@ int tessellate(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<const 
   }
   M.total = hh.ntri;
   F.prof.ntri += M.total;
-  if (M.total >= (uint64_t(1) << 32) - (uint64_t(1) << 24)) {
+  if (M.total >= kMaxFrameTriangles) {
     log_error("%llu triangles exceed the 2^32 triangle ids of one frame", (unsigned long long)M.total);
     return 1;
   }
-  if (ensure_capacity<R>(M)) return 1;
-  if (prof_begin(F, CONTOUR3D_AMD_STAGE_EMIT)) return 1;
-  C3D_HIP(launch_emit<R>(tjd, int(tj.size()), ntj, M.bstart.as<uint64_t>(), view_of<R>(M), hdr, L.tess_grid(ntj),
-                         F.st));
-  if (prof_end(F)) return 1;
-  return 0;
+  return ensure_capacity<R>(M);
+}
+
+// Count, scan, plan and emit every object of `objs` into M.  `rjobs`
+// (device, njobs entries) receive their triangle ranges and work offsets
+// from the plan kernel.  Synchronises once, after the plan, to size the
+// mesh; on return the emission is enqueued.
+template <typename R>
+int tessellate(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<const contour3d_contour_obj_t*>& objs,
+               RasterJob<R>* rjobs, int njobs, FrameHdr& hh) {
+  if (tess_prepare<R>(L, F, M, objs)) return 1;
+  if (tess_head<R>(L, F, M, rjobs, njobs, ~uint64_t(0))) return 1;
+  if (tess_readback<R>(F, M, hh)) return 1;
+  return tess_emit<R>(L, F, M);
 }
 
 // Line fragments of every image, appended to F.h_frags with frame pixel
@@ -843,20 +900,15 @@ int frame_lines(Frame& F, const std::vector<Image>& imgs) {
   return 0;
 }
 
-// z-buffer passes, lines and resolve of every image of a frame over the
-// mesh M, then the copies to host outputs; frame_finish waits for them.
+// Uploads the frame's line fragments (and their count into the header) and
+// sizes the z-buffer and the FP64 fragment list.
 template <typename R>
-int raster_frame(Lib& L, Frame& F, const contour3d_amd_mesh& M, std::vector<Image>& imgs, const RasterJob<R>* rjobs,
-                 uint64_t npx, const FrameHdr& hh) {
+int raster_prepare(Frame& F, uint64_t npx) {
   const bool fp64 = sizeof(R) == 8;
-  const int njobs = int(imgs.size());
   const Frags& LF = F.h_frags;
   const uint32_t nf = uint32_t(LF.pix.size());
-  FrameHdr* hdr = F.hdr.as<FrameHdr>();
   C3D_HIP(F.key.ensure(npx * sizeof(uint64_t)));
   if (fp64) C3D_HIP(F.id.ensure(npx * sizeof(uint32_t)));
-  uint32_t* id = fp64 ? F.id.as<uint32_t>() : nullptr;
-  uint64_t* key = F.key.as<uint64_t>();
   C3D_HIP(F.fpix.ensure((nf + 1) * sizeof(uint32_t)));
   C3D_HIP(F.fdep.ensure((nf + 1) * sizeof(double)));
   C3D_HIP(F.frgb.ensure(3 * size_t(nf + 1)));
@@ -866,15 +918,31 @@ int raster_frame(Lib& L, Frame& F, const contour3d_amd_mesh& M, std::vector<Imag
     C3D_HIP(hipMemcpyAsync(F.frgb.p, LF.rgb.data(), 3 * size_t(nf), hipMemcpyHostToDevice, F.st));
   }
   F.h_nline = nf;
-  C3D_HIP(hipMemcpyAsync(reinterpret_cast<unsigned char*>(hdr) + offsetof(FrameHdr, nline), &F.h_nline,
-                         sizeof(uint32_t), hipMemcpyHostToDevice, F.st));
+  C3D_HIP(hipMemcpyAsync(F.hdr.as<unsigned char>() + offsetof(FrameHdr, nline), &F.h_nline, sizeof(uint32_t),
+                         hipMemcpyHostToDevice, F.st));
+  if (fp64) {
+    if (F.frag_cap == 0) F.frag_cap = 1u << 22;
+    C3D_HIP(F.frags.ensure(size_t(F.frag_cap) * sizeof(Frag)));
+  }
+  return 0;
+}
+
+// line fragments the buffers hold (the line kernels' grid in a replayed frame)
+uint32_t line_capacity(const Frame& F) {
+  const size_t n = std::min(std::min(F.fpix.bytes / sizeof(uint32_t), F.fdep.bytes / sizeof(double)), F.frgb.bytes / 3);
+  return uint32_t(std::min<size_t>(n, 0x7fffffff));
+}
+
+// z-buffer passes, lines and resolve of every image over the mesh M;
+// `lines` sizes the line kernels' grid (at least the frame's line fragments)
+template <typename R>
+int raster_launch(Lib& L, Frame& F, const contour3d_amd_mesh& M, const RasterJob<R>* rjobs, int njobs, uint64_t npx,
+                  uint32_t rgrid, uint32_t lines) {
+  const bool fp64 = sizeof(R) == 8;
+  FrameHdr* hdr = F.hdr.as<FrameHdr>();
+  uint32_t* id = fp64 ? F.id.as<uint32_t>() : nullptr;
+  uint64_t* key = F.key.as<uint64_t>();
   const MeshView<R> mv = view_of<R>(M);
-  // FP64 rasterises with persistent workgroups (64 per CU) that overlap the
-  // next round's loads with drawing; FP32 with one workgroup per round
-  uint64_t rg = (hh.nwork + kThreads - 1) / kThreads;
-  const uint64_t rcap = L.raster_cap ? L.raster_cap : (fp64 ? uint64_t(persistent_blocks()) * 8 : 0);
-  if (rcap && rg > rcap) rg = rcap;
-  const uint32_t rgrid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(rg, 1u << 30)));
   if (prof_begin(F, CONTOUR3D_AMD_STAGE_RASTER)) return 1;
   C3D_HIP(launch_zinit(key, id, npx, fp64 ? 0 : 1, hdr, F.st));
   if constexpr (sizeof(R) == 8) {
@@ -882,25 +950,38 @@ int raster_frame(Lib& L, Frame& F, const contour3d_amd_mesh& M, std::vector<Imag
     // pass over that list; when the list overflows, the geometry id pass
     // runs instead (both decide on the device) and the list grows for the
     // next frame
-    if (F.frag_cap == 0) F.frag_cap = 1u << 22;
-    C3D_HIP(F.frags.ensure(size_t(F.frag_cap) * sizeof(Frag)));
     C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, kRasterDepth64, key, id, F.frags.as<Frag>(), F.frag_cap, rgrid,
                              F.st));
-    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), nf, hdr, 0, 0, key, id, F.st));
+    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 0, key, id, F.st));
     C3D_HIP(launch_frag_ids(F.frags.as<Frag>(), F.frag_cap, hdr, key, id,
                             std::min<uint32_t>((F.frag_cap + 255) / 256, 4096), F.st));
     C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, kRasterId64, key, id, nullptr, 0, uint32_t(persistent_blocks()),
                              F.st));
-    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), nf, hdr, 1, 0, key, id, F.st));
+    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 1, 0, key, id, F.st));
   } else {
     C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, kRasterFp32, key, id, nullptr, 0, rgrid, F.st));
-    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), nf, hdr, 0, 1, key, id, F.st));
+    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 1, key, id, F.st));
   }
   if (prof_end(F)) return 1;
   if (prof_begin(F, CONTOUR3D_AMD_STAGE_RESOLVE)) return 1;
   C3D_HIP(launch_resolve<R>(mv, rjobs, njobs, npx, M.meta.as<ObjMeta>(), int(M.base.size()), F.frgb.as<uint8_t>(),
                             hdr, key, id, 1u << 30, F.st));
   if (prof_end(F)) return 1;
+  return 0;
+}
+
+// raster workgroups: FP64 rasterises with persistent workgroups (64 per CU)
+// that overlap the next round's loads with drawing, FP32 with one workgroup
+// per round unless `persistent`
+uint32_t raster_grid(const Lib& L, bool fp64, uint64_t nwork, bool persistent) {
+  uint64_t rg = (nwork + kThreads - 1) / kThreads;
+  const uint64_t rcap = L.raster_cap ? L.raster_cap : ((fp64 || persistent) ? uint64_t(persistent_blocks()) * 8 : 0);
+  if (rcap && rg > rcap) rg = rcap;
+  return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(rg, 1u << 30)));
+}
+
+// copies to host outputs and the header, then the frame's completion event
+int raster_copies(Frame& F, const std::vector<Image>& imgs, bool fp64) {
   for (const Image& im : imgs) {
     const uint64_t n = uint64_t(im.hv.W) * uint64_t(im.hv.H);
     if (!im.rgb_dev)
@@ -909,10 +990,21 @@ int raster_frame(Lib& L, Frame& F, const contour3d_amd_mesh& M, std::vector<Imag
       C3D_HIP(hipMemcpyAsync(im.depth, F.depth.as<double>() + im.px0, n * sizeof(double), hipMemcpyDeviceToHost,
                              F.st));
   }
-  C3D_HIP(hipMemcpyAsync(F.hstage.p, hdr, sizeof(FrameHdr), hipMemcpyDeviceToHost, F.st));
+  C3D_HIP(hipMemcpyAsync(F.hstage.p, F.hdr.p, sizeof(FrameHdr), hipMemcpyDeviceToHost, F.st));
   C3D_HIP(hipEventRecord(F.done, F.st));
   F.fp64 = fp64;
   return 0;
+}
+
+// z-buffer passes, lines and resolve of every image of a frame over the
+// mesh M, then the copies to host outputs; frame_finish waits for them.
+template <typename R>
+int raster_frame(Lib& L, Frame& F, const contour3d_amd_mesh& M, std::vector<Image>& imgs, const RasterJob<R>* rjobs,
+                 uint64_t npx, const FrameHdr& hh) {
+  if (raster_prepare<R>(F, npx)) return 1;
+  const uint32_t rgrid = raster_grid(L, sizeof(R) == 8, hh.nwork, false);
+  if (raster_launch<R>(L, F, M, rjobs, int(imgs.size()), npx, rgrid, F.h_nline)) return 1;
+  return raster_copies(F, imgs, sizeof(R) == 8);
 }
 
 void release_geoms(Frame& F) {
@@ -926,7 +1018,7 @@ int frame_finish(Frame& F) {
   const hipError_t e = hipEventSynchronize(F.done);
   release_geoms(F);
   C3D_HIP(e);
-  FrameHdr fin;
+  FrameHdr& fin = F.last;
   std::memcpy(&fin, F.hstage.p, sizeof(FrameHdr));
   if (F.fp64 && (fin.overflow & 2u) && fin.nfrag > F.frag_cap) {
     const uint64_t grow = uint64_t(fin.nfrag) + fin.nfrag / 4;
@@ -1009,9 +1101,80 @@ void fill_rjobs(Frame& F, const std::vector<Image>& imgs) {
   }
 }
 
+// The frame's launch sequence (count ... resolve) as a graph, replayed
+// while its layout stays the same: the kernels' arguments and grids depend
+// only on the layout, and everything else (fields, thresholds, colours,
+// cameras, line fragments, outputs) is read from the device tables uploaded
+// before each launch.  The plan kernel flags a frame whose triangles exceed
+// the capacity reserved by earlier frames; the caller then renders it again
+// eagerly.  The first frame of a layout, and one whose line fragments
+// outgrow their buffers, runs eagerly.
+template <typename R>
+int graph_frame(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& imgs,
+                const std::vector<const contour3d_contour_obj_t*>& objs, RasterJob<R>* rjd, uint64_t npx,
+                bool& replayed) {
+  const bool fp64 = sizeof(R) == 8;
+  const int njobs = int(imgs.size());
+  if (tess_prepare<R>(L, F, M, objs)) return 1;
+  if (raster_prepare<R>(F, npx)) return 1;
+  const bool ready = M.cap > 0 && M.precision_built == M.precision && M.cap < kMaxFrameTriangles &&
+                     F.h_nline <= line_capacity(F);
+  if (!ready) {
+    F.drop_graph();
+    FrameHdr hh{};
+    if (tess_head<R>(L, F, M, rjd, njobs, ~uint64_t(0))) return 1;
+    if (tess_readback<R>(F, M, hh)) return 1;
+    if (tess_emit<R>(L, F, M)) return 1;
+    const uint32_t rgrid = raster_grid(L, fp64, hh.nwork, false);
+    if (raster_launch<R>(L, F, M, rjd, njobs, npx, rgrid, F.h_nline)) return 1;
+    return raster_copies(F, imgs, fp64);
+  }
+  const uint32_t lines = line_capacity(F);
+  const uint32_t rgrid = raster_grid(L, fp64, ~uint64_t(0) >> 20, true);
+  const std::vector<uint64_t> sig = {
+      sizeof(R), uint64_t(njobs), npx, F.tl.nbt, F.tl.ntj, uint64_t(F.tl.njobs), uint64_t(F.tl.nobj),
+      L.tess_cap, rgrid, lines, M.cap, F.frag_cap,
+      uint64_t(uintptr_t(F.cnt.p)), uint64_t(uintptr_t(F.tiles.p)), uint64_t(uintptr_t(F.tjobs.p)),
+      uint64_t(uintptr_t(F.rjobs.p)), uint64_t(uintptr_t(F.hdr.p)), uint64_t(uintptr_t(F.key.p)),
+      uint64_t(uintptr_t(F.id.p)), uint64_t(uintptr_t(F.frags.p)), uint64_t(uintptr_t(F.fpix.p)),
+      uint64_t(uintptr_t(F.fdep.p)), uint64_t(uintptr_t(F.frgb.p)), uint64_t(uintptr_t(M.bstart.p)),
+      uint64_t(uintptr_t(M.meta.p)), uint64_t(uintptr_t(M.pos.p)), uint64_t(uintptr_t(M.wfn.p)),
+      uint64_t(uintptr_t(M.cell.p)), uint64_t(uintptr_t(M.edges.p))};
+  if (!F.gexec || F.gsig != sig) {
+    F.drop_graph();
+    (void)persistent_blocks();  // device query outside the capture
+    const bool prof_on = F.prof.on;
+    F.prof.on = false;
+    C3D_HIP(hipStreamBeginCapture(F.st, hipStreamCaptureModeThreadLocal));
+    int rc = tess_head<R>(L, F, M, rjd, njobs, M.cap);
+    if (!rc) rc = tess_emit<R>(L, F, M);
+    if (!rc) rc = raster_launch<R>(L, F, M, rjd, njobs, npx, rgrid, lines);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(F.st, &g);
+    F.prof.on = prof_on;
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      log_error("graph capture failed: %s", hipGetErrorString(e));
+      return 1;
+    }
+    const hipError_t ei = hipGraphInstantiate(&F.gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      F.gexec = nullptr;
+      log_error("graph instantiation failed: %s", hipGetErrorString(ei));
+      return 1;
+    }
+    F.gsig = sig;
+  }
+  C3D_HIP(hipGraphLaunch(F.gexec, F.st));
+  replayed = true;
+  return raster_copies(F, imgs, fp64);
+}
+
 // Tessellate the objects of every image and render them all.
 template <typename R>
-int run_frame_t(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& imgs) {
+int run_frame_t(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& imgs, bool graph, bool& replayed) {
+  replayed = false;
   uint64_t npx = 0;
   if (frame_setup(F, imgs, npx)) return 1;
   // the frame's object list: each image's objects, unless an earlier image
@@ -1042,29 +1205,44 @@ int run_frame_t(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& img
   C3D_HIP(hipMemcpyAsync(F.rjobs.p, F.rjobs_host<R>().data(), imgs.size() * sizeof(RasterJob<R>),
                          hipMemcpyHostToDevice, F.st));
   RasterJob<R>* rjd = F.rjobs.as<RasterJob<R>>();
+  const int njobs = int(imgs.size());
+  if (graph) return graph_frame<R>(L, F, M, imgs, objs, rjd, npx, replayed);
   FrameHdr hh{};
-  if (tessellate<R>(L, F, M, objs, rjd, int(imgs.size()), hh)) return 1;
+  if (tessellate<R>(L, F, M, objs, rjd, njobs, hh)) return 1;
   return raster_frame<R>(L, F, M, imgs, rjd, npx, hh);
 }
 
 // Renders the images as one frame into F; with wait, also waits for the
-// frame (otherwise frame_finish must be called before F is reused).
+// frame (otherwise frame_finish must be called before F is reused).  With
+// graphs on, a waited frame replays the captured launch sequence of its
+// layout; a replay whose triangles outgrew the reserved capacity is
+// rendered again eagerly.
 int run_frame(Lib& L, Frame& F, std::vector<Image>& imgs, int precision, bool wait = true) {
-  int rc;
-  if (precision == CONTOUR3D_AMD_FP64) {
-    rc = run_frame_t<double>(L, F, F.mesh, imgs);
-  } else if (precision == CONTOUR3D_AMD_FP32) {
-    rc = run_frame_t<float>(L, F, F.mesh, imgs);
-  } else {
+  if (precision != CONTOUR3D_AMD_FP64 && precision != CONTOUR3D_AMD_FP32) {
     log_error("unknown precision %d", precision);
     return 1;
   }
-  if (rc) {
-    (void)hipStreamSynchronize(F.st);
-    release_geoms(F);
-    return rc;
+  const bool graph = wait && L.graphs;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    bool replayed = false;
+    const bool g = graph && attempt == 0;
+    const int rc = precision == CONTOUR3D_AMD_FP64 ? run_frame_t<double>(L, F, F.mesh, imgs, g, replayed)
+                                                   : run_frame_t<float>(L, F, F.mesh, imgs, g, replayed);
+    if (rc) {
+      (void)hipStreamSynchronize(F.st);
+      release_geoms(F);
+      F.drop_graph();
+      return rc;
+    }
+    if (!wait) return 0;
+    if (frame_finish(F)) {
+      F.drop_graph();
+      return 1;
+    }
+    if (!(replayed && (F.last.overflow & 1u))) return 0;
+    F.drop_graph();  // the mesh must grow: render again eagerly
   }
-  return wait ? frame_finish(F) : 0;
+  return 0;
 }
 
 // Render an already built mesh (all its objects) as one image.
@@ -1410,6 +1588,10 @@ int contour3d_amd_tune(int param, int64_t value) {
       return 0;
     case CONTOUR3D_AMD_TUNE_FRAG_CAPACITY:
       L.main.frag_cap = uint32_t(value ? value : 1);
+      return 0;
+    case CONTOUR3D_AMD_TUNE_GRAPHS:
+      L.graphs = value != 0;
+      if (!L.graphs) L.main.drop_graph();
       return 0;
     default:
       c3d::log_error("tune: unknown parameter %d", param);

@@ -648,6 +648,9 @@ __global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, const Raster
                                                      uint32_t cap) {
   __shared__ Frag s_frag[MODE == kRasterDepth64 ? kFragLds : 1];
   __shared__ uint32_t s_n, s_base;
+  // a frame whose triangles outgrew the mesh (bit 1, set by the plan of a
+  // replayed frame) has triangles that were never stored: draw nothing
+  if (hdr->overflow & 1u) return;
   if (MODE == kRasterId64 && !(hdr->overflow & 2u)) return;
   const uint64_t nwork = hdr->nwork;
   const uint64_t stride = uint64_t(gridDim.x) * kThreads;
@@ -784,6 +787,8 @@ __global__ __launch_bounds__(kThreads) void k_resolve(MeshView<R> m, const Raste
                                                       const FrameHdr* __restrict__ hdr,
                                                       const uint64_t* __restrict__ key,
                                                       const uint32_t* __restrict__ idbuf) {
+  // a frame whose triangles outgrew the mesh is rendered again: skip it
+  if (hdr->overflow & 1u) return;
   const uint64_t ntri = hdr->ntri;
   for (uint64_t p0 = uint64_t(blockIdx.x) * kThreads; p0 < npx_total; p0 += uint64_t(gridDim.x) * kThreads) {
     // images start on workgroup boundaries of the pixel space, so the
