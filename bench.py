@@ -1,16 +1,27 @@
-"""Benchmark: one frame of the tessellate -> smooth -> render path.
+"""Benchmark: the tessellate -> smooth -> render path on one MI355X.
 
-Workload (the plain method at GPU scale): a 512^3 float64 spectral
+Headline workload (the plain method at GPU scale): a 512^3 float64 spectral
 turbulence field (modes 1 <= |k| <= 64, k^-5/3 shell spectrum, seeded
 phases, mean 0 / variance 1) resident in HBM, thresholds {-1, 0, 1} as three
 contour objects, one 1920x1080 image written to HBM.  A step is one
 contour3d_amd_execute_rgb call; Gcells/s counts 3 * 511^3 cell-thresholds
-per frame.
+per frame.  `value` is the FP64 form (bit-identical to the reference).
 
-Prints one JSON line with the throughput of the FP64 form (bit-identical to
-the reference), the FP32-geometry form beside it, per-stage times from HIP
-events on the library stream, the HBM roofline of the dominant kernel, and
-the CPU oracle timed on a bounded slab of the same field.
+Beside it, in the same JSON line:
+  fp32        the FP32-geometry form of the same frame
+  batched     (1) four cameras on the same 512^3 scene: plain single-image
+              calls (FP64 and FP32) against one batched call (FP32, eager
+              and graph-replayed) and the serving loop; (2) 64 fields of
+              uneven size (each axis log-uniform in 64..384), 1-4 thresholds
+              and a camera each, 1024^2 images, in one frame, eager and
+              graph-replayed, with autotuned workgroup caps
+  served      Poisson arrivals (50/s) of FP32 requests against a pool of
+              device-resident fields (64^3..768^3, updated in place between
+              steps), 1-4 thresholds, images 512^2..2048^2, random cameras;
+              p50/p99 latency and frames/s of the serving loop, next to a
+              plain loop that renders the same arrivals one at a time
+  roofline    HBM roofline of the dominant kernel of the FP64 frame
+  cpu_baseline  the CPU oracle timed on a bounded slab of the same field
 """
 import argparse
 import json
@@ -93,6 +104,260 @@ def cpu_baseline(field_dev, view):
                        f"({cells / 1e6:.1f} M cell-thresholds) into the 1920x1080 view, {dt:.1f} s")
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _img(view, contours, rgb=None):
+    return dict(camera=view["camera"], light=view["light"], size=view["size"], center=view["center"],
+                local=view["local"], bg=view.get("bg", (0, 0, 0)), contours=contours, rgb=rgb)
+
+
+def _timed(fn, reps):
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def shared_scene(api, fields, contours, reps):
+    """Four cameras on the 512^3 scene: plain single-image calls against one
+    batched call (the batch tessellates the shared scene once)."""
+    import torch
+
+    views = [fields.example_view(W, H)] + [fields.sphere_view(e, a, W, H) for e, a in ((0.5, 0.7), (0.2, 2.6),
+                                                                                      (0.9, 4.4))]
+    outs = [torch.zeros((H, W, 3), dtype=torch.uint8, device="cuda") for _ in views]
+    imgs = [_img(v, contours, o) for v, o in zip(views, outs)]
+
+    def plain(prec):
+        for im in imgs:
+            api.execute_rgb(im["camera"], im["light"], im["size"], im["center"], im["local"], im["bg"], contours,
+                            precision=prec, rgb=im["rgb"])
+
+    res = {"cameras": len(views), "image": [W, H]}
+    for name, prec in (("fp64", api.FP64), ("fp32", api.FP32)):
+        plain(prec)
+        res[f"plain_{name}_fps"] = round(len(views) / _timed(lambda: plain(prec), reps), 2)
+    api.execute_batch(imgs, precision=api.FP32)
+    res["batched_fp32_fps"] = round(len(views) / _timed(lambda: api.execute_batch(imgs, precision=api.FP32), reps), 2)
+    api.execute_batch(imgs, precision=api.FP64)
+    res["batched_fp64_fps"] = round(len(views) / _timed(lambda: api.execute_batch(imgs, precision=api.FP64), reps), 2)
+    api.tune(api.TUNE_GRAPHS, 1)
+    try:
+        api.execute_batch(imgs, precision=api.FP32)
+        api.execute_batch(imgs, precision=api.FP32)
+        res["batched_fp32_graph_fps"] = round(
+            len(views) / _timed(lambda: api.execute_batch(imgs, precision=api.FP32), reps), 2)
+    finally:
+        api.tune(api.TUNE_GRAPHS, 0)
+    # the serving loop, all four requests submitted together each round
+    with api.Server(max_images=16) as srv:
+        def served():
+            ts = [srv.submit(im, api.FP32) for im in imgs]
+            for t in ts:
+                srv.wait(t)
+        served()
+        res["served_fp32_fps"] = round(len(views) / _timed(served, reps), 2)
+    res["batched_fp32_vs_plain_fp64"] = round(res["batched_fp32_fps"] / res["plain_fp64_fps"], 3)
+    res["batched_fp32_vs_plain_fp32"] = round(res["batched_fp32_fps"] / res["plain_fp32_fps"], 3)
+    return res
+
+
+def uneven_batch_frame(api, fields, nfields, side, reps, seed=0):
+    """64 fields of uneven, non-cubic size (each axis round(exp(U[ln 64,
+    ln 384]))), half Gaussian blobs and half spectral turbulence (k_max
+    clamped to min axis / 2 - 1), each with U{1..4} thresholds and its own
+    camera uniform on the sphere, side x side images; FP32 form."""
+    import math
+
+    import torch
+
+    rng = np.random.default_rng(seed)
+    imgs, cells, keep = [], 0, []
+    for n in range(nfields):
+        shape = tuple(int(round(math.exp(rng.uniform(math.log(64), math.log(384))))) for _ in range(3))
+        if rng.uniform() < 0.5:
+            f = fields.blobs_device(shape, seed=seed + n)
+            thr = rng.uniform(0.2, 1.0, size=rng.integers(1, 5))
+        else:
+            f = fields.turbulence_fast_device(shape, kmax=max(1, min(64, min(shape) // 2 - 1)), seed=seed + n)
+            thr = rng.uniform(-1.5, 1.5, size=rng.integers(1, 5))
+        keep.append(f)
+        elev = math.asin(rng.uniform(-1.0, 1.0))
+        azim = rng.uniform(0.0, 2 * math.pi)
+        v = fields.sphere_view(elev, azim, side, side)
+        cont = [dict(field=f, threshold=float(t), color=tuple(int(x) for x in rng.integers(40, 256, size=3)))
+                for t in thr]
+        cells += len(thr) * (shape[0] - 1) * (shape[1] - 1) * (shape[2] - 1)
+        imgs.append(_img(v, cont, torch.zeros((side, side, 3), dtype=torch.uint8, device="cuda")))
+    torch.cuda.synchronize()
+    run = lambda: api.execute_batch(imgs, precision=api.FP32)  # noqa: E731
+    run()
+    tuned, _ = api.autotune(run, reps=1)
+    api.profile_reset()
+    api.profile_enable(True)
+    eager = _timed(run, reps)
+    api.profile_enable(False)
+    _, _, ntri = api.profile_read()
+    api.tune(api.TUNE_GRAPHS, 1)
+    try:
+        run()
+        run()
+        graph = _timed(run, reps)
+    finally:
+        api.tune(api.TUNE_GRAPHS, 0)
+        api.tune(api.TUNE_TESS_GRID, 0)
+        api.tune(api.TUNE_RASTER_GRID, 0)
+    return {"fields": nfields, "images": nfields, "image": [side, side], "precision": "fp32",
+            "cell_thresholds": cells, "triangles": int(ntri / max(1, reps)),
+            "eager_ms": round(eager * 1e3, 3), "graph_ms": round(graph * 1e3, 3),
+            "frames_per_s_eager": round(nfields / eager, 1), "frames_per_s_graph": round(nfields / graph, 1),
+            "gcells_per_s_graph": round(cells / graph / 1e9, 2), "autotuned_workgroup_caps": tuned}
+
+
+def served_stream(api, fields, nreq, rate, nfield_pool, seed=0):
+    """Poisson arrivals against a pool of device-resident turbulence fields
+    (sides log-uniform in 64..768) that a simulated solver advances in place
+    (sign flip, a half-period phase shift) whenever no request reads them;
+    1-4 thresholds, square images with log-uniform side in 512..2048,
+    cameras uniform on the sphere, FP32 form.  The same arrival times are
+    then replayed through a plain loop of single execute_rgb calls."""
+    import math
+    import threading
+
+    import torch
+
+    rng = np.random.default_rng(seed)
+    pool = []
+    for n in range(nfield_pool):
+        side = int(round(math.exp(rng.uniform(math.log(64), math.log(768)))))
+        pool.append(fields.turbulence_fast_device((side, side, side), kmax=min(64, side // 2 - 1), seed=100 + n))
+    torch.cuda.synchronize()
+    reqs, t = [], 0.0
+    for q in range(nreq):
+        t += rng.exponential(1.0 / rate)
+        fi = int(rng.integers(0, nfield_pool))
+        thr = rng.uniform(-1.5, 1.5, size=rng.integers(1, 5))
+        side = int(round(math.exp(rng.uniform(math.log(512), math.log(2048)))))
+        elev = math.asin(rng.uniform(-1.0, 1.0))
+        azim = rng.uniform(0.0, 2 * math.pi)
+        reqs.append((t, fi, [float(x) for x in thr], side, fields.sphere_view(elev, azim, side, side)))
+    cols = ((255, 80, 40), (40, 220, 90), (60, 120, 255), (230, 230, 60))
+
+    def image_of(r):
+        _, fi, thr, side, v = r
+        cont = [dict(field=pool[fi], threshold=x, color=cols[k]) for k, x in enumerate(thr)]
+        return _img(v, cont, np.zeros((side, side, 3), np.uint8))
+
+    # both paths first render the largest requests once, so that neither
+    # times the growth of its buffers
+    big = sorted(reqs, key=lambda r: -(pool[r[1]].numel() * len(r[2]) + r[3] ** 2))[:3]
+    for r in big:
+        im = image_of(r)
+        api.execute_rgb(im["camera"], im["light"], im["size"], im["center"], im["local"], im["bg"], im["contours"],
+                        precision=api.FP32, rgb=im["rgb"])
+
+    # serving loop
+    lock = threading.Lock()
+    inflight = [0] * nfield_pool
+    lat = []
+    srv = api.Server(max_images=16, hbm_budget=200e9)
+    order = []
+    done = threading.Event()
+
+    def reaper():
+        k = 0
+        while True:
+            with lock:
+                if k >= len(order):
+                    if done.is_set():
+                        return
+                    item = None
+                else:
+                    item = order[k]
+            if item is None:
+                time.sleep(0.0005)
+                continue
+            ticket, fi = item
+            _, _, ms = srv.wait(ticket)
+            with lock:
+                inflight[fi] -= 1
+                lat.append(ms)
+            k += 1
+
+    for r in big:
+        srv.wait(srv.submit(image_of(r), api.FP32))
+    th = threading.Thread(target=reaper)
+    th.start()
+    t0 = time.perf_counter()
+    updates = 0
+    for q, r in enumerate(reqs):
+        while time.perf_counter() - t0 < r[0]:
+            time.sleep(0.0002)
+        if q % 8 == 7:
+            # a solver step: advance one field that no queued request reads
+            with lock:
+                free = [i for i in range(nfield_pool) if inflight[i] == 0]
+            if free:
+                pool[free[q % len(free)]].neg_()
+                torch.cuda.synchronize()
+                updates += 1
+        im = image_of(r)
+        with lock:
+            inflight[r[1]] += 1
+        ticket = srv.submit(im, api.FP32)
+        with lock:
+            order.append((ticket, r[1]))
+    done.set()
+    th.join()
+    st = srv.stats()
+    # saturation: every request submitted at once (the first pass sizes the
+    # buffers for the larger batches, the second is timed)
+    nsat = min(48, nreq)
+    for rep in range(2):
+        t1 = time.perf_counter()
+        ts = [srv.submit(image_of(r), api.FP32) for r in reqs[:nsat]]
+        for t_ in ts:
+            srv.wait(t_)
+        sat_served = nsat / (time.perf_counter() - t1)
+    sat_batch = srv.stats()["mean_batch"]
+    srv.stop()
+    lat.sort()
+    served = {"requests": nreq, "rate_per_s": rate, "precision": "fp32", "field_pool": nfield_pool,
+              "solver_updates": updates, "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
+              "frames_per_s": round(st["frames_per_s"], 2), "mean_batch": round(st["mean_batch"], 2),
+              "peak_admitted_gb": round(st["peak_admitted_bytes"] / 1e9, 2)}
+    # the same arrivals through plain single-image calls, one at a time
+    plat = []
+    t0 = time.perf_counter()
+    for r in reqs:
+        while time.perf_counter() - t0 < r[0]:
+            time.sleep(0.0002)
+        im = image_of(r)
+        api.execute_rgb(im["camera"], im["light"], im["size"], im["center"], im["local"], im["bg"], im["contours"],
+                        precision=api.FP32, rgb=im["rgb"])
+        plat.append((time.perf_counter() - t0 - r[0]) * 1e3)
+    span = time.perf_counter() - t0
+    plat.sort()
+    pct = lambda v, q: v[min(len(v) - 1, int(q * (len(v) - 1) + 0.5))]  # noqa: E731
+    served["plain"] = {"p50_ms": round(pct(plat, 0.5), 2), "p99_ms": round(pct(plat, 0.99), 2),
+                       "frames_per_s": round(nreq / span, 2)}
+    t1 = time.perf_counter()
+    for r in reqs[:nsat]:
+        im = image_of(r)
+        api.execute_rgb(im["camera"], im["light"], im["size"], im["center"], im["local"], im["bg"], im["contours"],
+                        precision=api.FP32, rgb=im["rgb"])
+    sat_plain = nsat / (time.perf_counter() - t1)
+    served["saturated"] = {"requests": nsat, "served_frames_per_s": round(sat_served, 2),
+                           "plain_frames_per_s": round(sat_plain, 2), "mean_batch_all": round(sat_batch, 2)}
+    return served
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +367,10 @@ def main():
     ap.add_argument("--n", type=int, default=N, help="lattice points per axis (default 512)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device three times (reference kernel for HBM counters)")
+    ap.add_argument("--forms", default="batched,served",
+                    help="comma list of extra forms to run beside the headline frame: batched, served (or none)")
+    ap.add_argument("--batch-fields", type=int, default=64)
+    ap.add_argument("--served-requests", type=int, default=150)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -183,8 +452,17 @@ def main():
                  "frames_per_s": round(1e3 / results["fp32"]["frame_ms"], 2),
                  "stages": results["fp32"]["stages"]},
     }
+    forms = [f for f in args.forms.split(",") if f and f != "none"]
+    if "batched" in forms:
+        log("batched: four cameras on the shared 512^3 scene")
+        out["batched"] = {"shared_scene": shared_scene(api, fields, contours, max(2, args.steps // 2))}
+        log(f"batched: {args.batch_fields} uneven fields, 1024^2 images")
+        out["batched"]["uneven"] = uneven_batch_frame(api, fields, args.batch_fields, 1024, 3)
+    if "served" in forms:
+        log(f"served: {args.served_requests} Poisson arrivals at 50/s")
+        out["served"] = served_stream(api, fields, args.served_requests, 50.0, 8)
     if not args.no_cpu:
-        print("[bench] cpu oracle baseline", file=sys.stderr, flush=True)
+        log("cpu oracle baseline")
         out["cpu_baseline"] = cpu_baseline(field, view)
     print(json.dumps(out), flush=True)
 

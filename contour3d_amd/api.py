@@ -230,16 +230,18 @@ def execute_batch(images, precision=FP64):
 def _fill_image(e, im, keep, made):
     """Fills contour3d_amd_image_t `e` from an execute_batch dict; returns
     its (rgb, depth) outputs.  `made` maps id(contours list) to the ctypes
-    array already built for it."""
+    array built for that list; the entry keeps the list (so its id is not
+    reused) and the buffers the array points into."""
     W, H = int(im["size"][0]), int(im["size"][1])
     cs = im.get("contours", ())
     key = id(cs)
-    if key not in made:
-        made[key] = (contour_objs(cs, keep), len(cs))
-    objs, nc = made[key]
+    if key not in made or made[key][3] is not cs:
+        ck = _Keep()
+        made[key] = (contour_objs(cs, ck), len(cs), ck, cs)
+    objs, nc = made[key][0], made[key][1]
     lines = im.get("lines", ())
     lobjs = line_objs(lines, keep)
-    keep.items += [objs, lobjs, cs]
+    keep.items += [objs, lobjs]
     rgb = im.get("rgb")
     if rgb is None:
         rgb = np.zeros((H, W, 3), np.uint8)
@@ -274,14 +276,13 @@ class Server:
         if lib().contour3d_amd_server_start(C.byref(o)) != 0:
             raise RuntimeError("contour3d_amd_server_start failed")
         self._pending = {}
-        self._made = {}
-        self._keep = _Keep()
 
     def submit(self, image, precision=FP64):
         keep = _Keep()
         e = _abi.Image()
-        out = _fill_image(e, image, keep, self._made)
-        self._keep.items += keep.items
+        made = {}
+        out = _fill_image(e, image, keep, made)
+        keep.items.append(made)
         t = C.c_uint64()
         if lib().contour3d_amd_server_submit(C.byref(e), int(precision), C.byref(t)) != 0:
             raise RuntimeError("contour3d_amd_server_submit failed")
@@ -305,8 +306,6 @@ class Server:
     def stop(self):
         lib().contour3d_amd_server_stop()
         self._pending.clear()
-        self._made.clear()
-        self._keep = _Keep()
 
     def __enter__(self):
         return self
@@ -399,6 +398,33 @@ def tune(param, value):
     not depend on them)."""
     if lib().contour3d_amd_tune(int(param), int(value)) != 0:
         raise RuntimeError("contour3d_amd_tune failed")
+
+
+def autotune(run, tess=(0, 2048, 4096, 8192), raster=(0, 4096, 16384, 32768), reps=2):
+    """Times run() under each tessellation and raster workgroup cap (results
+    are the same bits under every setting) and keeps the fastest pair.
+    Returns ({"tess": t, "raster": r}, best_ms)."""
+    import time
+
+    import torch
+
+    best = None
+    for t in tess:
+        for r in raster:
+            tune(TUNE_TESS_GRID, t)
+            tune(TUNE_RASTER_GRID, r)
+            run()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                run()
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) / reps * 1e3
+            if best is None or ms < best[1]:
+                best = ({"tess": t, "raster": r}, ms)
+    tune(TUNE_TESS_GRID, best[0]["tess"])
+    tune(TUNE_RASTER_GRID, best[0]["raster"])
+    return best
 
 
 def set_precision(p):

@@ -59,7 +59,9 @@ struct DevBuf {
       bytes = 0;
       if (e != hipSuccess) return e;
     }
-    size_t n = want + want / 8 + 256;
+    // a quarter of headroom: a growing workload reallocates rarely (each
+    // reallocation frees first, which waits for the device)
+    size_t n = want + want / 4 + 256;
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) {
       p = nullptr;
@@ -1326,7 +1328,8 @@ struct Flight {
   std::vector<std::shared_ptr<Request>> reqs;
   uint64_t bytes = 0;
   double cells = 0;
-  bool done = false;  // set by the stream's host function
+  double triangles = 0;  // of the frame, known once it is enqueued
+  bool done = false;     // set by the stream's host function
 };
 
 struct Server {
@@ -1371,18 +1374,27 @@ uint64_t estimate_bytes(const Server& S, const contour3d_amd_image_t& im, int pr
   return uint64_t(1.25 * tri * per_tri + upload + npx * per_px) + (uint64_t(1) << 20);
 }
 
-// runs on a runtime thread when the frame's stream reaches it; the flight
-// stays queued until the worker has seen `done`
+void complete_locked(Server& S, Flight& f, int rc, double tri_total);
+
+// Runs on a runtime thread once the frame's stream has passed its output
+// copies: the requests' images are in their buffers, so they complete here
+// (waiters wake at once); the worker then retires the frame.  Calls no HIP
+// function.
 void on_frame_done(void* p) {
   Flight* f = static_cast<Flight*>(p);
   std::lock_guard<std::mutex> g(f->srv->m);
   f->done = true;
+  complete_locked(*f->srv, *f, 0, f->triangles);
   f->srv->cv_work.notify_all();
 }
 
 void complete(Server& S, Flight& f, int rc, double tri_total) {
-  const double t = now_s();
   std::lock_guard<std::mutex> g(S.m);
+  complete_locked(S, f, rc, tri_total);
+}
+
+void complete_locked(Server& S, Flight& f, int rc, double tri_total) {
+  const double t = now_s();
   for (auto& r : f.reqs) {
     r->status = rc ? 1 : 0;
     r->t_done = t;
@@ -1434,9 +1446,8 @@ void server_loop(Lib& L, Server& S) {
       std::unique_ptr<Flight> f = std::move(S.flights.front());
       S.flights.pop_front();
       lk.unlock();
-      Frame& F = S.frames[f->slot];
-      const int rc = frame_finish(F);
-      complete(S, *f, rc, double(F.mesh.total));
+      // its requests completed in the stream's host function
+      if (frame_finish(S.frames[f->slot])) log_error("server: a frame failed after its images were copied");
       lk.lock();
       continue;
     }
@@ -1479,6 +1490,7 @@ void server_loop(Lib& L, Server& S) {
       {
         std::lock_guard<std::mutex> g(L.mu);
         rc = run_frame(L, F, imgs, prec, false);
+        fp->triangles = double(F.mesh.total);
       }
       if (!rc && hipLaunchHostFunc(F.st, on_frame_done, fp) != hipSuccess) {
         log_error("server: cannot schedule the completion of a frame");

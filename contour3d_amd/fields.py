@@ -91,6 +91,52 @@ def turbulence_device(shape, kmax=64, seed=0, kmin=1, device="cuda"):
     return f.contiguous()
 
 
+def blobs_device(shape, seed=0, nblobs=8, device="cuda"):
+    """blobs() evaluated on the GPU (same parameters; exp rounds differently
+    from numpy's, so parity tests use blobs() bytes)."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.2, 0.8, size=(nblobs, 3)) - 0.5
+    s = rng.uniform(0.05, 0.15, size=nblobs)
+    a = rng.uniform(0.5, 1.5, size=nblobs)
+    gx, gy, gz = (torch.from_numpy(g).to(device) for g in grids_for(shape))
+    f = torch.zeros(shape, dtype=torch.float64, device=device)
+    for b in range(nblobs):
+        ex = torch.exp(-((gx - c[b, 0]) ** 2) / (2 * s[b] ** 2))
+        ey = torch.exp(-((gy - c[b, 1]) ** 2) / (2 * s[b] ** 2))
+        ez = torch.exp(-((gz - c[b, 2]) ** 2) / (2 * s[b] ** 2))
+        f += a[b] * ez[:, None, None] * ey[None, :, None] * ex[None, None, :]
+    return f
+
+
+def turbulence_fast_device(shape, kmax=64, seed=0, kmin=1, device="cuda"):
+    """Spectral turbulence with the spectrum of turbulence() built on the GPU
+    and phases from torch's seeded device generator (so the phases differ
+    from turbulence()'s PCG64 draws); for generating many large benchmark
+    fields quickly."""
+    import torch
+
+    nz, ny, nx = shape
+    kz = torch.fft.fftfreq(nz, device=device, dtype=torch.float64) * nz
+    ky = torch.fft.fftfreq(ny, device=device, dtype=torch.float64) * ny
+    kx = torch.fft.rfftfreq(nx, device=device, dtype=torch.float64) * nx
+    k = torch.sqrt(kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2)
+    amp = torch.where((k >= kmin) & (k <= kmax), torch.clamp(k, min=1e-30) ** (-11.0 / 6.0),
+                      torch.zeros((), dtype=torch.float64, device=device))
+    del k
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    phase = torch.rand(amp.shape, generator=g, dtype=torch.float64, device=device) * (2 * math.pi)
+    spec = torch.polar(amp, phase)
+    del amp, phase
+    f = torch.fft.irfftn(spec, s=shape, dim=(0, 1, 2))
+    del spec
+    f -= f.mean()
+    f /= f.std(unbiased=False)
+    return f.contiguous()
+
+
 def example_field(n=128):
     """The reference example's field (src/main.c:23-111) on one rank.
 
