@@ -358,6 +358,72 @@ def served_stream(api, fields, nreq, rate, nfield_pool, seed=0):
     return served
 
 
+def precision_study(api, fields, n, side):
+    """The FP32 form against the FP64 form (bit-identical to the reference)
+    on an n^3 spectral turbulence field (|k| 1..128), thresholds
+    {-1.5, 0, 1.5}, one side x side image: per threshold the topology
+    mismatches, max vertex error and max / mean vertex-normal angle (device
+    comparison of the two meshes), and over the image the pixels whose
+    colour differs and the coverage difference."""
+    import torch
+
+    api.finalize()
+    torch.cuda.empty_cache()
+    f = fields.turbulence_fast_device((n, n, n), kmax=min(128, n // 2 - 1), seed=7)
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()  # the transform's temporaries go back to the device
+    g = fields.uniform_grid(n)
+    thr = (-1.5, 0.0, 1.5)
+    cont = [dict(field=f, grids=(g, g, g), threshold=t, color=c) for t, c in zip(thr, COLORS)]
+    v = fields.example_view(side, side)
+    v["bg"] = (0, 0, 0)
+    res = {"lattice": [n, n, n], "thresholds": list(thr), "image": [side, side]}
+    imgs = {}
+    for name, prec in (("fp64", api.FP64), ("fp32", api.FP32)):
+        rgb = torch.zeros((side, side, 3), dtype=torch.uint8, device="cuda")
+        dep = torch.zeros((side, side), dtype=torch.float64, device="cuda")
+        api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], cont,
+                        precision=prec, rgb=rgb, depth=dep)
+        torch.cuda.synchronize()
+        imgs[name] = (rgb, dep)
+    api.finalize()
+    d64, d32 = imgs["fp64"][1], imgs["fp32"][1]
+    cov64 = d64 > -1e300
+    cov32 = d32 > -1e300
+    diff = (imgs["fp64"][0].int() - imgs["fp32"][0].int()).abs().amax(dim=2)
+    res["pixels_covered"] = int(cov64.sum())
+    res["pixel_mismatches"] = int((diff > 0).sum())
+    res["pixel_mismatches_over_1_level"] = int((diff > 1).sum())
+    res["coverage_differences"] = int((cov64 ^ cov32).sum())
+    both = cov64 & cov32 & torch.isfinite(d64) & torch.isfinite(d32)
+    res["max_depth_err"] = float((d64[both] - d32[both]).abs().max()) if both.any() else 0.0
+    res["nonfinite_fp32_depths"] = int((cov32 & ~torch.isfinite(d32)).sum())
+    res["nonfinite_fp64_depths"] = int((cov64 & ~torch.isfinite(d64)).sum())
+    del imgs
+    per = []
+    for t, c in zip(thr, COLORS):
+        one = [dict(field=f, grids=(g, g, g), threshold=t, color=c)]
+        m64 = api.Mesh(one, precision=api.FP64)
+        m32 = api.Mesh(one, precision=api.FP32)
+        try:
+            st = m64.compare(m32, 0)
+        finally:
+            m64.free()
+            m32.free()
+        per.append(dict(threshold=t, **{k: (round(x, 12) if isinstance(x, float) else x) for k, x in st.items()}))
+    api.finalize()
+    res["per_threshold"] = per
+    res["max_vertex_err"] = max(p["max_vertex_err"] for p in per)
+    res["max_normal_angle_deg"] = max(p["max_normal_angle_deg"] for p in per)
+    res["topology_mismatches"] = sum(p["topology_mismatches"] for p in per)
+    nv = sum(p["normals"] for p in per)
+    res["normals_over_1deg_frac"] = sum(p["normals_over_1deg"] for p in per) / max(1, nv)
+    res["normals_over_10deg_frac"] = sum(p["normals_over_10deg"] for p in per) / max(1, nv)
+    del f
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -367,10 +433,11 @@ def main():
     ap.add_argument("--n", type=int, default=N, help="lattice points per axis (default 512)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device three times (reference kernel for HBM counters)")
-    ap.add_argument("--forms", default="batched,served",
-                    help="comma list of extra forms to run beside the headline frame: batched, served (or none)")
+    ap.add_argument("--forms", default="batched,served,precision",
+                    help="comma list of extra forms beside the headline frame: batched, served, precision (or none)")
     ap.add_argument("--batch-fields", type=int, default=64)
     ap.add_argument("--served-requests", type=int, default=150)
+    ap.add_argument("--precision-n", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -453,14 +520,27 @@ def main():
                  "stages": results["fp32"]["stages"]},
     }
     forms = [f for f in args.forms.split(",") if f and f != "none"]
+
+    def form(name, fn):
+        # a form that fails records its error; the headline line still prints
+        try:
+            return fn()
+        except Exception as e:  # noqa: BLE001
+            log(f"{name} failed: {e!r}")
+            return {"error": repr(e)}
+
     if "batched" in forms:
         log("batched: four cameras on the shared 512^3 scene")
-        out["batched"] = {"shared_scene": shared_scene(api, fields, contours, max(2, args.steps // 2))}
+        out["batched"] = {"shared_scene": form("shared_scene", lambda: shared_scene(
+            api, fields, contours, max(2, args.steps // 2)))}
         log(f"batched: {args.batch_fields} uneven fields, 1024^2 images")
-        out["batched"]["uneven"] = uneven_batch_frame(api, fields, args.batch_fields, 1024, 3)
+        out["batched"]["uneven"] = form("uneven", lambda: uneven_batch_frame(api, fields, args.batch_fields, 1024, 3))
     if "served" in forms:
         log(f"served: {args.served_requests} Poisson arrivals at 50/s")
-        out["served"] = served_stream(api, fields, args.served_requests, 50.0, 8)
+        out["served"] = form("served", lambda: served_stream(api, fields, args.served_requests, 50.0, 8))
+    if "precision" in forms:
+        log(f"precision study: {args.precision_n}^3, 2048^2")
+        out["precision_study"] = form("precision", lambda: precision_study(api, fields, args.precision_n, 2048))
     if not args.no_cpu:
         log("cpu oracle baseline")
         out["cpu_baseline"] = cpu_baseline(field, view)

@@ -39,6 +39,7 @@ EXPORTS = (
     "contour3d_amd_mesh_count",
     "contour3d_amd_mesh_copy",
     "contour3d_amd_mesh_render",
+    "contour3d_amd_mesh_compare",
     "contour3d_amd_mesh_free",
     "contour3d_amd_tune",
     "contour3d_amd_profile_enable",
@@ -89,6 +90,7 @@ def lib():
                                                 C.POINTER(_abi.Color), C.c_size_t, C.POINTER(_abi.LineObj),
                                                 C.c_void_p, C.c_void_p]
         L.contour3d_amd_mesh_free.argtypes = [C.c_void_p]
+        L.contour3d_amd_mesh_compare.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
         L.contour3d_amd_profile_enable.argtypes = [C.c_int]
         L.contour3d_amd_profile_read.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
@@ -358,6 +360,15 @@ class Mesh:
         if rc != 0:
             raise RuntimeError("mesh_render failed")
         return rgb, depth
+
+    def compare(self, other, obj):
+        """contour3d_amd_mesh_compare of this FP64 mesh with `other`."""
+        st = np.zeros(8, np.float64)
+        if lib().contour3d_amd_mesh_compare(self.handle, other.handle, obj, st.ctypes.data) != 0:
+            raise RuntimeError("mesh_compare failed")
+        return dict(triangles=int(st[0]), topology_mismatches=int(st[1]), max_vertex_err=st[2],
+                    max_normal_angle_deg=st[3], mean_normal_angle_deg=st[4], normals=int(st[5]),
+                    normals_over_1deg=int(st[6]), normals_over_10deg=int(st[7]))
 
     def free(self):
         if self.handle:

@@ -51,7 +51,10 @@ struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
   ~DevBuf() = default;
-  hipError_t ensure(size_t want) {
+  // grows to at least `want` bytes, with a quarter of headroom unless
+  // `exact` (a growing workload reallocates rarely; each reallocation frees
+  // first, which waits for the device)
+  hipError_t ensure(size_t want, bool exact = false) {
     if (want <= bytes) return hipSuccess;
     if (p) {
       hipError_t e = hipFree(p);
@@ -59,9 +62,7 @@ struct DevBuf {
       bytes = 0;
       if (e != hipSuccess) return e;
     }
-    // a quarter of headroom: a growing workload reallocates rarely (each
-    // reallocation frees first, which waits for the device)
-    size_t n = want + want / 4 + 256;
+    size_t n = exact ? want : want + want / 4 + 256;
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) {
       p = nullptr;
@@ -665,10 +666,11 @@ int ensure_capacity(contour3d_amd_mesh& M) {
   M.cell.release();
   M.edges.release();
   M.cap = 0;
-  C3D_HIP(M.pos.ensure(9 * cap * sizeof(R)));
-  C3D_HIP(M.wfn.ensure(3 * cap * sizeof(R)));
-  C3D_HIP(M.cell.ensure(cap * sizeof(uint32_t)));
-  C3D_HIP(M.edges.ensure(cap * sizeof(uint32_t)));
+  // the capacity already holds a quarter of headroom
+  C3D_HIP(M.pos.ensure(9 * cap * sizeof(R), true));
+  C3D_HIP(M.wfn.ensure(3 * cap * sizeof(R), true));
+  C3D_HIP(M.cell.ensure(cap * sizeof(uint32_t), true));
+  C3D_HIP(M.edges.ensure(cap * sizeof(uint32_t), true));
   M.cap = cap;
   return 0;
 }
@@ -1792,6 +1794,58 @@ int contour3d_amd_mesh_copy(const contour3d_amd_mesh_t* mesh, size_t obj, double
     }
   }
   if (cell && hipMemcpy(cell, mesh->cell.as<uint32_t>() + b, n * 4, hipMemcpyDeviceToHost) != hipSuccess) return 1;
+  return 0;
+}
+
+int contour3d_amd_mesh_compare(const contour3d_amd_mesh_t* a, const contour3d_amd_mesh_t* b, size_t obj,
+                               double stats[8]) {
+  if (!a || !b || !stats || obj >= a->count.size() || obj >= b->count.size() ||
+      a->precision != CONTOUR3D_AMD_FP64) {
+    c3d::log_error("mesh_compare: bad arguments (a must be an FP64 mesh holding object %zu)", obj);
+    return 1;
+  }
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  if (c3d::ensure_init(L, 0)) return 1;
+  const uint64_t n = a->count[obj];
+  stats[0] = double(n);
+  for (int k = 1; k < 8; k++) stats[k] = 0.0;
+  if (a->count[obj] != b->count[obj]) {
+    stats[1] = double(n > b->count[obj] ? n - b->count[obj] : b->count[obj] - n);
+    return 0;
+  }
+  c3d::Frame& F = L.main;
+  if (F.vntmp.ensure(8 * sizeof(unsigned long long)) != hipSuccess) {
+    c3d::log_error("mesh_compare: allocation failed");
+    return 1;
+  }
+  unsigned long long* out = F.vntmp.as<unsigned long long>();
+  hipError_t e = hipMemsetAsync(out, 0, 8 * sizeof(unsigned long long), F.st);
+  const c3d::MeshView<double> va = c3d::view_of<double>(*a);
+  const c3d::MeshView<double> vb64 = c3d::view_of<double>(*b);
+  const c3d::MeshView<float> vb32 = c3d::view_of<float>(*b);
+  const bool b32 = b->precision == CONTOUR3D_AMD_FP32;
+  if (e == hipSuccess)
+    e = c3d::launch_mesh_compare(va, a->meta.as<c3d::ObjMeta>() + obj, b32 ? static_cast<const void*>(&vb32)
+                                                                            : static_cast<const void*>(&vb64),
+                                 b32 ? 1 : 0, b->meta.as<c3d::ObjMeta>() + obj, n, out, F.st);
+  unsigned long long h[8] = {};
+  if (e == hipSuccess) e = hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, F.st);
+  if (e == hipSuccess) e = hipStreamSynchronize(F.st);
+  if (e != hipSuccess) {
+    c3d::log_error("mesh_compare failed: %s", hipGetErrorString(e));
+    return 1;
+  }
+  double verr, amax;
+  std::memcpy(&verr, &h[1], sizeof(double));
+  std::memcpy(&amax, &h[2], sizeof(double));
+  stats[1] = double(h[0]);
+  stats[2] = verr;
+  stats[3] = amax;
+  stats[4] = h[4] ? double(h[3]) * 1e-9 / double(h[4]) : 0.0;
+  stats[5] = double(h[4]);
+  stats[6] = double(h[5]);
+  stats[7] = double(h[6]);
   return 0;
 }
 

@@ -146,6 +146,10 @@ hipError_t launch_emit(const TessJob* jobs, int njobs, uint32_t ntiles, const ui
 template <typename R>
 hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, uint64_t count,
                                  double* out, hipStream_t st);
+// FP64 mesh a against mesh b (FP64, or FP32 when b_fp32; b points to its
+// MeshView) for one object: out[0..4] as k_mesh_compare documents
+hipError_t launch_mesh_compare(const MeshView<double>& a, const ObjMeta* ma, const void* b, int b_fp32,
+                               const ObjMeta* mb, uint64_t count, unsigned long long* out, hipStream_t st);
 hipError_t launch_zinit(uint64_t* key, uint32_t* id, uint64_t npx, int precision, FrameHdr* hdr,
                         hipStream_t st);
 // raster modes: FP32 single pass over packed (depth << 32 | id) keys; FP64

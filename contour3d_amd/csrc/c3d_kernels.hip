@@ -440,11 +440,19 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TessJob* __restrict__ j
           P[vv] = interp<R>(thr, pt, s_val[tail][l], ph, s_val[head][l]);
         }
         // face normal and area (triangulate.c:75-93); the smoothing weight
-        // area * unit normal is formed as normal.c:39-40 forms it
+        // area * unit normal is formed as normal.c:39-40 forms it.  The FP32
+        // form takes the cross product itself (the same vector): a triangle
+        // whose vertices rounded onto one point then weighs 0 instead of
+        // NaN (0 * (n / 0)), which would poison its neighbours' normals
         const V3<R> n = vcross(vsub(P[1], P[0]), vsub(P[2], P[0]));
-        const R area = sqrt(vdot(n, n));
-        const V3<R> fn = vscale(R(1) / area, n);
-        const V3<R> w = vscale(area, fn);
+        V3<R> w;
+        if constexpr (sizeof(R) == 8) {
+          const R area = sqrt(vdot(n, n));
+          const V3<R> fn = vscale(R(1) / area, n);
+          w = vscale(area, fn);
+        } else {
+          w = n;
+        }
         const bool in = L.i >= 1 && L.i + 2 <= f.ncx && L.j >= 1 && L.j + 2 <= f.ncy && L.k >= 1 &&
                         L.k + 2 <= f.ncz;
         uint64_t bq = base[0];
@@ -540,6 +548,54 @@ __global__ __launch_bounds__(kThreads) void k_vertex_normals(MeshView<R> m, cons
     out[9 * t + 3 * vv + 1] = n.y;
     out[9 * t + 3 * vv + 2] = n.z;
   }
+}
+
+// Compares object meshes built in two arithmetic forms, triangle by
+// triangle (the topology is the same by construction, which this checks):
+// out[0] topology mismatches (cell or edge ids), out[1] max |vertex
+// difference| (as the bits of a non-negative double), out[2] max vertex
+// normal angle in degrees (bits), out[3] sum of angles in 1e-9 degree
+// units, out[4] interior vertices compared, out[5] / out[6] vertices whose
+// angle exceeds 1 / 10 degrees.  Non-finite normals (zero-area
+// neighbourhoods) are skipped.
+template <typename RA, typename RB>
+__global__ __launch_bounds__(kThreads) void k_mesh_compare(MeshView<RA> a, const ObjMeta* __restrict__ ma,
+                                                           MeshView<RB> b, const ObjMeta* __restrict__ mb,
+                                                           uint64_t count, unsigned long long* __restrict__ out) {
+  const ObjMeta oa = *ma, ob = *mb;
+  double verr = 0.0, amax = 0.0;
+  unsigned long long asum = 0, nv = 0, bad = 0, over1 = 0, over10 = 0;
+  for (uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x; t < count; t += uint64_t(gridDim.x) * kThreads) {
+    const uint64_t ga = oa.base + t, gb = ob.base + t;
+    const uint32_t ea = a.edges[ga], eb = b.edges[gb];
+    if (ea != eb || a.cell[ga] != b.cell[gb]) {
+      bad++;
+      continue;
+    }
+#pragma unroll
+    for (int c = 0; c < 9; c++) verr = fmax(verr, fabs(double(a.pos[c * a.cap + ga]) - double(b.pos[c * b.cap + gb])));
+    if (!((ea >> 15) & 1u)) continue;
+    for (int v = 0; v < 3; v++) {
+      const uint32_t e = (ea >> (5 * v)) & 31u;
+      const V3<RA> na = vertex_normal<RA>(a, oa, a.cell[ga], e);
+      const V3<RB> nb = vertex_normal<RB>(b, ob, b.cell[gb], e);
+      const double d = double(na.x) * double(nb.x) + double(na.y) * double(nb.y) + double(na.z) * double(nb.z);
+      if (!(d == d) || !isfinite(d)) continue;
+      const double ang = acos(fmin(1.0, fmax(-1.0, d))) * (180.0 / 3.14159265358979323846);
+      amax = fmax(amax, ang);
+      asum += (unsigned long long)(ang * 1e9 + 0.5);
+      nv++;
+      over1 += ang > 1.0 ? 1u : 0u;
+      over10 += ang > 10.0 ? 1u : 0u;
+    }
+  }
+  if (bad) atomicAdd(out + 0, bad);
+  atomicMax(out + 1, (unsigned long long)__double_as_longlong(verr));
+  atomicMax(out + 2, (unsigned long long)__double_as_longlong(amax));
+  if (asum) atomicAdd(out + 3, asum);
+  if (nv) atomicAdd(out + 4, nv);
+  if (over1) atomicAdd(out + 5, over1);
+  if (over10) atomicAdd(out + 6, over10);
 }
 
 // ----------------------------------------------------------------------
@@ -673,12 +729,17 @@ __global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, const Raster
       bool vis = true;
 #pragma unroll
       for (int n = 0; n < 3; n++) vis = vis && project(v, V3<R>{cur.p[3 * n], cur.p[3 * n + 1], cur.p[3 * n + 2]}, P.q[n]);
+      R area = R(0);
       if (vis) {
-        const R area = raster_box(v, P);
+        area = raster_box(v, P);
+        // the FP32 form skips triangles whose projection has no area (their
+        // weights are all +-0 and their depth 1 / +-0)
+        if (sizeof(R) == 4 && area == R(0)) vis = false;
+      }
+      if (vis) {
         // FP64 pays for narrowing the box; in FP32 the box arithmetic
         // costs more than the candidates it removes
         if constexpr (sizeof(R) == 8) tight_box(v, area, P);
-        (void)area;
         const uint64_t row0 = rjobs[cur.jb].px0;
         const uint32_t tid1 = uint32_t(cur.t + 1);
         for (int32_t j = P.j0; j <= P.j1; j++) {
@@ -965,6 +1026,19 @@ hipError_t launch_resolve(const MeshView<R>& m, const RasterJob<R>* rjobs, int n
                           const uint64_t* key, const uint32_t* id, uint32_t grid, hipStream_t st) {
   hipLaunchKernelGGL(k_resolve<R>, dim3(grid8((npx_total + kThreads - 1) / kThreads, grid)), dim3(kThreads), 0, st,
                      m, rjobs, njobs, npx_total, meta, nobj, frag_rgb, hdr, key, id);
+  return hipGetLastError();
+}
+
+hipError_t launch_mesh_compare(const MeshView<double>& a, const ObjMeta* ma, const void* b, int b_fp32,
+                               const ObjMeta* mb, uint64_t count, unsigned long long* out, hipStream_t st) {
+  if (count == 0) return hipSuccess;
+  const uint32_t grid = grid8((count + kThreads - 1) / kThreads, uint32_t(persistent_blocks()) * 4);
+  if (b_fp32)
+    hipLaunchKernelGGL((k_mesh_compare<double, float>), dim3(grid), dim3(kThreads), 0, st, a, ma,
+                       *static_cast<const MeshView<float>*>(b), mb, count, out);
+  else
+    hipLaunchKernelGGL((k_mesh_compare<double, double>), dim3(grid), dim3(kThreads), 0, st, a, ma,
+                       *static_cast<const MeshView<double>*>(b), mb, count, out);
   return hipGetLastError();
 }
 

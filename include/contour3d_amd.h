@@ -189,6 +189,21 @@ extern int contour3d_amd_mesh_render(
     double * depth
 );
 
+/* Compare object `obj` of an FP64 mesh `a` with the same object of mesh
+ * `b` (built from the same contour objects, FP64 or FP32), on the device:
+ *   stats[0] triangles of the object in a
+ *   stats[1] topology mismatches (triangles whose cell or edge ids differ,
+ *            or the difference of the triangle counts)
+ *   stats[2] max |vertex coordinate difference|
+ *   stats[3] max vertex-normal angle, degrees (interior triangles)
+ *   stats[4] mean vertex-normal angle, degrees
+ *   stats[5] vertex normals compared (non-finite ones skipped)
+ *   stats[6] vertex normals off by more than 1 degree
+ *   stats[7] vertex normals off by more than 10 degrees
+ * The tool behind the FP32 form's stated errors. */
+extern int contour3d_amd_mesh_compare(const contour3d_amd_mesh_t * a, const contour3d_amd_mesh_t * b, size_t obj,
+                                      double stats[8]);
+
 extern int contour3d_amd_mesh_free(contour3d_amd_mesh_t * mesh);
 
 /* ---- tuning ------------------------------------------------------------- */

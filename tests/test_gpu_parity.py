@@ -242,6 +242,21 @@ def test_fp32_form_topology_exact_and_errors_bounded(orc):
                   f"normal angle mean {ang.mean():.3g} max {ang.max():.3g} deg")
             assert verr < 1e-6
             assert ang.mean() < 0.05 and ang.max() < 3.0
+            # the device comparison tool agrees with the same statistics
+            # taken per vertex on the host (vertices with a finite dot product)
+            d = np.sum(a["vn"][a["interior"]] * b["vn"][a["interior"]], axis=2)
+            angv = np.degrees(np.arccos(np.clip(d[np.isfinite(d)], -1, 1)))
+            dev = m64.compare(m32, q)
+            assert dev["triangles"] == len(a["cell"]) and dev["topology_mismatches"] == 0
+            assert dev["max_vertex_err"] == verr
+            assert dev["normals"] == angv.size
+            assert abs(dev["max_normal_angle_deg"] - angv.max()) < 1e-6
+            assert abs(dev["mean_normal_angle_deg"] - angv.mean()) < 1e-6
+            # against itself: no difference beyond acos near 1 (unit vectors
+            # are unit to rounding, so the angle floor is ~2e-6 degrees)
+            same = m64.compare(m64, q)
+            assert same["max_vertex_err"] == 0.0 and same["max_normal_angle_deg"] < 1e-5
+            assert same["topology_mismatches"] == 0
     finally:
         m64.free()
         m32.free()
