@@ -35,6 +35,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch past L2 of the FP64 frame's kernels at 512^3
+# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a plain 1 GiB copy
+# reads), measured with rocprofv3 --pmc on this same frame; the commands and
+# raw counts are in profiles/turbulence_512cube_3thr_1920x1080_kernels.md
+MEASURED_TRAFFIC_512 = {"emit": 19.19e9, "raster": 11.61e9, "count": 1.37e9, "resolve": 6.51e9}
 N = 512
 THRESHOLDS = (-1.0, 0.0, 1.0)
 COLORS = ((255, 80, 40), (40, 220, 90), (60, 120, 255))
@@ -509,7 +514,9 @@ def main():
                    "lattice": [n, n, n], "thresholds": list(THRESHOLDS), "image": [W, H],
                    "triangles_per_frame": int(r["T"])},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC_512.get(dom) if n == 512 else None,
+                     "algorithmic_bytes": round(dom_b)},
         "tessellation": {"ms": round(tess_ms, 4),
                          "gcells_per_s": round(cells / (tess_ms * 1e-3) / 1e9, 2),
                          "hbm_frac": round(tess_b / (tess_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
