@@ -186,13 +186,15 @@ struct Frame {
   hipStream_t st = nullptr;
   hipEvent_t done = nullptr;
   std::vector<std::unique_ptr<DevBuf>> uploads;
-  DevBuf cnt, tiles, tjobs, rjobs, hdr;
+  DevBuf cnt, tiles, tjobs, rjobs, hdr, jimg;
   DevBuf key, id, rgb, depth, fpix, fdep, frgb, frags, vntmp;
   uint32_t frag_cap = 0;
   HostBuf hrgb, hstage;
   // host sources of asynchronous uploads stay alive until the frame ends
   std::vector<TessJob> h_tjobs;
   std::vector<ObjMeta> h_meta;
+  std::vector<uint32_t> obj_job, obj_q;  // field pass and threshold slot of each object
+  std::vector<uint32_t> h_jimg;          // images of each field pass (offsets, then indices)
   std::vector<RasterJob<double>> h_rjobs64;
   std::vector<RasterJob<float>> h_rjobs32;
   uint32_t h_nline = 0;
@@ -223,7 +225,8 @@ struct Frame {
     if (st) (void)hipStreamSynchronize(st);
     for (auto& u : uploads) u->release();
     uploads.clear();
-    DevBuf* bufs[] = {&cnt, &tiles, &tjobs, &rjobs, &hdr, &key, &id, &rgb, &depth, &fpix, &fdep, &frgb, &frags, &vntmp};
+    DevBuf* bufs[] = {&cnt,  &tiles, &tjobs, &rjobs, &hdr,  &jimg, &key,  &id,
+                      &rgb,  &depth, &fpix,  &fdep,  &frgb, &frags, &vntmp};
     for (auto* b : bufs) b->release();
     mesh.release();
     frag_cap = 0;
@@ -656,6 +659,17 @@ bool same_obj(const contour3d_contour_obj_t& a, const contour3d_contour_obj_t& b
          a.color.g == b.color.g && a.color.b == b.color.b;
 }
 
+// A frame keeps only the cell index of each triangle (the fused raster
+// draws triangles from registers): 1.25x the triangles seen.
+int ensure_cells(contour3d_amd_mesh& M) {
+  if (M.cap >= M.total && M.cap != 0 && M.cell.p) return 0;
+  const uint64_t cap = M.total + M.total / 4 + 1024;
+  M.cap = 0;
+  C3D_HIP(M.cell.ensure(cap * sizeof(uint32_t), true));
+  M.cap = cap;
+  return 0;
+}
+
 template <typename R>
 int ensure_capacity(contour3d_amd_mesh& M) {
   if (M.cap >= M.total && M.cap != 0 && M.precision_built == M.precision) return 0;
@@ -758,6 +772,8 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
   // one read of it, up to kMaxThr at a time
   std::vector<TessJob>& tj = F.h_tjobs;
   tj.clear();
+  F.obj_job.assign(n, 0);
+  F.obj_q.assign(n, 0);
   uint64_t jt = 0;
   for (size_t o = 0; o < n; o++) {
     if (group[o] != int(o)) continue;
@@ -767,6 +783,9 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
       if (group[q] != int(o)) continue;
       job.ts.thr[job.ts.n] = objs[q]->threshold;
       job.ts.boff[job.ts.n] = boff[q];
+      job.ts.obj[job.ts.n] = uint32_t(q);
+      F.obj_job[q] = uint32_t(tj.size());
+      F.obj_q[q] = uint32_t(job.ts.n);
       job.ts.n++;
       if (job.ts.n == kMaxThr) {
         job.tile0 = uint32_t(jt);
@@ -802,6 +821,8 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
     hm[o].ncy = fd[o].ncy;
     hm[o].ncz = fd[o].ncz;
     hm[o].tile_off = boff[o];
+    hm[o].job = F.obj_job[o];
+    hm[o].q = F.obj_q[o];
     hm[o].rgb[0] = objs[o]->color.r;
     hm[o].rgb[1] = objs[o]->color.g;
     hm[o].rgb[2] = objs[o]->color.b;
@@ -843,9 +864,10 @@ int tess_emit(Lib& L, Frame& F, contour3d_amd_mesh& M) {
   return 0;
 }
 
-// Reads the plan's totals back (one synchronisation) and sizes the mesh.
+// Reads the plan's totals back (one synchronisation) and sizes the mesh
+// (only its cell indices for a frame).
 template <typename R>
-int tess_readback(Frame& F, contour3d_amd_mesh& M, FrameHdr& hh) {
+int tess_readback(Frame& F, contour3d_amd_mesh& M, FrameHdr& hh, bool cells_only = false) {
   const size_t n = size_t(F.tl.nobj);
   unsigned char* hs = static_cast<unsigned char*>(F.hstage.p);
   C3D_HIP(hipMemcpyAsync(hs, F.hdr.p, sizeof(FrameHdr), hipMemcpyDeviceToHost, F.st));
@@ -865,7 +887,7 @@ int tess_readback(Frame& F, contour3d_amd_mesh& M, FrameHdr& hh) {
     log_error("%llu triangles exceed the 2^32 triangle ids of one frame", (unsigned long long)M.total);
     return 1;
   }
-  return ensure_capacity<R>(M);
+  return cells_only ? ensure_cells(M) : ensure_capacity<R>(M);
 }
 
 // Count, scan, plan and emit every object of `objs` into M.  `rjobs`
@@ -1105,6 +1127,68 @@ void fill_rjobs(Frame& F, const std::vector<Image>& imgs) {
   }
 }
 
+// Uploads, for every field pass of the frame, the images that show any of
+// its objects: offsets [njobs + 1] into the same array, then image indices.
+int upload_pass_images(Frame& F, const std::vector<Image>& imgs) {
+  const size_t nj = F.h_tjobs.size();
+  std::vector<uint32_t>& J = F.h_jimg;
+  J.assign(nj + 1, 0);
+  for (size_t j = 0; j < nj; j++) {
+    J[j] = uint32_t(J.size());
+    const ThrSet& ts = F.h_tjobs[j].ts;
+    for (size_t k = 0; k < imgs.size(); k++) {
+      bool shows = false;
+      for (int q = 0; q < ts.n && !shows; q++) shows = ts.obj[q] >= imgs[k].obj0 && ts.obj[q] < imgs[k].obj1;
+      if (shows) J.push_back(uint32_t(k));
+    }
+  }
+  J[nj] = uint32_t(J.size());
+  C3D_HIP(F.jimg.ensure(J.size() * sizeof(uint32_t)));
+  C3D_HIP(hipMemcpyAsync(F.jimg.p, J.data(), J.size() * sizeof(uint32_t), hipMemcpyHostToDevice, F.st));
+  return 0;
+}
+
+// A frame without a stored mesh: z-buffer clear, the fused tessellation and
+// raster, lines and the FP64 id passes, and the resolve from the fields.
+// `lines` sizes the line kernels' grid (at least the frame's fragments).
+template <typename R>
+int fused_launch(Lib& L, Frame& F, const contour3d_amd_mesh& M, const RasterJob<R>* rjobs, int njobs, uint64_t npx,
+                 uint32_t lines) {
+  const bool fp64 = sizeof(R) == 8;
+  FrameHdr* hdr = F.hdr.as<FrameHdr>();
+  uint32_t* id = fp64 ? F.id.as<uint32_t>() : nullptr;
+  uint64_t* key = F.key.as<uint64_t>();
+  const TessLayout& tl = F.tl;
+  const TessJob* tj = F.tjobs.as<TessJob>();
+  const uint32_t* jimg = F.jimg.as<uint32_t>();
+  const uint64_t* start = M.bstart.as<uint64_t>();
+  uint32_t* cell = M.cell.as<uint32_t>();
+  const uint32_t grid = L.tess_grid(tl.ntj);
+  if (prof_begin(F, CONTOUR3D_AMD_STAGE_RASTER)) return 1;
+  C3D_HIP(launch_zinit(key, id, npx, fp64 ? 0 : 1, hdr, F.st));
+  if constexpr (sizeof(R) == 8) {
+    C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterDepth64, key, id,
+                                  F.frags.as<Frag>(), F.frag_cap, grid, F.st));
+    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 0, key, id, F.st));
+    C3D_HIP(launch_frag_ids(F.frags.as<Frag>(), F.frag_cap, hdr, key, id,
+                            std::min<uint32_t>((F.frag_cap + 255) / 256, 4096), F.st));
+    // the geometry id pass: returns at once unless the fragment list overflowed
+    C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterId64, key, id,
+                                  nullptr, 0, grid, F.st));
+    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 1, 0, key, id, F.st));
+  } else {
+    C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterFp32, key, id,
+                                  nullptr, 0, grid, F.st));
+    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 1, key, id, F.st));
+  }
+  if (prof_end(F)) return 1;
+  if (prof_begin(F, CONTOUR3D_AMD_STAGE_RESOLVE)) return 1;
+  C3D_HIP(launch_resolve_field<R>(tj, cell, rjobs, njobs, npx, M.meta.as<ObjMeta>(), int(M.base.size()),
+                                  F.frgb.as<uint8_t>(), hdr, key, id, F.st));
+  if (prof_end(F)) return 1;
+  return 0;
+}
+
 // The frame's launch sequence (count ... resolve) as a graph, replayed
 // while its layout stays the same: the kernels' arguments and grids depend
 // only on the layout, and everything else (fields, thresholds, colours,
@@ -1120,30 +1204,26 @@ int graph_frame(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& img
   const bool fp64 = sizeof(R) == 8;
   const int njobs = int(imgs.size());
   if (tess_prepare<R>(L, F, M, objs)) return 1;
+  if (upload_pass_images(F, imgs)) return 1;
   if (raster_prepare<R>(F, npx)) return 1;
-  const bool ready = M.cap > 0 && M.precision_built == M.precision && M.cap < kMaxFrameTriangles &&
-                     F.h_nline <= line_capacity(F);
+  const bool ready = M.cap > 0 && M.cell.p && M.cap < kMaxFrameTriangles && F.h_nline <= line_capacity(F);
   if (!ready) {
     F.drop_graph();
     FrameHdr hh{};
     if (tess_head<R>(L, F, M, rjd, njobs, ~uint64_t(0))) return 1;
-    if (tess_readback<R>(F, M, hh)) return 1;
-    if (tess_emit<R>(L, F, M)) return 1;
-    const uint32_t rgrid = raster_grid(L, fp64, hh.nwork, false);
-    if (raster_launch<R>(L, F, M, rjd, njobs, npx, rgrid, F.h_nline)) return 1;
+    if (tess_readback<R>(F, M, hh, true)) return 1;
+    if (fused_launch<R>(L, F, M, rjd, njobs, npx, F.h_nline)) return 1;
     return raster_copies(F, imgs, fp64);
   }
   const uint32_t lines = line_capacity(F);
-  const uint32_t rgrid = raster_grid(L, fp64, ~uint64_t(0) >> 20, true);
   const std::vector<uint64_t> sig = {
       sizeof(R), uint64_t(njobs), npx, F.tl.nbt, F.tl.ntj, uint64_t(F.tl.njobs), uint64_t(F.tl.nobj),
-      L.tess_cap, rgrid, lines, M.cap, F.frag_cap,
+      L.tess_cap, L.tess_grid(F.tl.ntj), lines, M.cap, F.frag_cap,
       uint64_t(uintptr_t(F.cnt.p)), uint64_t(uintptr_t(F.tiles.p)), uint64_t(uintptr_t(F.tjobs.p)),
-      uint64_t(uintptr_t(F.rjobs.p)), uint64_t(uintptr_t(F.hdr.p)), uint64_t(uintptr_t(F.key.p)),
-      uint64_t(uintptr_t(F.id.p)), uint64_t(uintptr_t(F.frags.p)), uint64_t(uintptr_t(F.fpix.p)),
-      uint64_t(uintptr_t(F.fdep.p)), uint64_t(uintptr_t(F.frgb.p)), uint64_t(uintptr_t(M.bstart.p)),
-      uint64_t(uintptr_t(M.meta.p)), uint64_t(uintptr_t(M.pos.p)), uint64_t(uintptr_t(M.wfn.p)),
-      uint64_t(uintptr_t(M.cell.p)), uint64_t(uintptr_t(M.edges.p))};
+      uint64_t(uintptr_t(F.rjobs.p)), uint64_t(uintptr_t(F.hdr.p)), uint64_t(uintptr_t(F.jimg.p)),
+      uint64_t(uintptr_t(F.key.p)), uint64_t(uintptr_t(F.id.p)), uint64_t(uintptr_t(F.frags.p)),
+      uint64_t(uintptr_t(F.fpix.p)), uint64_t(uintptr_t(F.fdep.p)), uint64_t(uintptr_t(F.frgb.p)),
+      uint64_t(uintptr_t(M.bstart.p)), uint64_t(uintptr_t(M.meta.p)), uint64_t(uintptr_t(M.cell.p))};
   if (!F.gexec || F.gsig != sig) {
     F.drop_graph();
     (void)persistent_blocks();  // device query outside the capture
@@ -1151,8 +1231,7 @@ int graph_frame(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& img
     F.prof.on = false;
     C3D_HIP(hipStreamBeginCapture(F.st, hipStreamCaptureModeThreadLocal));
     int rc = tess_head<R>(L, F, M, rjd, njobs, M.cap);
-    if (!rc) rc = tess_emit<R>(L, F, M);
-    if (!rc) rc = raster_launch<R>(L, F, M, rjd, njobs, npx, rgrid, lines);
+    if (!rc) rc = fused_launch<R>(L, F, M, rjd, njobs, npx, lines);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(F.st, &g);
     F.prof.on = prof_on;
@@ -1212,8 +1291,13 @@ int run_frame_t(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& img
   const int njobs = int(imgs.size());
   if (graph) return graph_frame<R>(L, F, M, imgs, objs, rjd, npx, replayed);
   FrameHdr hh{};
-  if (tessellate<R>(L, F, M, objs, rjd, njobs, hh)) return 1;
-  return raster_frame<R>(L, F, M, imgs, rjd, npx, hh);
+  if (tess_prepare<R>(L, F, M, objs)) return 1;
+  if (upload_pass_images(F, imgs)) return 1;
+  if (tess_head<R>(L, F, M, rjd, njobs, ~uint64_t(0))) return 1;
+  if (tess_readback<R>(F, M, hh, true)) return 1;
+  if (raster_prepare<R>(F, npx)) return 1;
+  if (fused_launch<R>(L, F, M, rjd, njobs, npx, F.h_nline)) return 1;
+  return raster_copies(F, imgs, sizeof(R) == 8);
 }
 
 // Renders the images as one frame into F; with wait, also waits for the

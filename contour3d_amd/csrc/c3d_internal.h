@@ -45,6 +45,7 @@ struct FieldDev {
 struct ThrSet {
   double thr[kMaxThr];
   uint32_t boff[kMaxThr];
+  uint32_t obj[kMaxThr];  // the frame's contour object of each threshold
   int n;
 };
 
@@ -76,6 +77,7 @@ struct ObjMeta {
   const uint64_t* bstart;  // triangle start of each tile of the object's field (nblocks + 1)
   uint32_t ncx, ncy, ncz;
   uint32_t tile_off;       // offset of the object's first tile in the frame's tile arrays
+  uint32_t job, q;         // its field pass and threshold slot there
   uint8_t rgb[4];
 };
 
@@ -150,6 +152,21 @@ hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, 
 // MeshView) for one object: out[0..4] as k_mesh_compare documents
 hipError_t launch_mesh_compare(const MeshView<double>& a, const ObjMeta* ma, const void* b, int b_fp32,
                                const ObjMeta* mb, uint64_t count, unsigned long long* out, hipStream_t st);
+// Frames without a stored mesh: tessellation fused with the raster.  Every
+// triangle is drawn from registers into each image that shows its object
+// (jimg: per-field-pass offsets [njobs + 1] into a list of the images that
+// show any of the pass's objects) and only its cell index is stored
+// (cell[cap]).  `mode` is a raster mode.
+template <typename R>
+hipError_t launch_tess_raster(const TessJob* jobs, int njobs, uint32_t ntiles, const uint64_t* start, uint32_t* cell,
+                              uint64_t cap, const RasterJob<R>* rjobs, const uint32_t* jimg, FrameHdr* hdr, int mode,
+                              uint64_t* key, uint32_t* id, Frag* frags, uint32_t fcap, uint32_t grid, hipStream_t st);
+// resolve of a fused frame: the nearest triangles and their neighbours'
+// smoothing weights are recomputed from the fields
+template <typename R>
+hipError_t launch_resolve_field(const TessJob* jobs, const uint32_t* cell, const RasterJob<R>* rjobs, int njobs,
+                                uint64_t npx_total, const ObjMeta* meta, int nobj, const uint8_t* frag_rgb,
+                                const FrameHdr* hdr, const uint64_t* key, const uint32_t* id, hipStream_t st);
 hipError_t launch_zinit(uint64_t* key, uint32_t* id, uint64_t npx, int precision, FrameHdr* hdr,
                         hipStream_t st);
 // raster modes: FP32 single pass over packed (depth << 32 | id) keys; FP64

@@ -332,6 +332,46 @@ __device__ __forceinline__ V3<R> interp(double thr, V3<R> p0, double v0, V3<R> p
   return vadd(p0, vscale(fct, vsub(p1, p0)));
 }
 
+// positions of the triangle in slot entry `ent` of cell L from its corner
+// values v (the arithmetic of the emit pass, from registers)
+template <typename R>
+__device__ __forceinline__ void slot_positions(const FieldDev& f, const CellLoc& L, double thr, uint64_t ent,
+                                               const double v[8], V3<R> P[3]) {
+#pragma unroll
+  for (int vv = 0; vv < 3; vv++) {
+    const uint32_t tail = uint32_t(ent >> (6 * vv)) & 7u;
+    const uint32_t head = uint32_t(ent >> (6 * vv + 3)) & 7u;
+    double vt = v[0], vh = v[0];
+#pragma unroll
+    for (int q = 1; q < 8; q++) {
+      vt = tail == uint32_t(q) ? v[q] : vt;
+      vh = head == uint32_t(q) ? v[q] : vh;
+    }
+    P[vv] = interp<R>(thr, corner_pos<R>(f, L, tail), vt, corner_pos<R>(f, L, head), vh);
+  }
+}
+
+// The smoothing weight of a triangle: area * unit face normal
+// (triangulate.c:75-93), formed as normal.c:39-40 forms it.  The FP32 form
+// takes the cross product itself (the same vector): a triangle whose
+// vertices rounded onto one point then weighs 0 instead of NaN
+// (0 * (n / 0)), which would poison its neighbours' normals.
+template <typename R>
+__device__ __forceinline__ V3<R> smoothing_weight(const V3<R> P[3]) {
+  const V3<R> n = vcross(vsub(P[1], P[0]), vsub(P[2], P[0]));
+  if constexpr (sizeof(R) == 8) {
+    const R area = sqrt(vdot(n, n));
+    const V3<R> fn = vscale(R(1) / area, n);
+    return vscale(area, fn);
+  } else {
+    return n;
+  }
+}
+
+__device__ __forceinline__ bool interior_cell(const FieldDev& f, const CellLoc& L) {
+  return L.i >= 1 && L.i + 2 <= f.ncx && L.j >= 1 && L.j + 2 <= f.ncy && L.k >= 1 && L.k + 2 <= f.ncz;
+}
+
 // ----------------------------------------------------------------------
 // tessellation: emit pass
 //
@@ -439,22 +479,8 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TessJob* __restrict__ j
           }
           P[vv] = interp<R>(thr, pt, s_val[tail][l], ph, s_val[head][l]);
         }
-        // face normal and area (triangulate.c:75-93); the smoothing weight
-        // area * unit normal is formed as normal.c:39-40 forms it.  The FP32
-        // form takes the cross product itself (the same vector): a triangle
-        // whose vertices rounded onto one point then weighs 0 instead of
-        // NaN (0 * (n / 0)), which would poison its neighbours' normals
-        const V3<R> n = vcross(vsub(P[1], P[0]), vsub(P[2], P[0]));
-        V3<R> w;
-        if constexpr (sizeof(R) == 8) {
-          const R area = sqrt(vdot(n, n));
-          const V3<R> fn = vscale(R(1) / area, n);
-          w = vscale(area, fn);
-        } else {
-          w = n;
-        }
-        const bool in = L.i >= 1 && L.i + 2 <= f.ncx && L.j >= 1 && L.j + 2 <= f.ncy && L.k >= 1 &&
-                        L.k + 2 <= f.ncz;
+        const V3<R> w = smoothing_weight<R>(P);
+        const bool in = interior_cell(f, L);
         uint64_t bq = base[0];
 #pragma unroll
         for (int z = 1; z < kMaxThr; z++) bq = q == z ? base[z] : bq;
@@ -463,19 +489,36 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TessJob* __restrict__ j
           hdr->overflow = 1u;  // capacity reserved ahead of a replayed frame was too small
           continue;
         }
+        // the FP64 mesh (104 B per triangle) streams past the caches:
+        // non-temporal stores measured 10-15% faster there, slower in FP32
         R* p = m.pos + g;
-#pragma unroll
-        for (int vv = 0; vv < 3; vv++) {
-          p[(3 * vv + 0) * cap] = P[vv].x;
-          p[(3 * vv + 1) * cap] = P[vv].y;
-          p[(3 * vv + 2) * cap] = P[vv].z;
-        }
         R* wp = m.wfn + g;
-        wp[0] = w.x;
-        wp[cap] = w.y;
-        wp[2 * cap] = w.z;
-        m.edges[g] = uint32_t(ent >> 18) | uint32_t(in) << 15;
-        m.cell[g] = cc;
+        const uint32_t ed = uint32_t(ent >> 18) | uint32_t(in) << 15;
+        if constexpr (sizeof(R) == 8) {
+#pragma unroll
+          for (int vv = 0; vv < 3; vv++) {
+            __builtin_nontemporal_store(P[vv].x, p + (3 * vv + 0) * cap);
+            __builtin_nontemporal_store(P[vv].y, p + (3 * vv + 1) * cap);
+            __builtin_nontemporal_store(P[vv].z, p + (3 * vv + 2) * cap);
+          }
+          __builtin_nontemporal_store(w.x, wp);
+          __builtin_nontemporal_store(w.y, wp + cap);
+          __builtin_nontemporal_store(w.z, wp + 2 * cap);
+          __builtin_nontemporal_store(ed, m.edges + g);
+          __builtin_nontemporal_store(cc, m.cell + g);
+        } else {
+#pragma unroll
+          for (int vv = 0; vv < 3; vv++) {
+            p[(3 * vv + 0) * cap] = P[vv].x;
+            p[(3 * vv + 1) * cap] = P[vv].y;
+            p[(3 * vv + 2) * cap] = P[vv].z;
+          }
+          wp[0] = w.x;
+          wp[cap] = w.y;
+          wp[2 * cap] = w.z;
+          m.edges[g] = ed;
+          m.cell[g] = cc;
+        }
       }
 #pragma unroll
       for (int q = 0; q < kMaxThr; q++) base[q] += qoff[q + 1] - qoff[q];
@@ -673,6 +716,61 @@ __device__ __forceinline__ int find_rjob(const RasterJob<R>* __restrict__ jobs, 
 //                   list overflowed
 constexpr int kFragLds = 1024;
 
+// Draws one triangle (world positions wpos, serial id tid1 - 1) into an
+// image: projection (project.c), the reference's box and barycentric test,
+// harmonic depth (render.c:60-124), and the z-buffer update of `MODE`.
+// Depth64 keeps fragments in the workgroup list s_frag (s_n entries,
+// `cap_lds` slots).
+template <typename R, int MODE>
+__device__ __forceinline__ void draw_triangle(const ViewDev<R>& v, uint64_t row0, const V3<R> wpos[3], uint32_t tid1,
+                                              uint64_t* __restrict__ key, uint32_t* __restrict__ id,
+                                              Frag* s_frag, uint32_t* s_n, uint32_t cap_lds) {
+  Projected<R> P;
+  bool vis = true;
+#pragma unroll
+  for (int n = 0; n < 3; n++) vis = vis && project(v, wpos[n], P.q[n]);
+  if (!vis) return;
+  const R area = raster_box(v, P);
+  // the FP32 form skips triangles whose projection has no area (their
+  // weights are all +-0 and their depth 1 / +-0)
+  if (sizeof(R) == 4 && area == R(0)) return;
+  // FP64 pays for narrowing the box; in FP32 the box arithmetic costs more
+  // than the candidates it removes
+  if constexpr (sizeof(R) == 8) tight_box(v, area, P);
+  for (int32_t j = P.j0; j <= P.j1; j++) {
+    const R cy = pixel_centre<R>(j, v.fh);
+    const uint64_t prow = row0 + uint64_t(j) * uint64_t(v.W);
+    for (int32_t i = P.i0; i <= P.i1; i++) {
+      R w0, w1, w2;
+      if (!pixel_weights(P, pixel_centre<R>(i, v.fw), cy, w0, w1, w2)) continue;
+      const R dep = pixel_depth(P, w0, w1, w2);
+      const uint64_t px = prow + uint64_t(i);
+      if constexpr (MODE == kRasterFp32) {
+        // keys only grow: a key at or below a value read from the buffer
+        // can never change it and skips the atomic
+        const uint64_t k = uint64_t(ord32(float(dep))) << 32 | uint64_t(tid1);
+        if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+      } else if constexpr (MODE == kRasterDepth64) {
+        const uint64_t k = ord64(double(dep));
+        if (k < key[px]) continue;
+        atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+        // every fragment whose key equals the final value is kept: its key
+        // was at least the value read before its own atomic
+        const uint32_t o = atomicAdd(s_n, 1u);
+        if (o < cap_lds) {
+          Frag fr;
+          fr.px = uint32_t(px);
+          fr.id = tid1;
+          fr.key = k;
+          s_frag[o] = fr;
+        }
+      } else {
+        if (ord64(double(dep)) == key[px]) atomicMax(id + px, tid1);
+      }
+    }
+  }
+}
+
 // registers of one raster work item, loaded a round ahead
 template <typename R>
 struct TriRegs {
@@ -724,53 +822,10 @@ __global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, const Raster
       __syncthreads();
     }
     if (cur.ok && ((cur.ed >> 15) & 1u)) {
+      const V3<R> wp[3] = {{cur.p[0], cur.p[1], cur.p[2]}, {cur.p[3], cur.p[4], cur.p[5]},
+                           {cur.p[6], cur.p[7], cur.p[8]}};
       const ViewDev<R> v = rjobs[cur.jb].v;
-      Projected<R> P;
-      bool vis = true;
-#pragma unroll
-      for (int n = 0; n < 3; n++) vis = vis && project(v, V3<R>{cur.p[3 * n], cur.p[3 * n + 1], cur.p[3 * n + 2]}, P.q[n]);
-      R area = R(0);
-      if (vis) {
-        area = raster_box(v, P);
-        // the FP32 form skips triangles whose projection has no area (their
-        // weights are all +-0 and their depth 1 / +-0)
-        if (sizeof(R) == 4 && area == R(0)) vis = false;
-      }
-      if (vis) {
-        // FP64 pays for narrowing the box; in FP32 the box arithmetic
-        // costs more than the candidates it removes
-        if constexpr (sizeof(R) == 8) tight_box(v, area, P);
-        const uint64_t row0 = rjobs[cur.jb].px0;
-        const uint32_t tid1 = uint32_t(cur.t + 1);
-        for (int32_t j = P.j0; j <= P.j1; j++) {
-          const R cy = pixel_centre<R>(j, v.fh);
-          const uint64_t prow = row0 + uint64_t(j) * uint64_t(v.W);
-          for (int32_t i = P.i0; i <= P.i1; i++) {
-            R w0, w1, w2;
-            if (!pixel_weights(P, pixel_centre<R>(i, v.fw), cy, w0, w1, w2)) continue;
-            const R dep = pixel_depth(P, w0, w1, w2);
-            const uint64_t px = prow + uint64_t(i);
-            if constexpr (MODE == kRasterFp32) {
-              const uint64_t k = uint64_t(ord32(float(dep))) << 32 | uint64_t(tid1);
-              if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
-            } else if constexpr (MODE == kRasterDepth64) {
-              const uint64_t k = ord64(double(dep));
-              if (k < key[px]) continue;
-              atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
-              const uint32_t o = atomicAdd(&s_n, 1u);
-              if (o < uint32_t(kFragLds)) {
-                Frag fr;
-                fr.px = uint32_t(px);
-                fr.id = tid1;
-                fr.key = k;
-                s_frag[o] = fr;
-              }
-            } else {
-              if (ord64(double(dep)) == key[px]) atomicMax(id + px, tid1);
-            }
-          }
-        }
-      }
+      draw_triangle<R, MODE>(v, rjobs[cur.jb].px0, wp, uint32_t(cur.t + 1), key, id, s_frag, &s_n, kFragLds);
     }
     if constexpr (MODE == kRasterDepth64) {
       __syncthreads();
@@ -787,6 +842,145 @@ __global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, const Raster
       __syncthreads();
     }
     cur = nxt;
+  }
+}
+
+// ----------------------------------------------------------------------
+// frames without a stored mesh: tessellation fused with the raster
+//
+// The emit pass's classification, scan and per-triangle arithmetic, after
+// which each triangle is drawn from registers into every image that shows
+// its object (jimg: offsets per field pass into a list of the images that
+// show any of its objects) and only its cell index is stored; the resolve recomputes what it needs from the
+// field.  In Depth64 mode the kept fragments collect in LDS and are flushed
+// once per chunk; Id64 (the geometry id pass) runs only after the list
+// overflowed.  A frame flagged by the plan as outgrowing `cap` draws
+// nothing (it is rendered again).
+constexpr uint32_t kFragLdsFused = 512;
+
+template <typename R, int MODE>
+__global__ __launch_bounds__(kThreads) void k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
+                                                          const uint64_t* __restrict__ start,
+                                                          uint32_t* __restrict__ cellout, uint64_t cap,
+                                                          const RasterJob<R>* __restrict__ rjobs,
+                                                          const uint32_t* __restrict__ jimg,
+                                                          FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
+                                                          uint32_t* __restrict__ id, Frag* __restrict__ frags,
+                                                          uint32_t fcap) {
+  __shared__ double s_val[8][kThreads];
+  __shared__ uint8_t s_code[kMaxThr][kThreads];
+  __shared__ uint16_t s_pref[kMaxThr][kThreads];
+  __shared__ uint64_t s_wave[kThreads / 64];
+  __shared__ uint8_t s_ntri[256];
+  __shared__ Frag s_frag[MODE == kRasterDepth64 ? kFragLdsFused : 1];
+  __shared__ uint32_t s_n, s_base;
+  if (hdr->overflow & 1u) return;
+  if (MODE == kRasterId64 && !(hdr->overflow & 2u)) return;
+  s_ntri[threadIdx.x] = c_ntri[threadIdx.x];
+  __syncthreads();
+  const XcdRange XR = xcd_range(ntiles);
+  for (uint32_t gt = XR.begin; gt < XR.end; gt += XR.step) {
+    const uint32_t jb = find_tjob(jobs, njobs, gt);
+    const FieldDev f = jobs[jb].f;
+    const ThrSet ts = jobs[jb].ts;
+    const uint32_t tile = gt - jobs[jb].tile0;
+    uint64_t base[kMaxThr];
+    double thrs[kMaxThr];
+    uint32_t objs[kMaxThr];
+#pragma unroll
+    for (int q = 0; q < kMaxThr; q++) {
+      base[q] = q < ts.n ? start[ts.boff[q] + tile] : 0;
+      thrs[q] = ts.thr[q];
+      objs[q] = ts.obj[q];
+    }
+    for (int chunk = 0; chunk < kCellsPerBlock / kThreads; chunk++) {
+      const uint32_t c0 = tile * kCellsPerBlock + chunk * kThreads;
+      if (c0 >= f.ncells) break;
+      const uint32_t c = c0 + threadIdx.x;
+      const bool valid = c < f.ncells;
+      double v[8];
+      if (valid) {
+        load_corners(f, locate(f, c), v);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++) s_val[q][threadIdx.x] = v[q];
+      if (MODE == kRasterDepth64 && threadIdx.x == 0) s_n = 0;
+      uint32_t code[kMaxThr];
+      const uint64_t packed = valid ? packed_counts(ts, v, code, s_ntri) : 0ull;
+      uint64_t tot;
+      const uint64_t ex = block_excl_scan64(packed, s_wave, tot);
+      uint32_t qoff[kMaxThr + 1];
+      qoff[0] = 0;
+#pragma unroll
+      for (int q = 0; q < kMaxThr; q++) {
+        qoff[q + 1] = qoff[q] + (uint32_t(tot >> (16 * q)) & 0xffffu);
+        s_pref[q][threadIdx.x] = uint16_t(ex >> (16 * q));
+        if (valid) s_code[q][threadIdx.x] = uint8_t(code[q]);
+      }
+      __syncthreads();
+      const uint32_t total = qoff[kMaxThr];
+      for (uint32_t it = threadIdx.x; it < total; it += kThreads) {
+        int q = 0;
+#pragma unroll
+        for (int z = 1; z < kMaxThr; z++) q += it >= qoff[z] ? 1 : 0;
+        uint32_t qsel = 0;
+#pragma unroll
+        for (int z = 1; z < kMaxThr; z++) qsel = q == z ? qoff[z] : qsel;
+        const uint32_t r = it - qsel;
+        uint32_t l = 0;
+#pragma unroll
+        for (uint32_t step = kThreads / 2; step; step >>= 1) l += s_pref[q][l + step] <= r ? step : 0u;
+        const uint32_t sl = r - s_pref[q][l];
+        double thr = thrs[0];
+        uint64_t bq = base[0];
+        uint32_t obj = objs[0];
+#pragma unroll
+        for (int z = 1; z < kMaxThr; z++) {
+          thr = q == z ? thrs[z] : thr;
+          bq = q == z ? base[z] : bq;
+          obj = q == z ? objs[z] : obj;
+        }
+        const uint32_t cc = c0 + l;
+        const CellLoc L = locate(f, cc);
+        const uint64_t ent = c_cslot[uint32_t(s_code[q][l]) * 12u + sl];
+        double cv[8];
+#pragma unroll
+        for (int z = 0; z < 8; z++) cv[z] = s_val[z][l];
+        V3<R> P[3];
+        slot_positions<R>(f, L, thr, ent, cv, P);
+        const uint64_t g = bq + r;
+        if (MODE != kRasterId64 && g < cap) cellout[g] = cc;
+        if (!interior_cell(f, L)) continue;
+        // the images of this field pass (uniform), each drawing the
+        // triangles of its own objects
+        for (uint32_t u = jimg[jb]; u < jimg[jb + 1]; u++) {
+          const uint32_t k = jimg[u];
+          if (obj < rjobs[k].obj0 || obj >= rjobs[k].obj1) continue;
+          const ViewDev<R> v = rjobs[k].v;
+          draw_triangle<R, MODE>(v, rjobs[k].px0, P, uint32_t(g + 1), key, id, s_frag, &s_n, kFragLdsFused);
+        }
+      }
+      __syncthreads();
+      if constexpr (MODE == kRasterDepth64) {
+        const uint32_t n = s_n < kFragLdsFused ? s_n : kFragLdsFused;
+        if (threadIdx.x == 0) {
+          s_base = n ? atomicAdd(&hdr->nfrag, n) : 0u;
+          if (s_n > kFragLdsFused || (n && uint64_t(s_base) + n > fcap)) atomicOr(&hdr->overflow, 2u);
+        }
+        __syncthreads();
+        const uint32_t b0 = s_base;
+        for (uint32_t e = threadIdx.x; e < n; e += kThreads) {
+          if (uint64_t(b0) + e < fcap) frags[b0 + e] = s_frag[e];
+        }
+        __syncthreads();
+      }
+      // the next chunk's triangles follow this chunk's in every object
+#pragma unroll
+      for (int q = 0; q < kMaxThr; q++) base[q] += qoff[q + 1] - qoff[q];
+    }
   }
 }
 
@@ -921,6 +1115,141 @@ __global__ __launch_bounds__(kThreads) void k_resolve(MeshView<R> m, const Raste
   }
 }
 
+// Vertex normal of the vertex on cell-local edge e of cell c, recomputed
+// from the field: the cells sharing the edge in increasing linear order
+// (normal.c:78-158), in each the triangles of threshold thr with a vertex
+// on the edge in slot order, their smoothing weights summed from +0 and
+// normalised -- the sums of vertex_normal over a stored mesh, bit for bit.
+template <typename R>
+__device__ V3<R> vertex_normal_field(const FieldDev& f, double thr, uint32_t c, uint32_t e) {
+  const int64_t sy = f.ncx, sz = int64_t(f.ncx) * f.ncy;
+  V3<R> sum = {R(0), R(0), R(0)};
+  const int nn = c_nbr.n[e];
+  for (int q = 0; q < nn; q++) {
+    const int dk = c_nbr.v[e][q][0], dj = c_nbr.v[e][q][1], di = c_nbr.v[e][q][2];
+    const uint32_t want = uint32_t(c_nbr.v[e][q][3]);
+    const uint32_t c2 = uint32_t(int64_t(c) + dk * sz + dj * sy + di);
+    const CellLoc L2 = locate(f, c2);
+    double v2[8];
+    load_corners(f, L2, v2);
+    const uint32_t code = corner_code(thr, v2);
+    const uint32_t nt = c_ntri[code];
+    for (uint32_t sl = 0; sl < nt; sl++) {
+      const uint64_t ent = c_cslot[code * 12u + sl];
+      const uint32_t eu = uint32_t(ent >> 18);
+      if ((eu & 31u) == want || ((eu >> 5) & 31u) == want || ((eu >> 10) & 31u) == want) {
+        V3<R> P[3];
+        slot_positions<R>(f, L2, thr, ent, v2, P);
+        sum = vadd(sum, smoothing_weight<R>(P));
+      }
+    }
+  }
+  return vunit(sum);
+}
+
+// k_resolve for a frame without a stored mesh: the nearest triangle is
+// found from its serial id (object by base, cell from the stored cell
+// index, slot by the cell's first triangle) and recomputed from its field.
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_resolve_field(const TessJob* __restrict__ jobs,
+                                                            const uint32_t* __restrict__ cell,
+                                                            const RasterJob<R>* __restrict__ rjobs, int njobs,
+                                                            uint64_t npx_total, const ObjMeta* __restrict__ meta,
+                                                            int nobj, const uint8_t* __restrict__ frag_rgb,
+                                                            const FrameHdr* __restrict__ hdr,
+                                                            const uint64_t* __restrict__ key,
+                                                            const uint32_t* __restrict__ idbuf) {
+  // a frame whose triangles outgrew the cell store is rendered again: skip it
+  if (hdr->overflow & 1u) return;
+  const uint64_t ntri = hdr->ntri;
+  for (uint64_t p0 = uint64_t(blockIdx.x) * kThreads; p0 < npx_total; p0 += uint64_t(gridDim.x) * kThreads) {
+    int jb = 0;
+    if (njobs > 1) {
+      int lo = 0, hi = njobs - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (rjobs[mid].px0 <= p0) lo = mid;
+        else hi = mid - 1;
+      }
+      jb = lo;
+    }
+    const RasterJob<R>& J = rjobs[jb];
+    const ViewDev<R> v = J.v;
+    const uint64_t p = p0 + threadIdx.x;
+    const uint64_t lp = p - J.px0;
+    if (lp >= uint64_t(v.W) * uint64_t(v.H)) continue;
+    const int32_t j = int32_t(lp / uint64_t(v.W));
+    const int32_t i = int32_t(lp - uint64_t(j) * uint64_t(v.W));
+    const uint64_t k = key[p];
+    uint32_t id;
+    double dep;
+    if constexpr (sizeof(R) == 8) {
+      id = idbuf[p];
+      dep = unord64(k);
+    } else {
+      id = uint32_t(k);
+      dep = id ? double(unord32(uint32_t(k >> 32))) : -DBL_MAX;
+    }
+    uint8_t r = J.bg[0], gg = J.bg[1], b = J.bg[2];
+    if (id != 0 && id <= ntri) {
+      const uint64_t t = id - 1;
+      int lo = 0, hi = nobj - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (meta[mid].base <= t) lo = mid;
+        else hi = mid - 1;
+      }
+      const ObjMeta om = meta[lo];
+      const FieldDev f = jobs[om.job].f;
+      double thr = jobs[om.job].ts.thr[0];
+      for (uint32_t z = 1; z < kMaxThr; z++) thr = om.q == z ? jobs[om.job].ts.thr[z] : thr;
+      const uint32_t c = cell[t];
+      // a cell index outside the object's field or its interior, or a slot
+      // outside its cell, can only come from a damaged cell store: such a pixel keeps
+      // the background rather than read out of bounds
+      do {
+        if (c >= f.ncells) break;
+        const uint32_t tl = c / kCellsPerBlock;
+        const uint64_t first = lower_bound_cell(cell, om.bstart[tl], om.bstart[tl + 1], c);
+        const CellLoc L = locate(f, c);
+        if (!interior_cell(f, L)) break;
+        double cv[8];
+        load_corners(f, L, cv);
+        const uint32_t code = corner_code(thr, cv);
+        if (t < first || t - first >= c_ntri[code]) break;
+        const uint64_t ent = c_cslot[code * 12u + uint32_t(t - first)];
+        V3<R> wpos[3];
+        slot_positions<R>(f, L, thr, ent, cv, wpos);
+        Projected<R> P;
+#pragma unroll
+        for (int n = 0; n < 3; n++) project(v, wpos[n], P.q[n]);
+        raster_box(v, P);
+        R w0, w1, w2;
+        pixel_weights(P, pixel_centre<R>(i, v.fw), pixel_centre<R>(j, v.fh), w0, w1, w2);
+        V3<R> nv[3];
+#pragma unroll 1
+        for (int n = 0; n < 3; n++) nv[n] = vertex_normal_field<R>(f, thr, c, uint32_t(ent >> (18 + 5 * n)) & 31u);
+        const V3<R> n = vunit(vadd(vscale(w0, nv[0]), vadd(vscale(w1, nv[1]), vscale(w2, nv[2]))));
+        const V3<R> light = {v.light[0], v.light[1], v.light[2]};
+        const R fct = fmax(R(0.15), fabs(vdot(n, light)));
+        r = uint8_t(fct * R(om.rgb[0]));
+        gg = uint8_t(fct * R(om.rgb[1]));
+        b = uint8_t(fct * R(om.rgb[2]));
+      } while (false);
+    } else if (id > ntri) {
+      const uint64_t fi = id - ntri - 1;
+      r = frag_rgb[3 * fi + 0];
+      gg = frag_rgb[3 * fi + 1];
+      b = frag_rgb[3 * fi + 2];
+    }
+    const uint64_t o = (uint64_t(v.H - 1 - j) * uint64_t(v.W) + uint64_t(i)) * 3;
+    J.rgb[o + 0] = r;
+    J.rgb[o + 1] = gg;
+    J.rgb[o + 2] = b;
+    if (J.depth) J.depth[lp] = dep;
+  }
+}
+
 // ----------------------------------------------------------------------
 // host launchers
 
@@ -1042,6 +1371,46 @@ hipError_t launch_mesh_compare(const MeshView<double>& a, const ObjMeta* ma, con
   return hipGetLastError();
 }
 
+template <typename R>
+hipError_t launch_tess_raster(const TessJob* jobs, int njobs, uint32_t ntiles, const uint64_t* start, uint32_t* cell,
+                              uint64_t cap, const RasterJob<R>* rjobs, const uint32_t* jimg, FrameHdr* hdr, int mode,
+                              uint64_t* key, uint32_t* id, Frag* frags, uint32_t fcap, uint32_t grid, hipStream_t st) {
+  if (ntiles == 0) return hipSuccess;
+  if ((sizeof(R) == 4) != (mode == kRasterFp32)) return hipErrorInvalidValue;
+  const dim3 g(grid8(ntiles, grid));
+  if (mode == kRasterFp32)
+    hipLaunchKernelGGL((k_tess_raster<R, kRasterFp32>), g, dim3(kThreads), 0, st, jobs, njobs, ntiles, start, cell, cap,
+                       rjobs, jimg, hdr, key, id, frags, fcap);
+  else if (mode == kRasterDepth64)
+    hipLaunchKernelGGL((k_tess_raster<R, kRasterDepth64>), g, dim3(kThreads), 0, st, jobs, njobs, ntiles, start, cell,
+                       cap, rjobs, jimg, hdr, key, id, frags, fcap);
+  else
+    hipLaunchKernelGGL((k_tess_raster<R, kRasterId64>), g, dim3(kThreads), 0, st, jobs, njobs, ntiles, start, cell,
+                       cap, rjobs, jimg, hdr, key, id, frags, fcap);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_resolve_field(const TessJob* jobs, const uint32_t* cell, const RasterJob<R>* rjobs, int njobs,
+                                uint64_t npx_total, const ObjMeta* meta, int nobj, const uint8_t* frag_rgb,
+                                const FrameHdr* hdr, const uint64_t* key, const uint32_t* id, hipStream_t st) {
+  hipLaunchKernelGGL(k_resolve_field<R>, dim3(grid8((npx_total + kThreads - 1) / kThreads, 1u << 30)), dim3(kThreads),
+                     0, st, jobs, cell, rjobs, njobs, npx_total, meta, nobj, frag_rgb, hdr, key, id);
+  return hipGetLastError();
+}
+
+template hipError_t launch_tess_raster<double>(const TessJob*, int, uint32_t, const uint64_t*, uint32_t*, uint64_t,
+                                               const RasterJob<double>*, const uint32_t*, FrameHdr*, int, uint64_t*,
+                                               uint32_t*, Frag*, uint32_t, uint32_t, hipStream_t);
+template hipError_t launch_tess_raster<float>(const TessJob*, int, uint32_t, const uint64_t*, uint32_t*, uint64_t,
+                                              const RasterJob<float>*, const uint32_t*, FrameHdr*, int, uint64_t*,
+                                              uint32_t*, Frag*, uint32_t, uint32_t, hipStream_t);
+template hipError_t launch_resolve_field<double>(const TessJob*, const uint32_t*, const RasterJob<double>*, int,
+                                                 uint64_t, const ObjMeta*, int, const uint8_t*, const FrameHdr*,
+                                                 const uint64_t*, const uint32_t*, hipStream_t);
+template hipError_t launch_resolve_field<float>(const TessJob*, const uint32_t*, const RasterJob<float>*, int,
+                                                uint64_t, const ObjMeta*, int, const uint8_t*, const FrameHdr*,
+                                                const uint64_t*, const uint32_t*, hipStream_t);
 template hipError_t launch_plan<double>(const uint64_t*, uint64_t, ObjMeta*, int, RasterJob<double>*, int, FrameHdr*,
                                         uint64_t, hipStream_t);
 template hipError_t launch_plan<float>(const uint64_t*, uint64_t, ObjMeta*, int, RasterJob<float>*, int, FrameHdr*,
