@@ -553,6 +553,7 @@ ViewDev<R> dev_view(const HostView& h) {
   v.lyy = R(h.lyy);
   v.fw = R(double(h.W));
   v.fh = R(double(h.H));
+  set_pitch(v);
   v.W = int32_t(h.W);
   v.H = int32_t(h.H);
   return v;

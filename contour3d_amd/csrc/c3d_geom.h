@@ -16,7 +16,9 @@
 
 namespace c3d {
 
+using std::ceil;
 using std::fabs;
+using std::floor;
 using std::fmax;
 using std::fmin;
 using std::sqrt;
@@ -60,9 +62,16 @@ template <typename R>
 struct ViewDev {
   R cam[3], ctr[3], lx[3], ly[3], nrm[3], light[3];
   R nc, d, lxx, lyy;
-  R fw, fh;  // width, height as reals
+  R fw, fh;   // width, height as reals
+  R pitch;    // max(1 / fw, 1 / fh): the larger pixel spacing (set_pitch)
   int32_t W, H;
 };
+
+// fills the derived pixel spacing of a view whose W, H, fw, fh are set
+template <typename R>
+C3D_HD void set_pitch(ViewDev<R>& v) {
+  v.pitch = fmax(R(1) / v.fw, R(1) / v.fh);
+}
 
 // perspective projection (project.c:9-67); false when rejected
 template <typename R>
@@ -150,48 +159,53 @@ C3D_HD R pixel_depth(const Projected<R>& P, R w0, R w1, R w2) {
 }
 
 // Narrows P's box to the pixels whose centres pixel_weights can accept,
-// so the coverage loop skips the reference box's margin.  Pixel centres are
-// evaluated with pixel_centre, the expression the coverage loop uses.
+// so the coverage loop skips the reference box's margin.  Needs the
+// P.ainv of raster_box and v.pitch (set_pitch).
 //
 // Bound: with exact weights w_i = E_i / A (sum 1, sum w_i q_i = p), a pixel
 // centre p farther than M to the left of every vertex has, for
 // d_i = q_i.x - p.x in (M, D], sum over negative w_i of |w_i| d_i equal to
 // the positive part, which is at least M; so some w_i < -M / (2D).  The
 // computed weights differ from the exact ones by at most 4 eps D / h (h a
-// lower bound of the triangle's heights) plus a relative 1e-3, so with
-// M = 2D (2e-8 + 32 eps D / h) (+ the rounding of the comparison bounds)
-// the computed test rejects p.  Same for the other three sides.  Thin or
-// tiny triangles (error bound above 1e-3, area near the 1e-16 clamp) and
-// non-finite values keep the reference box.
+// lower bound of the triangle's heights, here A / (1.5 emax)) plus a
+// relative 1e-3, so with M = 2D (2e-8 + 32 eps D / h) (+ the rounding of
+// the comparison bounds) the computed test rejects p.  Same for the other
+// three sides.  Thin or tiny triangles (error bound above 1e-3, area near
+// the 1e-16 clamp) and non-finite values keep the reference box.
+//
+// Columns: an accepted centre has pixel_centre(i) >= xlo, and with
+// u = (xlo + 0.5) fw - 0.5 the rounding of u and of pixel_centre stays
+// below delta = 4 eps (|u| + fw + 2) index units, so i >= u - delta: the
+// box starts at ceil(u - delta) (when u lies within delta of an integer it
+// may hold one column more than needed, never one less).  Same for the
+// other bounds.  No division is needed.
 template <typename R>
 C3D_HD void tight_box(const ViewDev<R>& v, R area, Projected<R>& P) {
   const R eps = sizeof(R) == 8 ? R(2.220446049250313e-16) : R(1.1920929e-7);
   const V3<R>* q = P.q;
   const R minx = fmin(q[0].x, fmin(q[1].x, q[2].x)), maxx = fmax(q[0].x, fmax(q[1].x, q[2].x));
   const R miny = fmin(q[0].y, fmin(q[1].y, q[2].y)), maxy = fmax(q[0].y, fmax(q[1].y, q[2].y));
-  const R pitch = fmax(R(1) / v.fw, R(1) / v.fh);
+  const R pitch = v.pitch;
   const R D = fmax(maxx - minx, maxy - miny) + R(4) * pitch;
   const R emax = fmax(fmax(fmax(fabs(q[1].x - q[0].x), fabs(q[1].y - q[0].y)),
                            fmax(fabs(q[2].x - q[1].x), fabs(q[2].y - q[1].y))),
                       fmax(fabs(q[0].x - q[2].x), fabs(q[0].y - q[2].y)));
   const R A = fabs(area);
   if (!(A > R(1.e-14)) || !(emax > R(0))) return;
-  const R h = A / (R(1.5) * emax);
-  const R rel = R(16) * eps * D / h;
+  // rel = 16 eps D / h = 24 eps D emax / A; |P.ainv| is 1 / A rounded once
+  // (A > 1e-16 is not clamped), and 1 + 2^-20 covers the products' rounding
+  const R rel = R(24) * eps * D * emax * fabs(P.ainv) * R(1.00000095367431640625);
   if (!(rel < R(1.e-3))) return;
   const R qmax = fmax(fmax(fmax(fabs(minx), fabs(maxx)), fmax(fabs(miny), fabs(maxy))), R(1));
   const R M = R(2) * D * (R(2.e-8) + R(2) * rel) + R(16) * eps * qmax;
   if (!(M < pitch)) return;
-  const R xlo = minx - M, xhi = maxx + M, ylo = miny - M, yhi = maxy + M;
-  // first column with centre >= xlo: start at or below it and walk up
-  int32_t a = clampi(P.i0, trunc_i<R>((xlo + R(0.5)) * v.fw - R(0.5)) - 1, P.i1 + 1);
-  while (a <= P.i1 && pixel_centre<R>(a, v.fw) < xlo) a++;
-  int32_t b = clampi(a - 1, trunc_i<R>((xhi + R(0.5)) * v.fw - R(0.5)) + 2, P.i1);
-  while (b >= a && pixel_centre<R>(b, v.fw) > xhi) b--;
-  int32_t c = clampi(P.j0, trunc_i<R>((ylo + R(0.5)) * v.fh - R(0.5)) - 1, P.j1 + 1);
-  while (c <= P.j1 && pixel_centre<R>(c, v.fh) < ylo) c++;
-  int32_t d = clampi(c - 1, trunc_i<R>((yhi + R(0.5)) * v.fh - R(0.5)) + 2, P.j1);
-  while (d >= c && pixel_centre<R>(d, v.fh) > yhi) d--;
+  const R ulo = (minx - M + R(0.5)) * v.fw - R(0.5), uhi = (maxx + M + R(0.5)) * v.fw - R(0.5);
+  const R vlo = (miny - M + R(0.5)) * v.fh - R(0.5), vhi = (maxy + M + R(0.5)) * v.fh - R(0.5);
+  const R dx = R(4) * eps * (v.fw + R(2)), dy = R(4) * eps * (v.fh + R(2));
+  const int32_t a = clampi(P.i0, trunc_i<R>(ceil(ulo - (dx + R(4) * eps * fabs(ulo)))), P.i1 + 1);
+  const int32_t b = clampi(a - 1, trunc_i<R>(floor(uhi + (dx + R(4) * eps * fabs(uhi)))), P.i1);
+  const int32_t c = clampi(P.j0, trunc_i<R>(ceil(vlo - (dy + R(4) * eps * fabs(vlo)))), P.j1 + 1);
+  const int32_t d = clampi(c - 1, trunc_i<R>(floor(vhi + (dy + R(4) * eps * fabs(vhi)))), P.j1);
   P.i0 = a;
   P.i1 = b;
   P.j0 = c;

@@ -734,9 +734,8 @@ __device__ __forceinline__ void draw_triangle(const ViewDev<R>& v, uint64_t row0
   // the FP32 form skips triangles whose projection has no area (their
   // weights are all +-0 and their depth 1 / +-0)
   if (sizeof(R) == 4 && area == R(0)) return;
-  // FP64 pays for narrowing the box; in FP32 the box arithmetic costs more
-  // than the candidates it removes
-  if constexpr (sizeof(R) == 8) tight_box(v, area, P);
+  // about 12 reference-box candidates of a pixel-sized triangle narrow to 3
+  tight_box(v, area, P);
   for (int32_t j = P.j0; j <= P.j1; j++) {
     const R cy = pixel_centre<R>(j, v.fh);
     const uint64_t prow = row0 + uint64_t(j) * uint64_t(v.W);
@@ -848,18 +847,108 @@ __global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, const Raster
 // ----------------------------------------------------------------------
 // frames without a stored mesh: tessellation fused with the raster
 //
-// The emit pass's classification, scan and per-triangle arithmetic, after
-// which each triangle is drawn from registers into every image that shows
-// its object (jimg: offsets per field pass into a list of the images that
-// show any of its objects) and only its cell index is stored; the resolve recomputes what it needs from the
-// field.  In Depth64 mode the kept fragments collect in LDS and are flushed
-// once per chunk; Id64 (the geometry id pass) runs only after the list
-// overflowed.  A frame flagged by the plan as outgrowing `cap` draws
-// nothing (it is rendered again).
+// Each tile is classified and scanned in chunks of 256 cells as in the emit
+// pass, but a chunk holds only about a third as many triangles as cells, so
+// its triangles are not drawn in place.  They are queued instead, as
+// compact items (serial position, cell, corner code, slot, threshold), in a
+// per-workgroup LDS ring that carries them over chunk and tile boundaries
+// (within one field pass), and the ring is drained 256 items at a time:
+// every lane then computes, projects and draws one triangle.  A drained
+// triangle rereads the corner values its vertices need (they were loaded
+// moments earlier by the same workgroup, so they come from cache).  Only
+// the cell index of each triangle is stored (at its serial position); the
+// resolve recomputes what it needs from the field.  Triangles are drawn
+// into every image of the field pass that shows their object (jimg:
+// offsets per field pass into a list of the images showing any of its
+// objects).  In Depth64 mode the kept fragments collect in LDS and are
+// flushed after every drain; Id64 (the geometry id pass) runs only after
+// the list overflowed.  A frame flagged by the plan as outgrowing `cap`
+// draws nothing (it is rendered again).
 constexpr uint32_t kFragLdsFused = 512;
+constexpr uint32_t kRing = 512;  // a power of two, at least 2 * kThreads
+
+struct FusedLds {
+  uint8_t code[kMaxThr][kThreads];
+  uint16_t pref[kMaxThr][kThreads];
+  uint64_t wave[kThreads / 64];
+  uint8_t ntri[256];
+  uint32_t rg[kRing];  // serial position of the queued triangle
+  uint32_t rc[kRing];  // its cell
+  uint16_t rx[kRing];  // corner code | slot << 8 | threshold << 12
+  uint32_t nfrag, fbase, fcnt;
+};
+
+// positions of the triangle in slot entry `ent` of cell L, reading the two
+// corner values of each vertex's lattice edge from the field
+template <typename R>
+__device__ __forceinline__ void slot_positions_mem(const FieldDev& f, const CellLoc& L, double thr, uint64_t ent,
+                                                   V3<R> P[3]) {
+  const double* p = f.f + L.p0;
+#pragma unroll
+  for (int vv = 0; vv < 3; vv++) {
+    const uint32_t tail = uint32_t(ent >> (6 * vv)) & 7u;
+    const uint32_t head = uint32_t(ent >> (6 * vv + 3)) & 7u;
+    const double vt = p[corner_offset(f, tail)];
+    const double vh = p[corner_offset(f, head)];
+    P[vv] = interp<R>(thr, corner_pos<R>(f, L, tail), vt, corner_pos<R>(f, L, head), vh);
+  }
+}
+
+// Draws the n (<= kThreads) oldest queued triangles of field pass jb, one
+// per thread, then flushes the kept fragments (Depth64).
+template <typename R, int MODE>
+__device__ __forceinline__ void drain_ring(FusedLds& S, Frag* s_frag, uint32_t head, uint32_t n,
+                                           const TessJob* __restrict__ jobs, uint32_t jb,
+                                           const RasterJob<R>* __restrict__ rjobs, const uint32_t* __restrict__ jimg,
+                                           FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
+                                           uint32_t* __restrict__ id, Frag* __restrict__ frags, uint32_t fcap) {
+  if (threadIdx.x < n) {
+    const FieldDev f = jobs[jb].f;
+    const uint32_t s = (head + threadIdx.x) & (kRing - 1u);
+    const uint32_t g = S.rg[s], cc = S.rc[s], x = S.rx[s];
+    const CellLoc L = locate(f, cc);
+    if (interior_cell(f, L)) {
+      const uint32_t q = x >> 12;
+      double thr = jobs[jb].ts.thr[0];
+      uint32_t obj = jobs[jb].ts.obj[0];
+#pragma unroll
+      for (uint32_t z = 1; z < kMaxThr; z++) {
+        thr = q == z ? jobs[jb].ts.thr[z] : thr;
+        obj = q == z ? jobs[jb].ts.obj[z] : obj;
+      }
+      const uint64_t ent = c_cslot[(x & 255u) * 12u + ((x >> 8) & 15u)];
+      V3<R> P[3];
+      slot_positions_mem<R>(f, L, thr, ent, P);
+      for (uint32_t u = jimg[jb]; u < jimg[jb + 1]; u++) {
+        const uint32_t k = jimg[u];
+        if (obj < rjobs[k].obj0 || obj >= rjobs[k].obj1) continue;
+        const ViewDev<R> v = rjobs[k].v;
+        draw_triangle<R, MODE>(v, rjobs[k].px0, P, g + 1u, key, id, s_frag, &S.nfrag, kFragLdsFused);
+      }
+    }
+  }
+  __syncthreads();
+  if constexpr (MODE == kRasterDepth64) {
+    if (threadIdx.x == 0) {
+      const uint32_t all = S.nfrag;
+      const uint32_t c = all < kFragLdsFused ? all : kFragLdsFused;
+      const uint32_t b = c ? atomicAdd(&hdr->nfrag, c) : 0u;
+      if (all > kFragLdsFused || (c && uint64_t(b) + c > fcap)) atomicOr(&hdr->overflow, 2u);
+      S.fbase = b;
+      S.fcnt = c;
+      S.nfrag = 0;
+    }
+    __syncthreads();
+    const uint32_t b0 = S.fbase, c = S.fcnt;
+    for (uint32_t e = threadIdx.x; e < c; e += kThreads) {
+      if (uint64_t(b0) + e < fcap) frags[b0 + e] = s_frag[e];
+    }
+    __syncthreads();
+  }
+}
 
 template <typename R, int MODE>
-__global__ __launch_bounds__(kThreads) void k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
                                                           const uint64_t* __restrict__ start,
                                                           uint32_t* __restrict__ cellout, uint64_t cap,
                                                           const RasterJob<R>* __restrict__ rjobs,
@@ -867,120 +956,105 @@ __global__ __launch_bounds__(kThreads) void k_tess_raster(const TessJob* __restr
                                                           FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
                                                           uint32_t* __restrict__ id, Frag* __restrict__ frags,
                                                           uint32_t fcap) {
-  __shared__ double s_val[8][kThreads];
-  __shared__ uint8_t s_code[kMaxThr][kThreads];
-  __shared__ uint16_t s_pref[kMaxThr][kThreads];
-  __shared__ uint64_t s_wave[kThreads / 64];
-  __shared__ uint8_t s_ntri[256];
+  __shared__ FusedLds S;
   __shared__ Frag s_frag[MODE == kRasterDepth64 ? kFragLdsFused : 1];
-  __shared__ uint32_t s_n, s_base;
   if (hdr->overflow & 1u) return;
   if (MODE == kRasterId64 && !(hdr->overflow & 2u)) return;
-  s_ntri[threadIdx.x] = c_ntri[threadIdx.x];
+  S.ntri[threadIdx.x] = c_ntri[threadIdx.x];
+  if (threadIdx.x == 0) S.nfrag = 0;
   __syncthreads();
+  // ring state, the same in every thread: oldest item, items queued, and
+  // the field pass they belong to
+  uint32_t head = 0, npend = 0, cur = 0;
   const XcdRange XR = xcd_range(ntiles);
   for (uint32_t gt = XR.begin; gt < XR.end; gt += XR.step) {
     const uint32_t jb = find_tjob(jobs, njobs, gt);
+    // the queue holds one field pass at a time
+    while (npend && jb != cur) {
+      const uint32_t n = npend < kThreads ? npend : kThreads;
+      drain_ring<R, MODE>(S, s_frag, head, n, jobs, cur, rjobs, jimg, hdr, key, id, frags, fcap);
+      head = (head + n) & (kRing - 1u);
+      npend -= n;
+    }
+    cur = jb;
     const FieldDev f = jobs[jb].f;
     const ThrSet ts = jobs[jb].ts;
     const uint32_t tile = gt - jobs[jb].tile0;
     uint64_t base[kMaxThr];
-    double thrs[kMaxThr];
-    uint32_t objs[kMaxThr];
 #pragma unroll
-    for (int q = 0; q < kMaxThr; q++) {
-      base[q] = q < ts.n ? start[ts.boff[q] + tile] : 0;
-      thrs[q] = ts.thr[q];
-      objs[q] = ts.obj[q];
-    }
+    for (int q = 0; q < kMaxThr; q++) base[q] = q < ts.n ? start[ts.boff[q] + tile] : 0;
     for (int chunk = 0; chunk < kCellsPerBlock / kThreads; chunk++) {
       const uint32_t c0 = tile * kCellsPerBlock + chunk * kThreads;
       if (c0 >= f.ncells) break;
       const uint32_t c = c0 + threadIdx.x;
       const bool valid = c < f.ncells;
-      double v[8];
-      if (valid) {
-        load_corners(f, locate(f, c), v);
-      } else {
-#pragma unroll
-        for (int q = 0; q < 8; q++) v[q] = 0.0;
-      }
-#pragma unroll
-      for (int q = 0; q < 8; q++) s_val[q][threadIdx.x] = v[q];
-      if (MODE == kRasterDepth64 && threadIdx.x == 0) s_n = 0;
       uint32_t code[kMaxThr];
-      const uint64_t packed = valid ? packed_counts(ts, v, code, s_ntri) : 0ull;
+      uint64_t packed = 0;
+      if (valid) {
+        double v[8];
+        load_corners(f, locate(f, c), v);
+        packed = packed_counts(ts, v, code, S.ntri);
+      }
       uint64_t tot;
-      const uint64_t ex = block_excl_scan64(packed, s_wave, tot);
+      const uint64_t ex = block_excl_scan64(packed, S.wave, tot);
       uint32_t qoff[kMaxThr + 1];
       qoff[0] = 0;
 #pragma unroll
       for (int q = 0; q < kMaxThr; q++) {
         qoff[q + 1] = qoff[q] + (uint32_t(tot >> (16 * q)) & 0xffffu);
-        s_pref[q][threadIdx.x] = uint16_t(ex >> (16 * q));
-        if (valid) s_code[q][threadIdx.x] = uint8_t(code[q]);
+        S.pref[q][threadIdx.x] = uint16_t(ex >> (16 * q));
+        if (valid) S.code[q][threadIdx.x] = uint8_t(code[q]);
       }
       __syncthreads();
       const uint32_t total = qoff[kMaxThr];
-      for (uint32_t it = threadIdx.x; it < total; it += kThreads) {
-        int q = 0;
+      for (uint32_t pos = 0; pos < total;) {
+        // queue as many of the chunk's triangles as the ring holds, in
+        // serial order (threshold, cell, slot)
+        const uint32_t room = kRing - npend;
+        const uint32_t take = total - pos < room ? total - pos : room;
+        for (uint32_t e = threadIdx.x; e < take; e += kThreads) {
+          const uint32_t it = pos + e;
+          int q = 0;
 #pragma unroll
-        for (int z = 1; z < kMaxThr; z++) q += it >= qoff[z] ? 1 : 0;
-        uint32_t qsel = 0;
+          for (int z = 1; z < kMaxThr; z++) q += it >= qoff[z] ? 1 : 0;
+          uint32_t qsel = 0;
 #pragma unroll
-        for (int z = 1; z < kMaxThr; z++) qsel = q == z ? qoff[z] : qsel;
-        const uint32_t r = it - qsel;
-        uint32_t l = 0;
+          for (int z = 1; z < kMaxThr; z++) qsel = q == z ? qoff[z] : qsel;
+          const uint32_t r = it - qsel;
+          uint32_t l = 0;
 #pragma unroll
-        for (uint32_t step = kThreads / 2; step; step >>= 1) l += s_pref[q][l + step] <= r ? step : 0u;
-        const uint32_t sl = r - s_pref[q][l];
-        double thr = thrs[0];
-        uint64_t bq = base[0];
-        uint32_t obj = objs[0];
+          for (uint32_t step = kThreads / 2; step; step >>= 1) l += S.pref[q][l + step] <= r ? step : 0u;
+          const uint32_t sl = r - S.pref[q][l];
+          uint64_t bq = base[0];
 #pragma unroll
-        for (int z = 1; z < kMaxThr; z++) {
-          thr = q == z ? thrs[z] : thr;
-          bq = q == z ? base[z] : bq;
-          obj = q == z ? objs[z] : obj;
+          for (int z = 1; z < kMaxThr; z++) bq = q == z ? base[z] : bq;
+          const uint64_t g = bq + r;
+          const uint32_t cc = c0 + l;
+          if (MODE != kRasterId64 && g < cap) cellout[g] = cc;
+          const uint32_t s = (head + npend + e) & (kRing - 1u);
+          S.rg[s] = uint32_t(g);
+          S.rc[s] = cc;
+          S.rx[s] = uint16_t(uint32_t(S.code[q][l]) | sl << 8 | uint32_t(q) << 12);
         }
-        const uint32_t cc = c0 + l;
-        const CellLoc L = locate(f, cc);
-        const uint64_t ent = c_cslot[uint32_t(s_code[q][l]) * 12u + sl];
-        double cv[8];
-#pragma unroll
-        for (int z = 0; z < 8; z++) cv[z] = s_val[z][l];
-        V3<R> P[3];
-        slot_positions<R>(f, L, thr, ent, cv, P);
-        const uint64_t g = bq + r;
-        if (MODE != kRasterId64 && g < cap) cellout[g] = cc;
-        if (!interior_cell(f, L)) continue;
-        // the images of this field pass (uniform), each drawing the
-        // triangles of its own objects
-        for (uint32_t u = jimg[jb]; u < jimg[jb + 1]; u++) {
-          const uint32_t k = jimg[u];
-          if (obj < rjobs[k].obj0 || obj >= rjobs[k].obj1) continue;
-          const ViewDev<R> v = rjobs[k].v;
-          draw_triangle<R, MODE>(v, rjobs[k].px0, P, uint32_t(g + 1), key, id, s_frag, &s_n, kFragLdsFused);
-        }
-      }
-      __syncthreads();
-      if constexpr (MODE == kRasterDepth64) {
-        const uint32_t n = s_n < kFragLdsFused ? s_n : kFragLdsFused;
-        if (threadIdx.x == 0) {
-          s_base = n ? atomicAdd(&hdr->nfrag, n) : 0u;
-          if (s_n > kFragLdsFused || (n && uint64_t(s_base) + n > fcap)) atomicOr(&hdr->overflow, 2u);
-        }
+        pos += take;
+        npend += take;
         __syncthreads();
-        const uint32_t b0 = s_base;
-        for (uint32_t e = threadIdx.x; e < n; e += kThreads) {
-          if (uint64_t(b0) + e < fcap) frags[b0 + e] = s_frag[e];
+        while (npend >= kThreads) {
+          drain_ring<R, MODE>(S, s_frag, head, kThreads, jobs, jb, rjobs, jimg, hdr, key, id, frags, fcap);
+          head = (head + kThreads) & (kRing - 1u);
+          npend -= kThreads;
         }
-        __syncthreads();
       }
       // the next chunk's triangles follow this chunk's in every object
 #pragma unroll
       for (int q = 0; q < kMaxThr; q++) base[q] += qoff[q + 1] - qoff[q];
     }
+  }
+  while (npend) {
+    const uint32_t n = npend < kThreads ? npend : kThreads;
+    drain_ring<R, MODE>(S, s_frag, head, n, jobs, cur, rjobs, jimg, hdr, key, id, frags, fcap);
+    head = (head + n) & (kRing - 1u);
+    npend -= n;
   }
 }
 

@@ -60,6 +60,7 @@ static Stats<R> run(unsigned seed, long per_kind) {
     v.H = sizes[view][1];
     v.fw = R(double(v.W));
     v.fh = R(double(v.H));
+    set_pitch(v);
     std::vector<R> ax(v.W + v.H);
     for (int32_t x = 0; x < v.W + v.H; x++)
       ax[x] = x < v.W ? pixel_centre<R>(x, v.fw) : pixel_centre<R>(x - v.W, v.fh);
