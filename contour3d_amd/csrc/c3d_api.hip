@@ -170,6 +170,9 @@ namespace {
 
 // triangle ids are u32 and line fragments take ids after the triangles
 constexpr uint64_t kMaxFrameTriangles = (uint64_t(1) << 32) - (uint64_t(1) << 24);
+// initial fragment-list entries of a frame (16 B each): every covered pixel
+// of a 512^3 field's frame at 1920x1080 fits
+constexpr uint32_t kDefaultFragCap = 1u << 24;
 
 // Host-side layout of a frame's tessellation.
 struct TessLayout {
@@ -947,10 +950,9 @@ int raster_prepare(Frame& F, uint64_t npx) {
   F.h_nline = nf;
   C3D_HIP(hipMemcpyAsync(F.hdr.as<unsigned char>() + offsetof(FrameHdr, nline), &F.h_nline, sizeof(uint32_t),
                          hipMemcpyHostToDevice, F.st));
-  if (fp64) {
-    if (F.frag_cap == 0) F.frag_cap = 1u << 22;
-    C3D_HIP(F.frags.ensure(size_t(F.frag_cap) * sizeof(Frag)));
-  }
+  (void)fp64;
+  if (F.frag_cap == 0) F.frag_cap = kDefaultFragCap;
+  C3D_HIP(F.frags.ensure(size_t(F.frag_cap) * sizeof(Frag)));
   return 0;
 }
 
@@ -1039,15 +1041,15 @@ void release_geoms(Frame& F) {
   F.geoms_used.clear();
 }
 
-// Waits for a frame's device work and host copies; grows the FP64 fragment
-// list after an overflow.
+// Waits for a frame's device work and host copies; grows the fragment list
+// after an overflow.
 int frame_finish(Frame& F) {
   const hipError_t e = hipEventSynchronize(F.done);
   release_geoms(F);
   C3D_HIP(e);
   FrameHdr& fin = F.last;
   std::memcpy(&fin, F.hstage.p, sizeof(FrameHdr));
-  if (F.fp64 && (fin.overflow & 2u) && fin.nfrag > F.frag_cap) {
+  if ((fin.overflow & 2u) && fin.nfrag > F.frag_cap) {
     const uint64_t grow = uint64_t(fin.nfrag) + fin.nfrag / 4;
     F.frag_cap = uint32_t(grow > 0xffffffffull ? 0xffffffffull : grow);
   }
@@ -1165,15 +1167,21 @@ int fused_launch(Lib& L, Frame& F, const contour3d_amd_mesh& M, const RasterJob<
   const uint64_t* start = M.bstart.as<uint64_t>();
   uint32_t* cell = M.cell.as<uint32_t>();
   const uint32_t grid = L.tess_grid(tl.ntj);
+  const uint32_t fgrid = std::min<uint32_t>((F.frag_cap + 255) / 256, 4096);
+  Frag* frags = F.frags.as<Frag>();
   if (prof_begin(F, CONTOUR3D_AMD_STAGE_RASTER)) return 1;
   C3D_HIP(launch_zinit(key, id, npx, fp64 ? 0 : 1, hdr, F.st));
+  // every covered pixel into the fragment list, then its keys; after an
+  // overflow of the list (which grows for the next frame) the z-test passes
+  // over the geometry run instead (each kernel decides on the device)
+  C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterAppend, key, id,
+                                frags, F.frag_cap, grid, F.st));
+  C3D_HIP(launch_frag_keys(frags, F.frag_cap, hdr, key, fgrid, F.st));
   if constexpr (sizeof(R) == 8) {
-    C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterDepth64, key, id,
-                                  F.frags.as<Frag>(), F.frag_cap, grid, F.st));
+    C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterKey64, key, id,
+                                  nullptr, 0, grid, F.st));
     C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 0, key, id, F.st));
-    C3D_HIP(launch_frag_ids(F.frags.as<Frag>(), F.frag_cap, hdr, key, id,
-                            std::min<uint32_t>((F.frag_cap + 255) / 256, 4096), F.st));
-    // the geometry id pass: returns at once unless the fragment list overflowed
+    C3D_HIP(launch_frag_ids(frags, F.frag_cap, hdr, key, id, fgrid, F.st));
     C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterId64, key, id,
                                   nullptr, 0, grid, F.st));
     C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 1, 0, key, id, F.st));
@@ -1686,7 +1694,7 @@ int contour3d_amd_tune(int param, int64_t value) {
       L.raster_cap = uint32_t(value);
       return 0;
     case CONTOUR3D_AMD_TUNE_FRAG_CAPACITY:
-      L.main.frag_cap = uint32_t(value ? value : 1);
+      L.main.frag_cap = value > 0 ? uint32_t(std::min<int64_t>(value, 0xffffffffll)) : c3d::kDefaultFragCap;
       return 0;
     case CONTOUR3D_AMD_TUNE_GRAPHS:
       L.graphs = value != 0;

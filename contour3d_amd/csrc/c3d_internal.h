@@ -169,13 +169,26 @@ hipError_t launch_resolve_field(const TessJob* jobs, const uint32_t* cell, const
                                 const FrameHdr* hdr, const uint64_t* key, const uint32_t* id, hipStream_t st);
 hipError_t launch_zinit(uint64_t* key, uint32_t* id, uint64_t npx, int precision, FrameHdr* hdr,
                         hipStream_t st);
-// raster modes: FP32 single pass over packed (depth << 32 | id) keys; FP64
-// depth pass keeping the fragments that may end up nearest; FP64 id pass
-// over the geometry (runs only when the fragment list overflowed)
-enum { kRasterFp32 = 0, kRasterDepth64 = 1, kRasterId64 = 2 };
+// raster modes:
+//   kRasterFp32     FP32 z-test over packed (depth << 32 | id) keys
+//   kRasterDepth64  FP64 z-test keeping the fragments that may end up nearest
+//   kRasterId64     FP64 id pass over the geometry (runs only when the
+//                   fragment list overflowed)
+//   kRasterAppend   no z-test: every covered pixel goes to the fragment list
+//                   with its key (FP64: the depth's order key; FP32: the
+//                   packed key); launch_frag_keys then applies the list
+//   kRasterKey64    FP64 z-test without a list (runs only when the list
+//                   overflowed, before the geometry id pass)
+// The fused kernel takes kRasterAppend and, after an overflow, kRasterFp32
+// or kRasterKey64 + kRasterId64; the stored-mesh raster takes the first three.
+enum { kRasterFp32 = 0, kRasterDepth64 = 1, kRasterId64 = 2, kRasterAppend = 3, kRasterKey64 = 4 };
 template <typename R>
 hipError_t launch_raster(const MeshView<R>& m, const RasterJob<R>* rjobs, int njobs, FrameHdr* hdr, int mode,
                          uint64_t* key, uint32_t* id, Frag* frags, uint32_t cap, uint32_t grid, hipStream_t st);
+// z-buffer keys of every listed fragment (reads the count from hdr; does
+// nothing when the list overflowed)
+hipError_t launch_frag_keys(const Frag* frags, uint32_t cap, const FrameHdr* hdr, uint64_t* key, uint32_t grid,
+                            hipStream_t st);
 // FP64 id pass over the fragment list (reads the count from hdr; does
 // nothing when the list overflowed)
 hipError_t launch_frag_ids(const Frag* frags, uint32_t cap, const FrameHdr* hdr, const uint64_t* key,

@@ -77,6 +77,16 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_wave
 // ----------------------------------------------------------------------
 // field access
 
+// Field and grid pointers reach the kernels through device-side job tables,
+// where the compiler cannot tell that they address global memory; read
+// through gptr they become global loads (scalar base + vector offset)
+// instead of flat ones, which also wait on the LDS counter.
+#define C3D_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const C3D_GLOBAL T* gptr(const T* p) {
+  return (const C3D_GLOBAL T*)p;
+}
+
 struct CellLoc {
   uint32_t i, j, k;
   uint64_t p0;  // lattice index of corner 0
@@ -100,9 +110,9 @@ template <typename R>
 __device__ __forceinline__ V3<R> corner_pos(const FieldDev& f, const CellLoc& L, uint32_t c) {
   if (f.cx) {
     const uint64_t p = L.p0 + corner_offset(f, c);
-    return {R(f.cx[p]), R(f.cy[p]), R(f.cz[p])};
+    return {R(gptr(f.cx)[p]), R(gptr(f.cy)[p]), R(gptr(f.cz)[p])};
   }
-  return {R(f.gx[L.i + (c & 1u)]), R(f.gy[L.j + ((c >> 1) & 1u)]), R(f.gz[L.k + (c >> 2)])};
+  return {R(gptr(f.gx)[L.i + (c & 1u)]), R(gptr(f.gy)[L.j + ((c >> 1) & 1u)]), R(gptr(f.gz)[L.k + (c >> 2)])};
 }
 
 // ----------------------------------------------------------------------
@@ -134,7 +144,7 @@ __device__ __forceinline__ uint64_t block_excl_scan64(uint64_t v, uint64_t* s_wa
 
 // the eight corner values of cell c (corner q = i + 2j + 4k offsets)
 __device__ __forceinline__ void load_corners(const FieldDev& f, const CellLoc& L, double v[8]) {
-  const double* p = f.f + L.p0;
+  const C3D_GLOBAL double* p = gptr(f.f) + L.p0;
   const uint64_t sy = f.nx, sz = uint64_t(f.nx) * f.ny;
   v[0] = p[0];
   v[1] = p[1];
@@ -417,12 +427,12 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TessJob* __restrict__ j
         const CellLoc L = locate(f, c);
         load_corners(f, L, v);
         if (!f.cx) {
-          s_grid[0][threadIdx.x] = f.gx[L.i];
-          s_grid[1][threadIdx.x] = f.gx[L.i + 1];
-          s_grid[2][threadIdx.x] = f.gy[L.j];
-          s_grid[3][threadIdx.x] = f.gy[L.j + 1];
-          s_grid[4][threadIdx.x] = f.gz[L.k];
-          s_grid[5][threadIdx.x] = f.gz[L.k + 1];
+          s_grid[0][threadIdx.x] = gptr(f.gx)[L.i];
+          s_grid[1][threadIdx.x] = gptr(f.gx)[L.i + 1];
+          s_grid[2][threadIdx.x] = gptr(f.gy)[L.j];
+          s_grid[3][threadIdx.x] = gptr(f.gy)[L.j + 1];
+          s_grid[4][threadIdx.x] = gptr(f.gz)[L.k];
+          s_grid[5][threadIdx.x] = gptr(f.gz)[L.k + 1];
         }
       } else {
 #pragma unroll
@@ -717,14 +727,38 @@ __device__ __forceinline__ int find_rjob(const RasterJob<R>* __restrict__ jobs, 
 constexpr int kFragLds = 1024;
 
 // Draws one triangle (world positions wpos, serial id tid1 - 1) into an
+// Appends fr to the workgroup's LDS fragment list (s_n entries, cap_lds
+// slots), one LDS atomic per wave; fragments that do not fit go straight to
+// the frame's list (gfrags, gcap entries, counted in hdr->nfrag).  Called
+// by any subset of a wave's lanes.
+__device__ __forceinline__ void append_frag(const Frag& fr, Frag* s_frag, uint32_t* s_n, uint32_t cap_lds,
+                                            Frag* __restrict__ gfrags, uint32_t gcap, FrameHdr* __restrict__ hdr) {
+  const uint64_t act = __ballot(1);
+  const int lane = int(__lane_id());
+  const int leader = __ffsll((long long)act) - 1;
+  uint32_t o0 = 0;
+  if (lane == leader) o0 = atomicAdd(s_n, uint32_t(__popcll(act)));
+  const uint32_t o = uint32_t(__shfl(int(o0), leader)) + uint32_t(__popcll(act & ((1ull << lane) - 1ull)));
+  if (o < cap_lds) {
+    s_frag[o] = fr;
+  } else {
+    const uint32_t g = atomicAdd(&hdr->nfrag, 1u);
+    if (g < gcap) gfrags[g] = fr;
+    else atomicOr(&hdr->overflow, 2u);
+  }
+}
+
 // image: projection (project.c), the reference's box and barycentric test,
 // harmonic depth (render.c:60-124), and the z-buffer update of `MODE`.
-// Depth64 keeps fragments in the workgroup list s_frag (s_n entries,
-// `cap_lds` slots).
+// Depth64 and Append put fragments in the workgroup list s_frag (s_n
+// entries, `cap_lds` slots); Append sends those that do not fit straight to
+// the frame's list (gfrags, gcap entries, counted in hdr->nfrag).
 template <typename R, int MODE>
 __device__ __forceinline__ void draw_triangle(const ViewDev<R>& v, uint64_t row0, const V3<R> wpos[3], uint32_t tid1,
                                               uint64_t* __restrict__ key, uint32_t* __restrict__ id,
-                                              Frag* s_frag, uint32_t* s_n, uint32_t cap_lds) {
+                                              Frag* s_frag, uint32_t* s_n, uint32_t cap_lds,
+                                              Frag* __restrict__ gfrags = nullptr, uint32_t gcap = 0,
+                                              FrameHdr* __restrict__ hdr = nullptr) {
   Projected<R> P;
   bool vis = true;
 #pragma unroll
@@ -748,6 +782,15 @@ __device__ __forceinline__ void draw_triangle(const ViewDev<R>& v, uint64_t row0
         // keys only grow: a key at or below a value read from the buffer
         // can never change it and skips the atomic
         const uint64_t k = uint64_t(ord32(float(dep))) << 32 | uint64_t(tid1);
+        if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+      } else if constexpr (MODE == kRasterAppend) {
+        Frag fr;
+        fr.px = uint32_t(px);
+        fr.id = tid1;
+        fr.key = sizeof(R) == 8 ? ord64(double(dep)) : (uint64_t(ord32(float(dep))) << 32 | uint64_t(tid1));
+        append_frag(fr, s_frag, s_n, cap_lds, gfrags, gcap, hdr);
+      } else if constexpr (MODE == kRasterKey64) {
+        const uint64_t k = ord64(double(dep));
         if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
       } else if constexpr (MODE == kRasterDepth64) {
         const uint64_t k = ord64(double(dep));
@@ -850,7 +893,7 @@ __global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, const Raster
 // Each tile is classified and scanned in chunks of 256 cells as in the emit
 // pass, but a chunk holds only about a third as many triangles as cells, so
 // its triangles are not drawn in place.  They are queued instead, as
-// compact items (serial position, cell, corner code, slot, threshold), in a
+// compact items (serial position, cell, vertex corners, threshold), in a
 // per-workgroup LDS ring that carries them over chunk and tile boundaries
 // (within one field pass), and the ring is drained 256 items at a time:
 // every lane then computes, projects and draws one triangle.  A drained
@@ -860,12 +903,13 @@ __global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, const Raster
 // resolve recomputes what it needs from the field.  Triangles are drawn
 // into every image of the field pass that shows their object (jimg:
 // offsets per field pass into a list of the images showing any of its
-// objects).  In Depth64 mode the kept fragments collect in LDS and are
-// flushed after every drain; Id64 (the geometry id pass) runs only after
-// the list overflowed.  A frame flagged by the plan as outgrowing `cap`
+// objects).  In Append mode every covered pixel collects in an LDS list
+// that is flushed to the frame's list once half full; the z-test modes run
+// only after that list overflowed.  A frame flagged by the plan as outgrowing `cap`
 // draws nothing (it is rendered again).
-constexpr uint32_t kFragLdsFused = 512;
-constexpr uint32_t kRing = 512;  // a power of two, at least 2 * kThreads
+constexpr uint32_t kFragLdsFused = 768;  // LDS fragment slots of a workgroup
+constexpr uint32_t kFragFlushAt = 384;   // flushed to the frame's list from this fill on
+constexpr uint32_t kRing = 512;          // a power of two, at least 2 * kThreads
 
 struct FusedLds {
   uint8_t code[kMaxThr][kThreads];
@@ -874,81 +918,118 @@ struct FusedLds {
   uint8_t ntri[256];
   uint32_t rg[kRing];  // serial position of the queued triangle
   uint32_t rc[kRing];  // its cell
-  uint16_t rx[kRing];  // corner code | slot << 8 | threshold << 12
+  uint32_t re[kRing];  // its vertices' (tail, head) corners (slot entry bits 0-17) | threshold << 18
   uint32_t nfrag, fbase, fcnt;
 };
 
-// positions of the triangle in slot entry `ent` of cell L, reading the two
-// corner values of each vertex's lattice edge from the field
+// positions of the triangle whose vertices join the corners in `ent` (slot
+// entry bits 0-17) of cell L, reading the two corner values of each
+// vertex's lattice edge from the field; on a grid lattice the cell's six
+// coordinates are read once.  Every load depends only on L and ent, so they
+// are in flight together.
 template <typename R>
-__device__ __forceinline__ void slot_positions_mem(const FieldDev& f, const CellLoc& L, double thr, uint64_t ent,
+__device__ __forceinline__ void slot_positions_mem(const FieldDev& f, const CellLoc& L, double thr, uint32_t ent,
                                                    V3<R> P[3]) {
-  const double* p = f.f + L.p0;
+  const C3D_GLOBAL double* p = gptr(f.f) + L.p0;
+  if (f.cx) {
+#pragma unroll
+    for (int vv = 0; vv < 3; vv++) {
+      const uint32_t tail = (ent >> (6 * vv)) & 7u;
+      const uint32_t head = (ent >> (6 * vv + 3)) & 7u;
+      P[vv] = interp<R>(thr, corner_pos<R>(f, L, tail), p[corner_offset(f, tail)], corner_pos<R>(f, L, head),
+                        p[corner_offset(f, head)]);
+    }
+    return;
+  }
+  const double x0 = gptr(f.gx)[L.i], x1 = gptr(f.gx)[L.i + 1], y0 = gptr(f.gy)[L.j], y1 = gptr(f.gy)[L.j + 1];
+  const double z0 = gptr(f.gz)[L.k], z1 = gptr(f.gz)[L.k + 1];
 #pragma unroll
   for (int vv = 0; vv < 3; vv++) {
-    const uint32_t tail = uint32_t(ent >> (6 * vv)) & 7u;
-    const uint32_t head = uint32_t(ent >> (6 * vv + 3)) & 7u;
-    const double vt = p[corner_offset(f, tail)];
-    const double vh = p[corner_offset(f, head)];
-    P[vv] = interp<R>(thr, corner_pos<R>(f, L, tail), vt, corner_pos<R>(f, L, head), vh);
+    const uint32_t tail = (ent >> (6 * vv)) & 7u;
+    const uint32_t head = (ent >> (6 * vv + 3)) & 7u;
+    const V3<R> pt = {R(tail & 1u ? x1 : x0), R(tail & 2u ? y1 : y0), R(tail & 4u ? z1 : z0)};
+    const V3<R> ph = {R(head & 1u ? x1 : x0), R(head & 2u ? y1 : y0), R(head & 4u ? z1 : z0)};
+    P[vv] = interp<R>(thr, pt, p[corner_offset(f, tail)], ph, p[corner_offset(f, head)]);
+  }
+}
+
+// Ends a drain: in Append mode, moves the workgroup's LDS fragments to the
+// frame's list once they fill kFragFlushAt slots (every time with `all`).
+// One reservation on the frame's counter per flush keeps that shared atomic
+// rare.
+template <int MODE>
+__device__ __forceinline__ void flush_frags(FusedLds& S, Frag* s_frag, FrameHdr* __restrict__ hdr,
+                                            Frag* __restrict__ frags, uint32_t fcap, bool all) {
+  __syncthreads();
+  if constexpr (MODE == kRasterAppend) {
+    if (threadIdx.x == 0) {
+      const uint32_t have = S.nfrag;
+      const uint32_t c = have < kFragLdsFused ? have : kFragLdsFused;
+      const bool go = c && (all || c >= kFragFlushAt);
+      uint32_t b = 0;
+      if (go) {
+        b = atomicAdd(&hdr->nfrag, c);
+        if (uint64_t(b) + c > fcap) atomicOr(&hdr->overflow, 2u);
+        S.nfrag = 0;
+      }
+      S.fbase = b;
+      S.fcnt = go ? c : 0u;
+    }
+    __syncthreads();
+    const uint32_t b0 = S.fbase, c = S.fcnt;
+    if (c) {
+      for (uint32_t e = threadIdx.x; e < c; e += kThreads) {
+        if (uint64_t(b0) + e < fcap) frags[b0 + e] = s_frag[e];
+      }
+      __syncthreads();
+    }
   }
 }
 
 // Draws the n (<= kThreads) oldest queued triangles of field pass jb, one
-// per thread, then flushes the kept fragments (Depth64).
+// per thread.
 template <typename R, int MODE>
 __device__ __forceinline__ void drain_ring(FusedLds& S, Frag* s_frag, uint32_t head, uint32_t n,
                                            const TessJob* __restrict__ jobs, uint32_t jb,
                                            const RasterJob<R>* __restrict__ rjobs, const uint32_t* __restrict__ jimg,
                                            FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
                                            uint32_t* __restrict__ id, Frag* __restrict__ frags, uint32_t fcap) {
+  V3<R> P[3];
+  bool drawn = false;
+  uint32_t obj = 0, g = 0;
   if (threadIdx.x < n) {
     const FieldDev f = jobs[jb].f;
     const uint32_t s = (head + threadIdx.x) & (kRing - 1u);
-    const uint32_t g = S.rg[s], cc = S.rc[s], x = S.rx[s];
+    g = S.rg[s];
+    const uint32_t cc = S.rc[s], x = S.re[s];
     const CellLoc L = locate(f, cc);
     if (interior_cell(f, L)) {
-      const uint32_t q = x >> 12;
+      const uint32_t q = x >> 18;
       double thr = jobs[jb].ts.thr[0];
-      uint32_t obj = jobs[jb].ts.obj[0];
+      obj = jobs[jb].ts.obj[0];
 #pragma unroll
       for (uint32_t z = 1; z < kMaxThr; z++) {
         thr = q == z ? jobs[jb].ts.thr[z] : thr;
         obj = q == z ? jobs[jb].ts.obj[z] : obj;
       }
-      const uint64_t ent = c_cslot[(x & 255u) * 12u + ((x >> 8) & 15u)];
-      V3<R> P[3];
-      slot_positions_mem<R>(f, L, thr, ent, P);
-      for (uint32_t u = jimg[jb]; u < jimg[jb + 1]; u++) {
-        const uint32_t k = jimg[u];
-        if (obj < rjobs[k].obj0 || obj >= rjobs[k].obj1) continue;
-        const ViewDev<R> v = rjobs[k].v;
-        draw_triangle<R, MODE>(v, rjobs[k].px0, P, g + 1u, key, id, s_frag, &S.nfrag, kFragLdsFused);
-      }
+      slot_positions_mem<R>(f, L, thr, x & 0x3ffffu, P);
+      drawn = true;
     }
   }
-  __syncthreads();
-  if constexpr (MODE == kRasterDepth64) {
-    if (threadIdx.x == 0) {
-      const uint32_t all = S.nfrag;
-      const uint32_t c = all < kFragLdsFused ? all : kFragLdsFused;
-      const uint32_t b = c ? atomicAdd(&hdr->nfrag, c) : 0u;
-      if (all > kFragLdsFused || (c && uint64_t(b) + c > fcap)) atomicOr(&hdr->overflow, 2u);
-      S.fbase = b;
-      S.fcnt = c;
-      S.nfrag = 0;
-    }
-    __syncthreads();
-    const uint32_t b0 = S.fbase, c = S.fcnt;
-    for (uint32_t e = threadIdx.x; e < c; e += kThreads) {
-      if (uint64_t(b0) + e < fcap) frags[b0 + e] = s_frag[e];
-    }
-    __syncthreads();
+  // the images of this field pass (the same for the whole workgroup)
+  for (uint32_t u = jimg[jb]; u < jimg[jb + 1]; u++) {
+    const uint32_t k = jimg[u];
+    const bool here = drawn && obj >= rjobs[k].obj0 && obj < rjobs[k].obj1;
+    const ViewDev<R> v = rjobs[k].v;
+    if (here)
+      draw_triangle<R, MODE>(v, rjobs[k].px0, P, g + 1u, key, id, s_frag, &S.nfrag, kFragLdsFused, frags, fcap, hdr);
   }
+  flush_frags<MODE>(S, s_frag, hdr, frags, fcap, false);
 }
 
 template <typename R, int MODE>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 4 : 5))) void
+k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
                                                           const uint64_t* __restrict__ start,
                                                           uint32_t* __restrict__ cellout, uint64_t cap,
                                                           const RasterJob<R>* __restrict__ rjobs,
@@ -956,10 +1037,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                                                           FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
                                                           uint32_t* __restrict__ id, Frag* __restrict__ frags,
                                                           uint32_t fcap) {
+  static_assert(MODE != kRasterDepth64, "the fused kernel keeps every fragment (kRasterAppend)");
   __shared__ FusedLds S;
-  __shared__ Frag s_frag[MODE == kRasterDepth64 ? kFragLdsFused : 1];
+  __shared__ Frag s_frag[MODE == kRasterAppend ? kFragLdsFused : 1];
   if (hdr->overflow & 1u) return;
-  if (MODE == kRasterId64 && !(hdr->overflow & 2u)) return;
+  // the z-test passes run only after the fragment list overflowed
+  if (MODE != kRasterAppend && !(hdr->overflow & 2u)) return;
   S.ntri[threadIdx.x] = c_ntri[threadIdx.x];
   if (threadIdx.x == 0) S.nfrag = 0;
   __syncthreads();
@@ -1030,11 +1113,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
           for (int z = 1; z < kMaxThr; z++) bq = q == z ? base[z] : bq;
           const uint64_t g = bq + r;
           const uint32_t cc = c0 + l;
-          if (MODE != kRasterId64 && g < cap) cellout[g] = cc;
+          if (MODE == kRasterAppend && g < cap) cellout[g] = cc;
           const uint32_t s = (head + npend + e) & (kRing - 1u);
           S.rg[s] = uint32_t(g);
           S.rc[s] = cc;
-          S.rx[s] = uint16_t(uint32_t(S.code[q][l]) | sl << 8 | uint32_t(q) << 12);
+          S.re[s] = (uint32_t(c_cslot[uint32_t(S.code[q][l]) * 12u + sl]) & 0x3ffffu) | uint32_t(q) << 18;
         }
         pos += take;
         npend += take;
@@ -1056,6 +1139,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     head = (head + n) & (kRing - 1u);
     npend -= n;
   }
+  flush_frags<MODE>(S, s_frag, hdr, frags, fcap, true);
 }
 
 // the latest triangle among the kept fragments at each pixel's final depth
@@ -1066,6 +1150,17 @@ __global__ void k_frag_ids(const Frag* __restrict__ frags, uint32_t cap, const F
   for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < n; f += gridDim.x * blockDim.x) {
     const Frag fr = frags[f];
     if (fr.key == key[fr.px]) atomicMax(id + fr.px, fr.id);
+  }
+}
+
+// the z-buffer keys of every listed fragment
+__global__ void k_frag_keys(const Frag* __restrict__ frags, uint32_t cap, const FrameHdr* __restrict__ hdr,
+                            uint64_t* __restrict__ key) {
+  if (hdr->overflow & 2u) return;
+  const uint32_t n = min(hdr->nfrag, cap);
+  for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < n; f += gridDim.x * blockDim.x) {
+    const Frag fr = frags[f];
+    atomicMax(reinterpret_cast<unsigned long long*>(key + fr.px), (unsigned long long)fr.key);
   }
 }
 
@@ -1409,6 +1504,12 @@ hipError_t launch_raster(const MeshView<R>& m, const RasterJob<R>* rjobs, int nj
   return hipGetLastError();
 }
 
+hipError_t launch_frag_keys(const Frag* frags, uint32_t cap, const FrameHdr* hdr, uint64_t* key, uint32_t grid,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_frag_keys, dim3(grid), dim3(256), 0, st, frags, cap, hdr, key);
+  return hipGetLastError();
+}
+
 hipError_t launch_frag_ids(const Frag* frags, uint32_t cap, const FrameHdr* hdr, const uint64_t* key, uint32_t* id,
                            uint32_t grid, hipStream_t st) {
   hipLaunchKernelGGL(k_frag_ids, dim3(grid), dim3(256), 0, st, frags, cap, hdr, key, id);
@@ -1450,17 +1551,24 @@ hipError_t launch_tess_raster(const TessJob* jobs, int njobs, uint32_t ntiles, c
                               uint64_t cap, const RasterJob<R>* rjobs, const uint32_t* jimg, FrameHdr* hdr, int mode,
                               uint64_t* key, uint32_t* id, Frag* frags, uint32_t fcap, uint32_t grid, hipStream_t st) {
   if (ntiles == 0) return hipSuccess;
-  if ((sizeof(R) == 4) != (mode == kRasterFp32)) return hipErrorInvalidValue;
   const dim3 g(grid8(ntiles, grid));
-  if (mode == kRasterFp32)
+  if (mode == kRasterAppend) {
+    hipLaunchKernelGGL((k_tess_raster<R, kRasterAppend>), g, dim3(kThreads), 0, st, jobs, njobs, ntiles, start, cell,
+                       cap, rjobs, jimg, hdr, key, id, frags, fcap);
+  } else if constexpr (sizeof(R) == 4) {
+    if (mode != kRasterFp32) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_tess_raster<R, kRasterFp32>), g, dim3(kThreads), 0, st, jobs, njobs, ntiles, start, cell, cap,
                        rjobs, jimg, hdr, key, id, frags, fcap);
-  else if (mode == kRasterDepth64)
-    hipLaunchKernelGGL((k_tess_raster<R, kRasterDepth64>), g, dim3(kThreads), 0, st, jobs, njobs, ntiles, start, cell,
-                       cap, rjobs, jimg, hdr, key, id, frags, fcap);
-  else
-    hipLaunchKernelGGL((k_tess_raster<R, kRasterId64>), g, dim3(kThreads), 0, st, jobs, njobs, ntiles, start, cell,
-                       cap, rjobs, jimg, hdr, key, id, frags, fcap);
+  } else {
+    if (mode == kRasterKey64)
+      hipLaunchKernelGGL((k_tess_raster<R, kRasterKey64>), g, dim3(kThreads), 0, st, jobs, njobs, ntiles, start, cell,
+                         cap, rjobs, jimg, hdr, key, id, frags, fcap);
+    else if (mode == kRasterId64)
+      hipLaunchKernelGGL((k_tess_raster<R, kRasterId64>), g, dim3(kThreads), 0, st, jobs, njobs, ntiles, start, cell,
+                         cap, rjobs, jimg, hdr, key, id, frags, fcap);
+    else
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

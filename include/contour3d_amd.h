@@ -213,8 +213,9 @@ extern int contour3d_amd_mesh_free(contour3d_amd_mesh_t * mesh);
 enum {
   CONTOUR3D_AMD_TUNE_TESS_GRID = 0,     /* max workgroups of count/emit (0 = one per tile) */
   CONTOUR3D_AMD_TUNE_RASTER_GRID = 1,   /* max workgroups of the raster passes (0 = one per 256 triangles) */
-  CONTOUR3D_AMD_TUNE_FRAG_CAPACITY = 2, /* FP64 fragment list entries (grows by itself after an overflow;
-                                           an overflowing frame takes the slower geometry id pass) */
+  CONTOUR3D_AMD_TUNE_FRAG_CAPACITY = 2, /* fragment list entries, 16 B each (0: the default, 2^24; grows by
+                                           itself after an overflow; an overflowing frame takes the slower
+                                           z-test passes over the geometry) */
   CONTOUR3D_AMD_TUNE_GRAPHS = 3         /* 1: contour3d_execute, _execute_rgb and _execute_batch capture
                                            their launch sequence as a HIP graph and replay it while the
                                            frame layout (image sizes, objects' fields and lattices,

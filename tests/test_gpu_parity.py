@@ -292,3 +292,32 @@ def test_errors_return_one():
     g = fields.blobs((12, 12, 12), seed=1)
     api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"],
                     [dict(field=g, threshold=0.5, color=(1, 1, 1))])
+
+
+def test_large_triangles_fill_the_fragment_lists(orc):
+    """A 9^3 field at 320x240: each triangle covers hundreds of pixels, so
+    the workgroups' LDS fragment lists overflow into the frame's list.  FP64
+    stays bit-exact with the oracle, and in both forms the fragment-list
+    path renders the same bits as the z-test passes that run after a list
+    overflow (forced with a one-entry list)."""
+    banner("large triangles: LDS list spill, list vs z-test passes")
+    contours = [dict(field=fields.blobs((9, 9, 9), seed=5), threshold=0.45, color=(200, 120, 40)),
+                dict(field=fields.blobs((9, 9, 9), seed=6), threshold=0.5, color=(40, 120, 200))]
+    v = view_for(320, 240)
+    args = (v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], contours)
+    ref_rgb, ref_dep = orc.execute(*args)
+    assert (ref_dep > -1e300).sum() > 5000
+    out = {}
+    try:
+        for cap in (0, 1):
+            api.tune(api.TUNE_FRAG_CAPACITY, cap)
+            for prec in (api.FP64, api.FP32):
+                out[cap, prec] = api.execute_rgb(*args, precision=prec, want_depth=True)
+    finally:
+        api.tune(api.TUNE_FRAG_CAPACITY, 0)
+    rgb, dep = out[0, api.FP64]
+    assert same_bits(dep, ref_dep)
+    assert np.array_equal(rgb, ref_rgb)
+    for prec in (api.FP64, api.FP32):
+        assert np.array_equal(out[0, prec][0], out[1, prec][0]), prec
+        assert same_bits(out[0, prec][1], out[1, prec][1]), prec
