@@ -20,7 +20,11 @@ Beside it, in the same JSON line:
               steps), 1-4 thresholds, images 512^2..2048^2, random cameras;
               p50/p99 latency and frames/s of the serving loop, next to a
               plain loop that renders the same arrivals one at a time
-  roofline    HBM roofline of the dominant kernel of the FP64 frame
+  roofline    HBM roofline of the dominant kernel of the FP64 frame: the
+              fused tessellation + raster stage, charged with the bytes of
+              the stages it replaces (B_tess + B_raster below)
+  tessellation  the stored-mesh tessellation (count + scan + emit of
+              contour3d_amd_mesh_build) on the same field, against B_tess
   cpu_baseline  the CPU oracle timed on a bounded slab of the same field
 """
 import argparse
@@ -35,11 +39,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch past L2 of the FP64 frame's kernels at 512^3
-# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a plain 1 GiB copy
-# reads), measured with rocprofv3 --pmc on this same frame; the commands and
-# raw counts are in profiles/turbulence_512cube_3thr_1920x1080_kernels.md
-MEASURED_TRAFFIC_512 = {"emit": 19.19e9, "raster": 11.61e9, "count": 1.37e9, "resolve": 6.51e9}
+# HBM bytes per launch past L2 of the FP64 frame's stages at 512^3 (rocprofv3
+# --pmc FETCH_SIZE / WRITE_SIZE, scaled by a plain copy's readings); filled
+# in from the profile summary under profiles/
+MEASURED_TRAFFIC_512 = {}
 N = 512
 THRESHOLDS = (-1.0, 0.0, 1.0)
 COLORS = ((255, 80, 40), (40, 220, 90), (60, 120, 255))
@@ -50,10 +53,12 @@ def algorithmic_bytes(stage, T, npts, npx, fp64):
     """Irreducible HBM bytes per frame of each stage.
 
     count    the field, read once for all thresholds (8 B per lattice point)
-    emit     the field once + per triangle 3 positions and the weighted face
-             normal (96 B in FP64, 48 B in FP32): the tessellation bytes
-    raster   per triangle its 3 positions, plus the z-buffer keys written
-             once and read once per pixel
+    emit     B_tess: the field once + per triangle 3 positions and the
+             weighted face normal (96 B in FP64, 48 B in FP32)
+    fused    the frame's tessellation + raster stage, charged with the work
+             of the stages it replaces: B_tess + B_raster, where B_raster is
+             72 B per triangle (positions and vertex normals, FP64) plus
+             19 B per pixel (key init, key read, RGB)
     resolve  the keys read and the RGB bytes written per pixel (the vertex
              normals of the nearest triangles are gathered here and add
              O(pixels) bytes, not counted)
@@ -63,11 +68,24 @@ def algorithmic_bytes(stage, T, npts, npx, fp64):
         return 8 * npts
     if stage == "emit":
         return 8 * npts + 12 * r * T
-    if stage == "raster":
-        return 9 * r * T + 16 * npx
+    if stage == "fused":
+        return 8 * npts + 12 * r * T + 9 * r * T + 19 * npx
     if stage == "resolve":
         return 8 * npx + 3 * npx
     return 0
+
+
+def mesh_tessellation(api, contours, precision, reps):
+    """Stage times of contour3d_amd_mesh_build (the stored mesh) on the
+    frame's objects: one warm-up build, then `reps` timed ones."""
+    api.Mesh(contours, precision).free()
+    api.profile_reset()
+    api.profile_enable(True)
+    for _ in range(reps):
+        api.Mesh(contours, precision).free()
+    api.profile_enable(False)
+    ms, calls, ntri = api.profile_read()
+    return {k: ms[k] / reps for k in ("count", "scan", "emit", "smooth")}, ntri / reps
 
 
 def run_form(api, contours, view, precision, steps, warmup, rgb_dev):
@@ -482,19 +500,26 @@ def main():
         T = ntri / max(1, args.steps)
         stages = {}
         for s in api.STAGES:
-            if s == "smooth":
-                continue  # vertex normals are gathered inside the resolve pass
+            if s in ("smooth", "emit"):
+                continue  # frames store no mesh: the fused raster stage tessellates
             per = ms[s] / max(1, args.steps)
-            b = algorithmic_bytes(s, T, npts, npx, prec == api.FP64)
+            b = algorithmic_bytes("fused" if s == "raster" else s, T, npts, npx, prec == api.FP64)
             stages[s] = dict(ms=round(per, 4), gbs=round(b / (per * 1e-3) / 1e9, 1) if per > 0 and b else None)
         results[name] = dict(dt=dt, T=T, stages=stages, frame_ms=dt / args.steps * 1e3)
 
     r = results["fp64"]
-    dom = max(("count", "emit", "raster", "resolve"), key=lambda s: r["stages"][s]["ms"])
-    dom_b = algorithmic_bytes(dom, r["T"], npts, npx, True)
+    dom = max(("count", "raster", "resolve"), key=lambda s: r["stages"][s]["ms"])
+    dom_b = algorithmic_bytes("fused" if dom == "raster" else dom, r["T"], npts, npx, True)
     achieved = dom_b / (r["stages"][dom]["ms"] * 1e-3) / 1e9
-    tess_b = algorithmic_bytes("emit", r["T"], npts, npx, True)
-    tess_ms = r["stages"]["count"]["ms"] + r["stages"]["scan"]["ms"] + r["stages"]["emit"]["ms"]
+    log("stored-mesh tessellation (mesh build), FP64 and FP32")
+    tess = {}
+    for name, prec in (("fp64", api.FP64), ("fp32", api.FP32)):
+        st, tri = mesh_tessellation(api, contours, prec, 3)
+        t_ms = st["count"] + st["scan"] + st["emit"]
+        b = algorithmic_bytes("emit", tri, npts, npx, prec == api.FP64)
+        tess[name] = {"ms": round(t_ms, 4), "count_ms": round(st["count"], 4), "emit_ms": round(st["emit"], 4),
+                      "smooth_ms": round(st["smooth"], 4), "gcells_per_s": round(cells / (t_ms * 1e-3) / 1e9, 2),
+                      "algorithmic_bytes": round(b), "hbm_frac": round(b / (t_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
     out = {
         "metric": "Gcells/s + frames/s at 512^3 fp64; served p50/p99 latency vs plain GPU path",
         "value": round(cells * args.steps / r["dt"] / 1e9, 3),
@@ -513,13 +538,12 @@ def main():
                                f"thresholds {{-1,0,1}}, {W}x{H} image in HBM",
                    "lattice": [n, n, n], "thresholds": list(THRESHOLDS), "image": [W, H],
                    "triangles_per_frame": int(r["T"])},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+        "roofline": {"bound": "hbm", "kernel": "k_tess_raster (fused tessellation + raster stage)" if dom == "raster"
+                     else dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": MEASURED_TRAFFIC_512.get(dom) if n == 512 else None,
                      "algorithmic_bytes": round(dom_b)},
-        "tessellation": {"ms": round(tess_ms, 4),
-                         "gcells_per_s": round(cells / (tess_ms * 1e-3) / 1e9, 2),
-                         "hbm_frac": round(tess_b / (tess_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
+        "tessellation": tess,
         "stages_fp64": r["stages"],
         "fp32": {"value": round(cells * args.steps / results["fp32"]["dt"] / 1e9, 3), "unit": "Gcells/s",
                  "ms_per_step": round(results["fp32"]["frame_ms"], 3),

@@ -383,161 +383,6 @@ __device__ __forceinline__ bool interior_cell(const FieldDev& f, const CellLoc& 
 }
 
 // ----------------------------------------------------------------------
-// tessellation: emit pass
-//
-// Each tile of 1024 consecutive cells is processed in chunks of 256: every
-// thread classifies one cell against all thresholds, a block scan of the
-// packed counts places the chunk's triangles (threshold-major, then cell,
-// then slot order), and the whole block then computes the chunk's triangles
-// one per thread -- finding its cell by binary search over the chunk's
-// prefix -- so that the stores of consecutive triangles coalesce.
-template <typename R>
-__global__ __launch_bounds__(kThreads) void k_emit(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
-                                                   const uint64_t* __restrict__ start, MeshView<R> m,
-                                                   FrameHdr* __restrict__ hdr) {
-  __shared__ double s_val[8][kThreads];
-  __shared__ double s_grid[6][kThreads];  // x0 x1 y0 y1 z0 z1 of each cell (identity map)
-  __shared__ uint8_t s_code[kMaxThr][kThreads];
-  __shared__ uint16_t s_pref[kMaxThr][kThreads];  // exclusive prefix of the cells' counts
-  __shared__ uint64_t s_wave[kThreads / 64];
-  __shared__ uint8_t s_ntri[256];
-  s_ntri[threadIdx.x] = c_ntri[threadIdx.x];
-  const uint64_t cap = m.cap;
-  __syncthreads();
-  const XcdRange XR = xcd_range(ntiles);
-  for (uint32_t gt = XR.begin; gt < XR.end; gt += XR.step) {
-    const uint32_t jb = find_tjob(jobs, njobs, gt);
-    const FieldDev f = jobs[jb].f;
-    const ThrSet ts = jobs[jb].ts;
-    const uint32_t tile = gt - jobs[jb].tile0;
-    uint64_t base[kMaxThr];
-    double thrs[kMaxThr];
-#pragma unroll
-    for (int q = 0; q < kMaxThr; q++) {
-      base[q] = q < ts.n ? start[ts.boff[q] + tile] : 0;
-      thrs[q] = ts.thr[q];
-    }
-    for (int chunk = 0; chunk < kCellsPerBlock / kThreads; chunk++) {
-      const uint32_t c0 = tile * kCellsPerBlock + chunk * kThreads;
-      if (c0 >= f.ncells) break;
-      const uint32_t c = c0 + threadIdx.x;
-      const bool valid = c < f.ncells;
-      double v[8];
-      if (valid) {
-        const CellLoc L = locate(f, c);
-        load_corners(f, L, v);
-        if (!f.cx) {
-          s_grid[0][threadIdx.x] = gptr(f.gx)[L.i];
-          s_grid[1][threadIdx.x] = gptr(f.gx)[L.i + 1];
-          s_grid[2][threadIdx.x] = gptr(f.gy)[L.j];
-          s_grid[3][threadIdx.x] = gptr(f.gy)[L.j + 1];
-          s_grid[4][threadIdx.x] = gptr(f.gz)[L.k];
-          s_grid[5][threadIdx.x] = gptr(f.gz)[L.k + 1];
-        }
-      } else {
-#pragma unroll
-        for (int q = 0; q < 8; q++) v[q] = 0.0;
-      }
-#pragma unroll
-      for (int q = 0; q < 8; q++) s_val[q][threadIdx.x] = v[q];
-      uint32_t code[kMaxThr];
-      const uint64_t packed = valid ? packed_counts(ts, v, code, s_ntri) : 0ull;
-      uint64_t tot;
-      const uint64_t ex = block_excl_scan64(packed, s_wave, tot);
-      uint32_t qoff[kMaxThr + 1];
-      qoff[0] = 0;
-#pragma unroll
-      for (int q = 0; q < kMaxThr; q++) {
-        qoff[q + 1] = qoff[q] + (uint32_t(tot >> (16 * q)) & 0xffffu);
-        s_pref[q][threadIdx.x] = uint16_t(ex >> (16 * q));
-        if (valid) s_code[q][threadIdx.x] = uint8_t(code[q]);
-      }
-      __syncthreads();
-      const uint32_t total = qoff[kMaxThr];
-      for (uint32_t it = threadIdx.x; it < total; it += kThreads) {
-        // threshold, then the last cell whose first triangle is at or before
-        // the item (cells without triangles share their successor's prefix)
-        int q = 0;
-#pragma unroll
-        for (int z = 1; z < kMaxThr; z++) q += it >= qoff[z] ? 1 : 0;
-        uint32_t qsel = 0;
-#pragma unroll
-        for (int z = 1; z < kMaxThr; z++) qsel = q == z ? qoff[z] : qsel;
-        const uint32_t r = it - qsel;
-        uint32_t l = 0;
-#pragma unroll
-        for (uint32_t step = kThreads / 2; step; step >>= 1) l += s_pref[q][l + step] <= r ? step : 0u;
-        const uint32_t sl = r - s_pref[q][l];
-        double thr = thrs[0];
-#pragma unroll
-        for (int z = 1; z < kMaxThr; z++) thr = q == z ? thrs[z] : thr;
-        const uint32_t cc = c0 + l;
-        const CellLoc L = locate(f, cc);
-        const uint64_t ent = c_cslot[uint32_t(s_code[q][l]) * 12u + sl];
-        V3<R> P[3];
-#pragma unroll
-        for (int vv = 0; vv < 3; vv++) {
-          const uint32_t tail = uint32_t(ent >> (6 * vv)) & 7u;
-          const uint32_t head = uint32_t(ent >> (6 * vv + 3)) & 7u;
-          V3<R> pt, ph;
-          if (f.cx) {
-            pt = corner_pos<R>(f, L, tail);
-            ph = corner_pos<R>(f, L, head);
-          } else {
-            pt = {R(s_grid[tail & 1u][l]), R(s_grid[2 + ((tail >> 1) & 1u)][l]), R(s_grid[4 + (tail >> 2)][l])};
-            ph = {R(s_grid[head & 1u][l]), R(s_grid[2 + ((head >> 1) & 1u)][l]), R(s_grid[4 + (head >> 2)][l])};
-          }
-          P[vv] = interp<R>(thr, pt, s_val[tail][l], ph, s_val[head][l]);
-        }
-        const V3<R> w = smoothing_weight<R>(P);
-        const bool in = interior_cell(f, L);
-        uint64_t bq = base[0];
-#pragma unroll
-        for (int z = 1; z < kMaxThr; z++) bq = q == z ? base[z] : bq;
-        const uint64_t g = bq + r;
-        if (g >= cap) {
-          hdr->overflow = 1u;  // capacity reserved ahead of a replayed frame was too small
-          continue;
-        }
-        // the FP64 mesh (104 B per triangle) streams past the caches:
-        // non-temporal stores measured 10-15% faster there, slower in FP32
-        R* p = m.pos + g;
-        R* wp = m.wfn + g;
-        const uint32_t ed = uint32_t(ent >> 18) | uint32_t(in) << 15;
-        if constexpr (sizeof(R) == 8) {
-#pragma unroll
-          for (int vv = 0; vv < 3; vv++) {
-            __builtin_nontemporal_store(P[vv].x, p + (3 * vv + 0) * cap);
-            __builtin_nontemporal_store(P[vv].y, p + (3 * vv + 1) * cap);
-            __builtin_nontemporal_store(P[vv].z, p + (3 * vv + 2) * cap);
-          }
-          __builtin_nontemporal_store(w.x, wp);
-          __builtin_nontemporal_store(w.y, wp + cap);
-          __builtin_nontemporal_store(w.z, wp + 2 * cap);
-          __builtin_nontemporal_store(ed, m.edges + g);
-          __builtin_nontemporal_store(cc, m.cell + g);
-        } else {
-#pragma unroll
-          for (int vv = 0; vv < 3; vv++) {
-            p[(3 * vv + 0) * cap] = P[vv].x;
-            p[(3 * vv + 1) * cap] = P[vv].y;
-            p[(3 * vv + 2) * cap] = P[vv].z;
-          }
-          wp[0] = w.x;
-          wp[cap] = w.y;
-          wp[2 * cap] = w.z;
-          m.edges[g] = ed;
-          m.cell[g] = cc;
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < kMaxThr; q++) base[q] += qoff[q + 1] - qoff[q];
-      __syncthreads();
-    }
-  }
-}
-
-// ----------------------------------------------------------------------
 // vertex-normal smoothing
 
 __constant__ NbrTable c_nbr;
@@ -918,7 +763,8 @@ struct FusedLds {
   uint8_t ntri[256];
   uint32_t rg[kRing];  // serial position of the queued triangle
   uint32_t rc[kRing];  // its cell
-  uint32_t re[kRing];  // its vertices' (tail, head) corners (slot entry bits 0-17) | threshold << 18
+  uint32_t re[kRing];  // its vertices' (tail, head) corners (slot entry bits 0-17) | threshold << 18 |
+                       // slot << 20 | corner code << 24
   uint32_t nfrag, fbase, fcnt;
 };
 
@@ -1004,7 +850,7 @@ __device__ __forceinline__ void drain_ring(FusedLds& S, Frag* s_frag, uint32_t h
     const uint32_t cc = S.rc[s], x = S.re[s];
     const CellLoc L = locate(f, cc);
     if (interior_cell(f, L)) {
-      const uint32_t q = x >> 18;
+      const uint32_t q = (x >> 18) & 3u;
       double thr = jobs[jb].ts.thr[0];
       obj = jobs[jb].ts.obj[0];
 #pragma unroll
@@ -1027,25 +873,16 @@ __device__ __forceinline__ void drain_ring(FusedLds& S, Frag* s_frag, uint32_t h
   flush_frags<MODE>(S, s_frag, hdr, frags, fcap, false);
 }
 
-template <typename R, int MODE>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 4 : 5))) void
-k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
-                                                          const uint64_t* __restrict__ start,
-                                                          uint32_t* __restrict__ cellout, uint64_t cap,
-                                                          const RasterJob<R>* __restrict__ rjobs,
-                                                          const uint32_t* __restrict__ jimg,
-                                                          FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
-                                                          uint32_t* __restrict__ id, Frag* __restrict__ frags,
-                                                          uint32_t fcap) {
-  static_assert(MODE != kRasterDepth64, "the fused kernel keeps every fragment (kRasterAppend)");
-  __shared__ FusedLds S;
-  __shared__ Frag s_frag[MODE == kRasterAppend ? kFragLdsFused : 1];
-  if (hdr->overflow & 1u) return;
-  // the z-test passes run only after the fragment list overflowed
-  if (MODE != kRasterAppend && !(hdr->overflow & 2u)) return;
-  S.ntri[threadIdx.x] = c_ntri[threadIdx.x];
-  if (threadIdx.x == 0) S.nfrag = 0;
-  __syncthreads();
+// The tile loop shared by the queue kernels: classifies each tile's cells
+// in chunks of 256, queues the chunk's triangles in the LDS ring in serial
+// order and calls drain(head, n, jb) for every full 256 items (and for the
+// rest when the field pass changes and at the end).  With kStoreCell the
+// cell index of each triangle is stored at its serial position as it is
+// queued.  Expects S.ntri loaded and a barrier since.
+template <bool kStoreCell, typename Drain>
+__device__ __forceinline__ void ring_tiles(FusedLds& S, const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
+                                           const uint64_t* __restrict__ start, uint32_t* __restrict__ cellout,
+                                           uint64_t cap, Drain&& drain) {
   // ring state, the same in every thread: oldest item, items queued, and
   // the field pass they belong to
   uint32_t head = 0, npend = 0, cur = 0;
@@ -1055,7 +892,7 @@ k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
     // the queue holds one field pass at a time
     while (npend && jb != cur) {
       const uint32_t n = npend < kThreads ? npend : kThreads;
-      drain_ring<R, MODE>(S, s_frag, head, n, jobs, cur, rjobs, jimg, hdr, key, id, frags, fcap);
+      drain(head, n, cur);
       head = (head + n) & (kRing - 1u);
       npend -= n;
     }
@@ -1113,17 +950,18 @@ k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
           for (int z = 1; z < kMaxThr; z++) bq = q == z ? base[z] : bq;
           const uint64_t g = bq + r;
           const uint32_t cc = c0 + l;
-          if (MODE == kRasterAppend && g < cap) cellout[g] = cc;
+          if (kStoreCell && g < cap) cellout[g] = cc;
+          const uint32_t cd = S.code[q][l];
           const uint32_t s = (head + npend + e) & (kRing - 1u);
           S.rg[s] = uint32_t(g);
           S.rc[s] = cc;
-          S.re[s] = (uint32_t(c_cslot[uint32_t(S.code[q][l]) * 12u + sl]) & 0x3ffffu) | uint32_t(q) << 18;
+          S.re[s] = (uint32_t(c_cslot[cd * 12u + sl]) & 0x3ffffu) | uint32_t(q) << 18 | sl << 20 | cd << 24;
         }
         pos += take;
         npend += take;
         __syncthreads();
         while (npend >= kThreads) {
-          drain_ring<R, MODE>(S, s_frag, head, kThreads, jobs, jb, rjobs, jimg, hdr, key, id, frags, fcap);
+          drain(head, kThreads, jb);
           head = (head + kThreads) & (kRing - 1u);
           npend -= kThreads;
         }
@@ -1135,11 +973,108 @@ k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
   }
   while (npend) {
     const uint32_t n = npend < kThreads ? npend : kThreads;
-    drain_ring<R, MODE>(S, s_frag, head, n, jobs, cur, rjobs, jimg, hdr, key, id, frags, fcap);
+    drain(head, n, cur);
     head = (head + n) & (kRing - 1u);
     npend -= n;
   }
+}
+
+template <typename R, int MODE>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 4 : 5))) void
+k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles, const uint64_t* __restrict__ start,
+              uint32_t* __restrict__ cellout, uint64_t cap, const RasterJob<R>* __restrict__ rjobs,
+              const uint32_t* __restrict__ jimg, FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
+              uint32_t* __restrict__ id, Frag* __restrict__ frags, uint32_t fcap) {
+  static_assert(MODE != kRasterDepth64, "the fused kernel keeps every fragment (kRasterAppend)");
+  __shared__ FusedLds S;
+  __shared__ Frag s_frag[MODE == kRasterAppend ? kFragLdsFused : 1];
+  if (hdr->overflow & 1u) return;
+  // the z-test passes run only after the fragment list overflowed
+  if (MODE != kRasterAppend && !(hdr->overflow & 2u)) return;
+  S.ntri[threadIdx.x] = c_ntri[threadIdx.x];
+  if (threadIdx.x == 0) S.nfrag = 0;
+  __syncthreads();
+  ring_tiles<MODE == kRasterAppend>(S, jobs, njobs, ntiles, start, cellout, cap,
+                                    [&](uint32_t head, uint32_t n, uint32_t jb) {
+                                      drain_ring<R, MODE>(S, s_frag, head, n, jobs, jb, rjobs, jimg, hdr, key, id,
+                                                          frags, fcap);
+                                    });
   flush_frags<MODE>(S, s_frag, hdr, frags, fcap, true);
+}
+
+// ----------------------------------------------------------------------
+// tessellation: emit pass (the stored mesh)
+//
+// The tile loop of the queue kernels; every drain computes 256 triangles,
+// one per thread, and writes them at their serial positions: positions,
+// smoothing weight, lattice edge ids with the interior flag, cell.  The
+// 256 consecutive positions of a drain make every output stream's stores
+// contiguous.
+template <typename R>
+__device__ __forceinline__ void emit_ring(FusedLds& S, uint32_t head, uint32_t n, const TessJob* __restrict__ jobs,
+                                          uint32_t jb, const MeshView<R>& m, FrameHdr* __restrict__ hdr) {
+  if (threadIdx.x < n) {
+    const FieldDev f = jobs[jb].f;
+    const uint32_t s = (head + threadIdx.x) & (kRing - 1u);
+    const uint64_t g = S.rg[s];
+    const uint32_t cc = S.rc[s], x = S.re[s];
+    const CellLoc L = locate(f, cc);
+    const uint32_t q = (x >> 18) & 3u;
+    double thr = jobs[jb].ts.thr[0];
+#pragma unroll
+    for (uint32_t z = 1; z < kMaxThr; z++) thr = q == z ? jobs[jb].ts.thr[z] : thr;
+    V3<R> P[3];
+    slot_positions_mem<R>(f, L, thr, x & 0x3ffffu, P);
+    const V3<R> w = smoothing_weight<R>(P);
+    const uint32_t ent_edges = uint32_t(c_cslot[(x >> 24) * 12u + ((x >> 20) & 15u)] >> 18);
+    const uint32_t ed = ent_edges | uint32_t(interior_cell(f, L)) << 15;
+    const uint64_t cap = m.cap;
+    if (g >= cap) {
+      hdr->overflow = 1u;  // capacity reserved ahead of a replayed frame was too small
+    } else {
+      // the FP64 mesh (104 B per triangle) streams past the caches:
+      // non-temporal stores measured faster there, slower in FP32
+      R* p = m.pos + g;
+      R* wp = m.wfn + g;
+      if constexpr (sizeof(R) == 8) {
+#pragma unroll
+        for (int vv = 0; vv < 3; vv++) {
+          __builtin_nontemporal_store(P[vv].x, p + (3 * vv + 0) * cap);
+          __builtin_nontemporal_store(P[vv].y, p + (3 * vv + 1) * cap);
+          __builtin_nontemporal_store(P[vv].z, p + (3 * vv + 2) * cap);
+        }
+        __builtin_nontemporal_store(w.x, wp);
+        __builtin_nontemporal_store(w.y, wp + cap);
+        __builtin_nontemporal_store(w.z, wp + 2 * cap);
+        __builtin_nontemporal_store(ed, m.edges + g);
+        __builtin_nontemporal_store(cc, m.cell + g);
+      } else {
+#pragma unroll
+        for (int vv = 0; vv < 3; vv++) {
+          p[(3 * vv + 0) * cap] = P[vv].x;
+          p[(3 * vv + 1) * cap] = P[vv].y;
+          p[(3 * vv + 2) * cap] = P[vv].z;
+        }
+        wp[0] = w.x;
+        wp[cap] = w.y;
+        wp[2 * cap] = w.z;
+        m.edges[g] = ed;
+        m.cell[g] = cc;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_emit(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
+                                                   const uint64_t* __restrict__ start, MeshView<R> m,
+                                                   FrameHdr* __restrict__ hdr) {
+  __shared__ FusedLds S;
+  S.ntri[threadIdx.x] = c_ntri[threadIdx.x];
+  __syncthreads();
+  ring_tiles<false>(S, jobs, njobs, ntiles, start, nullptr, 0,
+                    [&](uint32_t head, uint32_t n, uint32_t jb) { emit_ring<R>(S, head, n, jobs, jb, m, hdr); });
 }
 
 // the latest triangle among the kept fragments at each pixel's final depth
