@@ -1171,13 +1171,14 @@ int fused_launch(Lib& L, Frame& F, const contour3d_amd_mesh& M, const RasterJob<
   Frag* frags = F.frags.as<Frag>();
   if (prof_begin(F, CONTOUR3D_AMD_STAGE_RASTER)) return 1;
   C3D_HIP(launch_zinit(key, id, npx, fp64 ? 0 : 1, hdr, F.st));
-  // every covered pixel into the fragment list, then its keys; after an
-  // overflow of the list (which grows for the next frame) the z-test passes
-  // over the geometry run instead (each kernel decides on the device)
-  C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterAppend, key, id,
-                                frags, F.frag_cap, grid, F.st));
-  C3D_HIP(launch_frag_keys(frags, F.frag_cap, hdr, key, fgrid, F.st));
   if constexpr (sizeof(R) == 8) {
+    // every covered pixel into the fragment list, then its keys; after an
+    // overflow of the list (which grows for the next frame) the z-test
+    // passes over the geometry run instead (each kernel decides on the
+    // device)
+    C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterAppend, key, id,
+                                  frags, F.frag_cap, grid, F.st));
+    C3D_HIP(launch_frag_keys(frags, F.frag_cap, hdr, key, fgrid, F.st));
     C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterKey64, key, id,
                                   nullptr, 0, grid, F.st));
     C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 0, key, id, F.st));
@@ -1186,6 +1187,7 @@ int fused_launch(Lib& L, Frame& F, const contour3d_amd_mesh& M, const RasterJob<
                                   nullptr, 0, grid, F.st));
     C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 1, 0, key, id, F.st));
   } else {
+    // one pass: the packed key carries the id
     C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterFp32, key, id,
                                   nullptr, 0, grid, F.st));
     C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 1, key, id, F.st));

@@ -989,12 +989,14 @@ k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles, cons
   __shared__ FusedLds S;
   __shared__ Frag s_frag[MODE == kRasterAppend ? kFragLdsFused : 1];
   if (hdr->overflow & 1u) return;
-  // the z-test passes run only after the fragment list overflowed
-  if (MODE != kRasterAppend && !(hdr->overflow & 2u)) return;
+  // the FP64 z-test passes run only after the fragment list overflowed
+  if ((MODE == kRasterKey64 || MODE == kRasterId64) && !(hdr->overflow & 2u)) return;
   S.ntri[threadIdx.x] = c_ntri[threadIdx.x];
   if (threadIdx.x == 0) S.nfrag = 0;
   __syncthreads();
-  ring_tiles<MODE == kRasterAppend>(S, jobs, njobs, ntiles, start, cellout, cap,
+  // the first pass of a frame (Append in FP64, the packed-key z-test in
+  // FP32) stores the cell indices
+  ring_tiles<MODE == kRasterAppend || MODE == kRasterFp32>(S, jobs, njobs, ntiles, start, cellout, cap,
                                     [&](uint32_t head, uint32_t n, uint32_t jb) {
                                       drain_ring<R, MODE>(S, s_frag, head, n, jobs, jb, rjobs, jimg, hdr, key, id,
                                                           frags, fcap);
