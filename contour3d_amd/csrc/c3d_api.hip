@@ -1456,8 +1456,10 @@ double obj_cells(const contour3d_contour_obj_t& c) {
   return n;
 }
 
-// device bytes a request adds to a frame: triangles, z-buffer and image
-// buffers, FP64 fragment list share, and host fields to upload
+// device bytes a request adds to a frame: the cell index of each triangle
+// (a frame stores no triangles), z-buffer and image buffers, an FP64
+// fragment-list share (four covered pixels per pixel), and host fields to
+// upload
 uint64_t estimate_bytes(const Server& S, const contour3d_amd_image_t& im, int precision) {
   double tri = 0, upload = 0;
   for (size_t o = 0; o < im.num_contours; o++) {
@@ -1465,10 +1467,9 @@ uint64_t estimate_bytes(const Server& S, const contour3d_amd_image_t& im, int pr
     tri += obj_cells(c) * S.tri_per_cell;
     if (!is_device_ptr(c.array)) upload += 8.0 * double(c.glsizes[0]) * double(c.glsizes[1]) * double(c.glsizes[2]);
   }
-  const double per_tri = precision == CONTOUR3D_AMD_FP64 ? 9 * 8 + 3 * 8 + 8 : 9 * 4 + 3 * 4 + 8;
   const double npx = double(im.screen_sizes[0]) * double(im.screen_sizes[1]);
-  const double per_px = 8 + (precision == CONTOUR3D_AMD_FP64 ? 4 + 16 : 0) + 3 + (im.depth ? 8 : 0);
-  return uint64_t(1.25 * tri * per_tri + upload + npx * per_px) + (uint64_t(1) << 20);
+  const double per_px = 8 + (precision == CONTOUR3D_AMD_FP64 ? 4 + 4 * 16 : 0) + 3 + (im.depth ? 8 : 0);
+  return uint64_t(1.25 * tri * sizeof(uint32_t) + upload + npx * per_px) + (uint64_t(1) << 20);
 }
 
 void complete_locked(Server& S, Flight& f, int rc, double tri_total);
