@@ -166,6 +166,9 @@ __device__ __forceinline__ void load_corners(const FieldDev& f, const CellLoc& L
 //                           cell-local edge ids (triangulate.c:23-32)
 __constant__ uint8_t c_ntri[256];
 __constant__ uint64_t c_cslot[256 * 12];
+//   c_emask[code * 19 + e]  the slots (bit s) of such a cell whose triangle
+//                           has a vertex on cell-local edge e
+__constant__ uint16_t c_emask[256 * 19];
 
 __device__ __forceinline__ uint32_t corner_code(double thr, const double v[8]) {
   uint32_t code = 0;
@@ -593,11 +596,75 @@ __device__ __forceinline__ void append_frag(const Frag& fr, Frag* s_frag, uint32
   }
 }
 
-// image: projection (project.c), the reference's box and barycentric test,
-// harmonic depth (render.c:60-124), and the z-buffer update of `MODE`.
-// Depth64 and Append put fragments in the workgroup list s_frag (s_n
-// entries, `cap_lds` slots); Append sends those that do not fit straight to
-// the frame's list (gfrags, gcap entries, counted in hdr->nfrag).
+// Projection (project.c), the reference's box and area (render.c:60-89)
+// and the narrowed box of one triangle; returns its candidate pixels (0
+// when a vertex fails to project or, in FP32, the projection has no area:
+// such a triangle's weights are all +-0 and its depth 1 / +-0).
+template <typename R>
+__device__ __forceinline__ uint32_t setup_triangle(const ViewDev<R>& v, const V3<R> wpos[3], Projected<R>& P) {
+  bool vis = true;
+#pragma unroll
+  for (int n = 0; n < 3; n++) vis = vis && project(v, wpos[n], P.q[n]);
+  if (!vis) return 0;
+  const R area = raster_box(v, P);
+  if (sizeof(R) == 4 && area == R(0)) return 0;
+  // about 12 reference-box candidates of a pixel-sized triangle narrow to 3
+  tight_box(v, area, P);
+  if (P.i1 < P.i0 || P.j1 < P.j0) return 0;
+  // a box lies inside one image: fewer than 2^31 pixels
+  return uint32_t(P.i1 - P.i0 + 1) * uint32_t(P.j1 - P.j0 + 1);
+}
+
+// One candidate pixel (i, j) of triangle P: the reference's barycentric
+// test and harmonic depth (render.c:91-124), then the z-buffer update of
+// `MODE`.  Depth64 and Append put fragments in the workgroup list s_frag
+// (s_n entries, `cap_lds` slots); Append sends those that do not fit
+// straight to the frame's list (gfrags, gcap entries, counted in
+// hdr->nfrag).
+template <typename R, int MODE>
+__device__ __forceinline__ void pixel_update(const ViewDev<R>& v, uint64_t row0, const Projected<R>& P, int32_t i,
+                                             int32_t j, uint32_t tid1, uint64_t* __restrict__ key,
+                                             uint32_t* __restrict__ id, Frag* s_frag, uint32_t* s_n,
+                                             uint32_t cap_lds, Frag* __restrict__ gfrags, uint32_t gcap,
+                                             FrameHdr* __restrict__ hdr) {
+  R w0, w1, w2;
+  if (!pixel_weights(P, pixel_centre<R>(i, v.fw), pixel_centre<R>(j, v.fh), w0, w1, w2)) return;
+  const R dep = pixel_depth(P, w0, w1, w2);
+  const uint64_t px = row0 + uint64_t(j) * uint64_t(v.W) + uint64_t(i);
+  if constexpr (MODE == kRasterFp32) {
+    // keys only grow: a key at or below a value read from the buffer can
+    // never change it and skips the atomic
+    const uint64_t k = uint64_t(ord32(float(dep))) << 32 | uint64_t(tid1);
+    if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+  } else if constexpr (MODE == kRasterAppend) {
+    Frag fr;
+    fr.px = uint32_t(px);
+    fr.id = tid1;
+    fr.key = sizeof(R) == 8 ? ord64(double(dep)) : (uint64_t(ord32(float(dep))) << 32 | uint64_t(tid1));
+    append_frag(fr, s_frag, s_n, cap_lds, gfrags, gcap, hdr);
+  } else if constexpr (MODE == kRasterKey64) {
+    const uint64_t k = ord64(double(dep));
+    if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+  } else if constexpr (MODE == kRasterDepth64) {
+    const uint64_t k = ord64(double(dep));
+    if (k < key[px]) return;
+    atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
+    // every fragment whose key equals the final value is kept: its key was
+    // at least the value read before its own atomic
+    const uint32_t o = atomicAdd(s_n, 1u);
+    if (o < cap_lds) {
+      Frag fr;
+      fr.px = uint32_t(px);
+      fr.id = tid1;
+      fr.key = k;
+      s_frag[o] = fr;
+    }
+  } else {
+    if (ord64(double(dep)) == key[px]) atomicMax(id + px, tid1);
+  }
+}
+
+// one triangle drawn by its own lane: every candidate of its box in turn
 template <typename R, int MODE>
 __device__ __forceinline__ void draw_triangle(const ViewDev<R>& v, uint64_t row0, const V3<R> wpos[3], uint32_t tid1,
                                               uint64_t* __restrict__ key, uint32_t* __restrict__ id,
@@ -605,57 +672,10 @@ __device__ __forceinline__ void draw_triangle(const ViewDev<R>& v, uint64_t row0
                                               Frag* __restrict__ gfrags = nullptr, uint32_t gcap = 0,
                                               FrameHdr* __restrict__ hdr = nullptr) {
   Projected<R> P;
-  bool vis = true;
-#pragma unroll
-  for (int n = 0; n < 3; n++) vis = vis && project(v, wpos[n], P.q[n]);
-  if (!vis) return;
-  const R area = raster_box(v, P);
-  // the FP32 form skips triangles whose projection has no area (their
-  // weights are all +-0 and their depth 1 / +-0)
-  if (sizeof(R) == 4 && area == R(0)) return;
-  // about 12 reference-box candidates of a pixel-sized triangle narrow to 3
-  tight_box(v, area, P);
-  for (int32_t j = P.j0; j <= P.j1; j++) {
-    const R cy = pixel_centre<R>(j, v.fh);
-    const uint64_t prow = row0 + uint64_t(j) * uint64_t(v.W);
-    for (int32_t i = P.i0; i <= P.i1; i++) {
-      R w0, w1, w2;
-      if (!pixel_weights(P, pixel_centre<R>(i, v.fw), cy, w0, w1, w2)) continue;
-      const R dep = pixel_depth(P, w0, w1, w2);
-      const uint64_t px = prow + uint64_t(i);
-      if constexpr (MODE == kRasterFp32) {
-        // keys only grow: a key at or below a value read from the buffer
-        // can never change it and skips the atomic
-        const uint64_t k = uint64_t(ord32(float(dep))) << 32 | uint64_t(tid1);
-        if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
-      } else if constexpr (MODE == kRasterAppend) {
-        Frag fr;
-        fr.px = uint32_t(px);
-        fr.id = tid1;
-        fr.key = sizeof(R) == 8 ? ord64(double(dep)) : (uint64_t(ord32(float(dep))) << 32 | uint64_t(tid1));
-        append_frag(fr, s_frag, s_n, cap_lds, gfrags, gcap, hdr);
-      } else if constexpr (MODE == kRasterKey64) {
-        const uint64_t k = ord64(double(dep));
-        if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
-      } else if constexpr (MODE == kRasterDepth64) {
-        const uint64_t k = ord64(double(dep));
-        if (k < key[px]) continue;
-        atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
-        // every fragment whose key equals the final value is kept: its key
-        // was at least the value read before its own atomic
-        const uint32_t o = atomicAdd(s_n, 1u);
-        if (o < cap_lds) {
-          Frag fr;
-          fr.px = uint32_t(px);
-          fr.id = tid1;
-          fr.key = k;
-          s_frag[o] = fr;
-        }
-      } else {
-        if (ord64(double(dep)) == key[px]) atomicMax(id + px, tid1);
-      }
-    }
-  }
+  if (!setup_triangle<R>(v, wpos, P)) return;
+  for (int32_t j = P.j0; j <= P.j1; j++)
+    for (int32_t i = P.i0; i <= P.i1; i++)
+      pixel_update<R, MODE>(v, row0, P, i, j, tid1, key, id, s_frag, s_n, cap_lds, gfrags, gcap, hdr);
 }
 
 // registers of one raster work item, loaded a round ahead
@@ -1224,30 +1244,35 @@ __global__ __launch_bounds__(kThreads) void k_resolve(MeshView<R> m, const Raste
 // Vertex normal of the vertex on cell-local edge e of cell c, recomputed
 // from the field: the cells sharing the edge in increasing linear order
 // (normal.c:78-158), in each the triangles of threshold thr with a vertex
-// on the edge in slot order, their smoothing weights summed from +0 and
-// normalised -- the sums of vertex_normal over a stored mesh, bit for bit.
+// on the edge in slot order (c_emask lists them), their smoothing weights
+// summed from +0 and normalised -- the sums of vertex_normal over a stored
+// mesh, bit for bit.  Lc, vc and code_c are cell c's location, corner
+// values and corner code.
 template <typename R>
-__device__ V3<R> vertex_normal_field(const FieldDev& f, double thr, uint32_t c, uint32_t e) {
+__device__ V3<R> vertex_normal_field(const FieldDev& f, double thr, uint32_t c, const CellLoc& Lc,
+                                     const double vc[8], uint32_t code_c, uint32_t e) {
   const int64_t sy = f.ncx, sz = int64_t(f.ncx) * f.ncy;
   V3<R> sum = {R(0), R(0), R(0)};
   const int nn = c_nbr.n[e];
   for (int q = 0; q < nn; q++) {
     const int dk = c_nbr.v[e][q][0], dj = c_nbr.v[e][q][1], di = c_nbr.v[e][q][2];
     const uint32_t want = uint32_t(c_nbr.v[e][q][3]);
+    const bool self = (dk | dj | di) == 0;
     const uint32_t c2 = uint32_t(int64_t(c) + dk * sz + dj * sy + di);
-    const CellLoc L2 = locate(f, c2);
+    const CellLoc L2 = self ? Lc : locate(f, c2);
     double v2[8];
-    load_corners(f, L2, v2);
-    const uint32_t code = corner_code(thr, v2);
-    const uint32_t nt = c_ntri[code];
-    for (uint32_t sl = 0; sl < nt; sl++) {
-      const uint64_t ent = c_cslot[code * 12u + sl];
-      const uint32_t eu = uint32_t(ent >> 18);
-      if ((eu & 31u) == want || ((eu >> 5) & 31u) == want || ((eu >> 10) & 31u) == want) {
-        V3<R> P[3];
-        slot_positions<R>(f, L2, thr, ent, v2, P);
-        sum = vadd(sum, smoothing_weight<R>(P));
-      }
+    if (self) {
+#pragma unroll
+      for (int n = 0; n < 8; n++) v2[n] = vc[n];
+    } else {
+      load_corners(f, L2, v2);
+    }
+    const uint32_t code = self ? code_c : corner_code(thr, v2);
+    for (uint32_t m = c_emask[code * 19u + want]; m; m &= m - 1u) {
+      const uint64_t ent = c_cslot[code * 12u + uint32_t(__builtin_ctz(m))];
+      V3<R> P[3];
+      slot_positions<R>(f, L2, thr, ent, v2, P);
+      sum = vadd(sum, smoothing_weight<R>(P));
     }
   }
   return vunit(sum);
@@ -1334,7 +1359,8 @@ __global__ __launch_bounds__(kThreads) void k_resolve_field(const TessJob* __res
         pixel_weights(P, pixel_centre<R>(i, v.fw), pixel_centre<R>(j, v.fh), w0, w1, w2);
         V3<R> nv[3];
 #pragma unroll 1
-        for (int n = 0; n < 3; n++) nv[n] = vertex_normal_field<R>(f, thr, c, uint32_t(ent >> (18 + 5 * n)) & 31u);
+        for (int n = 0; n < 3; n++)
+          nv[n] = vertex_normal_field<R>(f, thr, c, L, cv, code, uint32_t(ent >> (18 + 5 * n)) & 31u);
         const V3<R> n = vunit(vadd(vscale(w0, nv[0]), vadd(vscale(w1, nv[1]), vscale(w2, nv[2]))));
         const V3<R> light = {v.light[0], v.light[1], v.light[2]};
         const R fct = fmax(R(0.15), fabs(vdot(n, light)));
@@ -1573,6 +1599,7 @@ hipError_t upload_tables(const NbrTable& nt) {
   // per corner code: triangle count and every slot's corners and edge ids
   static uint8_t nt8[256];
   static uint64_t cs[256 * 12];
+  static uint16_t em[256 * 19];
   auto eid = [](uint32_t a, uint32_t b) -> uint32_t {
     const uint32_t lo = a < b ? a : b;
     switch (a ^ b) {
@@ -1600,6 +1627,7 @@ hipError_t upload_tables(const NbrTable& nt) {
           const uint32_t head = uint32_t(tet[t][pairs[mm][s][q][1]]);
           ent |= uint64_t(tail | head << 3) << (6 * v);
           ent |= uint64_t(eid(tail, head)) << (18 + 5 * v);
+          em[code * 19 + eid(tail, head)] |= uint16_t(1u << sl);
         }
         cs[code * 12 + sl] = ent;
       }
@@ -1609,6 +1637,8 @@ hipError_t upload_tables(const NbrTable& nt) {
   e = hipMemcpyToSymbol(HIP_SYMBOL(c_ntri), nt8, sizeof(nt8));
   if (e != hipSuccess) return e;
   e = hipMemcpyToSymbol(HIP_SYMBOL(c_cslot), cs, sizeof(cs));
+  if (e != hipSuccess) return e;
+  e = hipMemcpyToSymbol(HIP_SYMBOL(c_emask), em, sizeof(em));
   if (e != hipSuccess) return e;
   return hipMemcpyToSymbol(HIP_SYMBOL(c_nbr), &nt, sizeof(nt));
 }
