@@ -1000,7 +1000,7 @@ __device__ __forceinline__ void ring_tiles(FusedLds& S, const TessJob* __restric
 }
 
 template <typename R, int MODE>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 4 : 5))) void
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void
 k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles, const uint64_t* __restrict__ start,
               uint32_t* __restrict__ cellout, uint64_t cap, const RasterJob<R>* __restrict__ rjobs,
               const uint32_t* __restrict__ jimg, FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
