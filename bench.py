@@ -339,16 +339,21 @@ def served_stream(api, fields, nreq, rate, nfield_pool, seed=0):
     done.set()
     th.join()
     st = srv.stats()
-    # saturation: every request submitted at once (the first pass sizes the
-    # buffers for the larger batches, the second is timed)
+    # saturation: every request submitted at once; the images (with their
+    # host buffers) are built once, and each path keeps its best of three
+    # passes (the first sizes the buffers for the larger batches)
     nsat = min(48, nreq)
-    for rep in range(2):
+    sat_imgs = [image_of(r) for r in reqs[:nsat]]
+    st0 = srv.stats()
+    sat_served = 0.0
+    for rep in range(3):
         t1 = time.perf_counter()
-        ts = [srv.submit(image_of(r), api.FP32) for r in reqs[:nsat]]
+        ts = [srv.submit(im, api.FP32) for im in sat_imgs]
         for t_ in ts:
             srv.wait(t_)
-        sat_served = nsat / (time.perf_counter() - t1)
-    sat_batch = srv.stats()["mean_batch"]
+        sat_served = max(sat_served, nsat / (time.perf_counter() - t1))
+    st1 = srv.stats()
+    sat_batch = (st1["completed"] - st0["completed"]) / max(1, st1["batches"] - st0["batches"])
     srv.stop()
     lat.sort()
     served = {"requests": nreq, "rate_per_s": rate, "precision": "fp32", "field_pool": nfield_pool,
@@ -370,14 +375,15 @@ def served_stream(api, fields, nreq, rate, nfield_pool, seed=0):
     pct = lambda v, q: v[min(len(v) - 1, int(q * (len(v) - 1) + 0.5))]  # noqa: E731
     served["plain"] = {"p50_ms": round(pct(plat, 0.5), 2), "p99_ms": round(pct(plat, 0.99), 2),
                        "frames_per_s": round(nreq / span, 2)}
-    t1 = time.perf_counter()
-    for r in reqs[:nsat]:
-        im = image_of(r)
-        api.execute_rgb(im["camera"], im["light"], im["size"], im["center"], im["local"], im["bg"], im["contours"],
-                        precision=api.FP32, rgb=im["rgb"])
-    sat_plain = nsat / (time.perf_counter() - t1)
+    sat_plain = 0.0
+    for rep in range(3):
+        t1 = time.perf_counter()
+        for im in sat_imgs:
+            api.execute_rgb(im["camera"], im["light"], im["size"], im["center"], im["local"], im["bg"],
+                            im["contours"], precision=api.FP32, rgb=im["rgb"])
+        sat_plain = max(sat_plain, nsat / (time.perf_counter() - t1))
     served["saturated"] = {"requests": nsat, "served_frames_per_s": round(sat_served, 2),
-                           "plain_frames_per_s": round(sat_plain, 2), "mean_batch_all": round(sat_batch, 2)}
+                           "plain_frames_per_s": round(sat_plain, 2), "mean_batch": round(sat_batch, 2)}
     return served
 
 
