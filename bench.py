@@ -178,8 +178,9 @@ def shared_scene(api, fields, contours, reps):
             len(views) / _timed(lambda: api.execute_batch(imgs, precision=api.FP32), reps), 2)
     finally:
         api.tune(api.TUNE_GRAPHS, 0)
-    # the serving loop, all four requests submitted together each round
-    with api.Server(max_images=16) as srv:
+    # the serving loop, all four requests submitted together each round; a
+    # 0.5 ms batching window lets the four share one frame (one tessellation)
+    with api.Server(max_images=16, max_wait_ms=0.5) as srv:
         def served():
             ts = [srv.submit(im, api.FP32) for im in imgs]
             for t in ts:

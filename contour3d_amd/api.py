@@ -119,6 +119,20 @@ class _Keep:
         self.items = []
 
 
+_UNIT_GRIDS = {}
+
+
+def _unit_grid(m):
+    """The cell-centred grid of m points on [-0.5, 0.5] (src/main.c:48-54),
+    one read-only array per size."""
+    g = _UNIT_GRIDS.get(m)
+    if g is None:
+        g = -0.5 + (np.arange(m) + 0.5) / m
+        g.setflags(write=False)
+        _UNIT_GRIDS[m] = g
+    return g
+
+
 def contour_objs(contours, keep):
     """ctypes array of contour3d_contour_obj_t from dicts with keys field
     (nz, ny, nx), threshold, color and optional grids / converter."""
@@ -133,7 +147,7 @@ def contour_objs(contours, keep):
         nz, ny, nx = _shape(f)
         grids = c.get("grids")
         if grids is None:
-            grids = tuple(-0.5 + (np.arange(m) + 0.5) / m for m in (nx, ny, nz))
+            grids = tuple(_unit_grid(m) for m in (nx, ny, nz))
         grids = [np.ascontiguousarray(g, dtype=np.float64) for g in grids]
         keep.items += [f] + grids
         conv = c.get("converter", ident)

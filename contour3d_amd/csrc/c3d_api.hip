@@ -107,11 +107,16 @@ struct HostBuf {
 };
 
 // lattice coordinates of one (converter, grids) geometry
+// geometry cache limits (entries, bytes of converter coordinate tables)
+constexpr size_t kMaxGeoms = 1024;
+constexpr size_t kGeomTableBytes = size_t(16) << 30;
+
 struct Geom {
   void* conv = nullptr;
   const double* g[3] = {nullptr, nullptr, nullptr};
   size_t n[3] = {0, 0, 0};
   uint64_t hash = 0;
+  std::vector<double> host;  // the grid values (nx + ny + nz), compared on a hash match
   bool identity = true;
   DevBuf grids;  // nx + ny + nz doubles
   DevBuf table;  // 3 * npts doubles when not the identity
@@ -389,9 +394,13 @@ int get_geometry(Lib& L, const contour3d_contour_obj_t& o, Geom** out) {
   }
   uint64_t h = 1469598103934665603ull;
   for (int d = 0; d < 3; d++) h = fnv(gp[d], o.glsizes[d] * sizeof(double), h);
+  // a geometry is its converter and grid values: callers that rebuild equal
+  // grid arrays for every call still hit the cache
   for (auto& g : L.geoms) {
     if (g->conv == (void*)o.converter && g->hash == h && g->n[0] == nx && g->n[1] == ny && g->n[2] == nz &&
-        g->g[0] == o.grids[0] && g->g[1] == o.grids[1] && g->g[2] == o.grids[2]) {
+        std::memcmp(g->host.data(), gp[0], nx * sizeof(double)) == 0 &&
+        std::memcmp(g->host.data() + nx, gp[1], ny * sizeof(double)) == 0 &&
+        std::memcmp(g->host.data() + nx + ny, gp[2], nz * sizeof(double)) == 0) {
       g->stamp = ++L.stamp;
       *out = g.get();
       return 0;
@@ -404,6 +413,9 @@ int get_geometry(Lib& L, const contour3d_contour_obj_t& o, Geom** out) {
     g->n[d] = o.glsizes[d];
   }
   g->hash = h;
+  g->host.assign(gp[0], gp[0] + nx);
+  g->host.insert(g->host.end(), gp[1], gp[1] + ny);
+  g->host.insert(g->host.end(), gp[2], gp[2] + nz);
   C3D_HIP(g->grids.ensure((nx + ny + nz) * sizeof(double)));
   C3D_HIP(hipMemcpy(g->grids.as<double>(), gp[0], nx * sizeof(double), hipMemcpyHostToDevice));
   C3D_HIP(hipMemcpy(g->grids.as<double>() + nx, gp[1], ny * sizeof(double), hipMemcpyHostToDevice));
@@ -445,18 +457,23 @@ int get_geometry(Lib& L, const contour3d_contour_obj_t& o, Geom** out) {
     }
   }
   g->stamp = ++L.stamp;
-  // keep a handful of geometries; evict the least recently used one that
-  // no frame in flight reads
-  if (L.geoms.size() >= 16) {
+  // Keep up to kMaxGeoms geometries and kGeomTableBytes of coordinate
+  // tables; evict the least recently used ones that no frame in flight
+  // reads.  A grid-only geometry holds a few kilobytes, so a batch of many
+  // lattices stays cached: an eviction frees device memory, and a free
+  // waits for the device.
+  size_t table_bytes = g->table.bytes;
+  for (auto& e : L.geoms) table_bytes += e->table.bytes;
+  while (L.geoms.size() >= kMaxGeoms || table_bytes > kGeomTableBytes) {
     size_t victim = L.geoms.size();
     for (size_t q = 0; q < L.geoms.size(); q++)
       if (L.geoms[q]->busy.load() == 0 && (victim == L.geoms.size() || L.geoms[q]->stamp < L.geoms[victim]->stamp))
         victim = q;
-    if (victim < L.geoms.size()) {
-      L.geoms[victim]->grids.release();
-      L.geoms[victim]->table.release();
-      L.geoms.erase(L.geoms.begin() + victim);
-    }
+    if (victim == L.geoms.size()) break;
+    table_bytes -= L.geoms[victim]->table.bytes;
+    L.geoms[victim]->grids.release();
+    L.geoms[victim]->table.release();
+    L.geoms.erase(L.geoms.begin() + victim);
   }
   *out = g.get();
   L.geoms.push_back(std::move(g));
