@@ -321,3 +321,23 @@ def test_large_triangles_fill_the_fragment_lists(orc):
     for prec in (api.FP64, api.FP32):
         assert np.array_equal(out[0, prec][0], out[1, prec][0]), prec
         assert same_bits(out[0, prec][1], out[1, prec][1]), prec
+
+
+def test_geometry_cache_follows_grid_values(orc):
+    """Lattice geometries are cached by converter and grid values: equal
+    grids passed as new arrays reuse the cache, different grids of the same
+    size do not.  Alternating the two, each image stays bit-exact."""
+    banner("geometry cache keyed on grid values")
+    f = fields.blobs((20, 18, 16), seed=9)
+    nz, ny, nx = f.shape
+    uni = [(-0.5 + (np.arange(m) + 0.5) / m) for m in (nx, ny, nz)]
+    cheb = [0.5 * np.sin(np.pi * (np.arange(m) + 0.5) / m - np.pi / 2) for m in (nx, ny, nz)]
+    v = view_for(96, 72)
+    for grids in (uni, cheb, [g.copy() for g in uni], [g.copy() for g in cheb]):
+        cont = [dict(field=f, grids=tuple(grids), threshold=0.45, color=(200, 150, 60))]
+        rgb, dep = api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], cont,
+                                   precision=api.FP64, want_depth=True)
+        ref_rgb, ref_dep = orc.execute(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], cont)
+        assert (ref_dep > -1e300).sum() > 100
+        assert same_bits(dep, ref_dep)
+        assert np.array_equal(rgb, ref_rgb)
