@@ -364,6 +364,41 @@ __device__ __forceinline__ void slot_positions(const FieldDev& f, const CellLoc&
   }
 }
 
+// The six grid coordinates of cell L (x0 x1 y0 y1 z0 z1) on a grid lattice.
+__device__ __forceinline__ void cell_grid(const FieldDev& f, const CellLoc& L, double g6[6]) {
+  g6[0] = gptr(f.gx)[L.i];
+  g6[1] = gptr(f.gx)[L.i + 1];
+  g6[2] = gptr(f.gy)[L.j];
+  g6[3] = gptr(f.gy)[L.j + 1];
+  g6[4] = gptr(f.gz)[L.k];
+  g6[5] = gptr(f.gz)[L.k + 1];
+}
+
+// slot_positions with the cell's corner values v and, on a grid lattice,
+// its six grid coordinates g6 (cell_grid) already in registers
+template <typename R>
+__device__ __forceinline__ void slot_positions_reg(const FieldDev& f, const CellLoc& L, double thr, uint64_t ent,
+                                                   const double v[8], const double g6[6], V3<R> P[3]) {
+  if (f.cx) {
+    slot_positions<R>(f, L, thr, ent, v, P);
+    return;
+  }
+#pragma unroll
+  for (int vv = 0; vv < 3; vv++) {
+    const uint32_t tail = uint32_t(ent >> (6 * vv)) & 7u;
+    const uint32_t head = uint32_t(ent >> (6 * vv + 3)) & 7u;
+    double vt = v[0], vh = v[0];
+#pragma unroll
+    for (int q = 1; q < 8; q++) {
+      vt = tail == uint32_t(q) ? v[q] : vt;
+      vh = head == uint32_t(q) ? v[q] : vh;
+    }
+    const V3<R> pt = {R(tail & 1u ? g6[1] : g6[0]), R(tail & 2u ? g6[3] : g6[2]), R(tail & 4u ? g6[5] : g6[4])};
+    const V3<R> ph = {R(head & 1u ? g6[1] : g6[0]), R(head & 2u ? g6[3] : g6[2]), R(head & 4u ? g6[5] : g6[4])};
+    P[vv] = interp<R>(thr, pt, vt, ph, vh);
+  }
+}
+
 // The smoothing weight of a triangle: area * unit face normal
 // (triangulate.c:75-93), formed as normal.c:39-40 forms it.  The FP32 form
 // takes the cross product itself (the same vector): a triangle whose
@@ -1268,10 +1303,13 @@ __device__ V3<R> vertex_normal_field(const FieldDev& f, double thr, uint32_t c, 
       load_corners(f, L2, v2);
     }
     const uint32_t code = self ? code_c : corner_code(thr, v2);
-    for (uint32_t m = c_emask[code * 19u + want]; m; m &= m - 1u) {
+    uint32_t m = c_emask[code * 19u + want];
+    double g6[6] = {0, 0, 0, 0, 0, 0};
+    if (m && !f.cx) cell_grid(f, L2, g6);
+    for (; m; m &= m - 1u) {
       const uint64_t ent = c_cslot[code * 12u + uint32_t(__builtin_ctz(m))];
       V3<R> P[3];
-      slot_positions<R>(f, L2, thr, ent, v2, P);
+      slot_positions_reg<R>(f, L2, thr, ent, v2, g6, P);
       sum = vadd(sum, smoothing_weight<R>(P));
     }
   }
