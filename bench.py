@@ -164,12 +164,15 @@ def shared_scene(api, fields, contours, reps):
 
     res = {"cameras": len(views), "image": [W, H]}
     for name, prec in (("fp64", api.FP64), ("fp32", api.FP32)):
+        log(f"shared scene: plain {name}")
         plain(prec)
         res[f"plain_{name}_fps"] = round(len(views) / _timed(lambda: plain(prec), reps), 2)
+    log("shared scene: batched FP32 and FP64")
     api.execute_batch(imgs, precision=api.FP32)
     res["batched_fp32_fps"] = round(len(views) / _timed(lambda: api.execute_batch(imgs, precision=api.FP32), reps), 2)
     api.execute_batch(imgs, precision=api.FP64)
     res["batched_fp64_fps"] = round(len(views) / _timed(lambda: api.execute_batch(imgs, precision=api.FP64), reps), 2)
+    log("shared scene: graph-replayed FP32")
     api.tune(api.TUNE_GRAPHS, 1)
     try:
         api.execute_batch(imgs, precision=api.FP32)
@@ -178,9 +181,9 @@ def shared_scene(api, fields, contours, reps):
             len(views) / _timed(lambda: api.execute_batch(imgs, precision=api.FP32), reps), 2)
     finally:
         api.tune(api.TUNE_GRAPHS, 0)
-    # the serving loop, all four requests submitted together each round; a
-    # 0.5 ms batching window lets the four share one frame (one tessellation)
-    with api.Server(max_images=16, max_wait_ms=0.5) as srv:
+    # the serving loop, all four requests submitted together each round
+    log("shared scene: served FP32")
+    with api.Server(max_images=16) as srv:
         def served():
             ts = [srv.submit(im, api.FP32) for im in imgs]
             for t in ts:
