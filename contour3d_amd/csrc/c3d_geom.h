@@ -162,16 +162,24 @@ C3D_HD R pixel_depth(const Projected<R>& P, R w0, R w1, R w2) {
 // so the coverage loop skips the reference box's margin.  Needs the
 // P.ainv of raster_box and v.pitch (set_pitch).
 //
-// Bound: with exact weights w_i = E_i / A (sum 1, sum w_i q_i = p), a pixel
-// centre p farther than M to the left of every vertex has, for
-// d_i = q_i.x - p.x in (M, D], sum over negative w_i of |w_i| d_i equal to
-// the positive part, which is at least M; so some w_i < -M / (2D).  The
-// computed weights differ from the exact ones by at most 4 eps D / h (h a
-// lower bound of the triangle's heights, here A / (1.5 emax)) plus a
-// relative 1e-3, so with M = 2D (2e-8 + 32 eps D / h) (+ the rounding of
-// the comparison bounds) the computed test rejects p.  Same for the other
-// three sides.  Thin or tiny triangles (error bound above 1e-3, area near
-// the 1e-16 clamp) and non-finite values keep the reference box.
+// Bound (u the unit roundoff, g = 4.5u >= 4u / (1 - 4u)).  The computed
+// test accepts p only if, for every edge i, ainv * e_i >= -1e-8 with e_i
+// its computed edge function; with |A| the computed doubled area (not
+// clamped: |A| > 1e-14) that gives s * e_i >= -1e-8 |A| (1 + 4u), s the
+// sign of A.  Each e_i is a difference of two products of differences,
+// so it is within g (|cx - ax| |by - ay| + |cy - ay| |bx - ax|) <=
+// g sqrt(2) D L_i of the exact value (D >= the distance of any reference
+// box centre from a vertex, L_i the edge's length), and so is the area
+// within 2 g emax^2 (emax the largest coordinate difference along an
+// edge).  When |A| > 4 g emax^2 the computed area has the exact sign, and
+// an accepted p lies within d_i = 1e-8 |A| (1 + 4u) / L_i + g sqrt(2) D
+// outside every edge: inside the triangle whose edges are pushed out by
+// d_i, whose vertex between edges j and k moves by at most
+// (d_j + d_k) L_j L_k / |A| <= (4.04e-8 Lmax |A| + 2 sqrt(2) g D Lmax^2) / |A|,
+// Lmax <= sqrt(2) emax.  So with M = 5.8e-8 emax + 12 g D emax^2 / |A|
+// (+ the rounding of the comparison bounds), every accepted centre lies
+// within M of the vertices' box.  Tiny, collinear and non-finite
+// triangles keep the reference box.
 //
 // Columns: an accepted centre has pixel_centre(i) >= xlo, and with
 // u = (xlo + 0.5) fw - 0.5 the rounding of u and of pixel_centre stays
@@ -182,6 +190,7 @@ C3D_HD R pixel_depth(const Projected<R>& P, R w0, R w1, R w2) {
 template <typename R>
 C3D_HD void tight_box(const ViewDev<R>& v, R area, Projected<R>& P) {
   const R eps = sizeof(R) == 8 ? R(2.220446049250313e-16) : R(1.1920929e-7);
+  const R g = R(2.25) * eps;  // 4.5 u
   const V3<R>* q = P.q;
   const R minx = fmin(q[0].x, fmin(q[1].x, q[2].x)), maxx = fmax(q[0].x, fmax(q[1].x, q[2].x));
   const R miny = fmin(q[0].y, fmin(q[1].y, q[2].y)), maxy = fmax(q[0].y, fmax(q[1].y, q[2].y));
@@ -192,12 +201,11 @@ C3D_HD void tight_box(const ViewDev<R>& v, R area, Projected<R>& P) {
                       fmax(fabs(q[0].x - q[2].x), fabs(q[0].y - q[2].y)));
   const R A = fabs(area);
   if (!(A > R(1.e-14)) || !(emax > R(0))) return;
-  // rel = 16 eps D / h = 24 eps D emax / A; |P.ainv| is 1 / A rounded once
-  // (A > 1e-16 is not clamped), and 1 + 2^-20 covers the products' rounding
-  const R rel = R(24) * eps * D * emax * fabs(P.ainv) * R(1.00000095367431640625);
-  if (!(rel < R(1.e-3))) return;
+  const R e2 = emax * emax;
+  if (!(A > R(4.1) * g * e2)) return;
+  // |P.ainv| is 1 / A rounded once; 1 + 2^-20 covers the products' rounding
   const R qmax = fmax(fmax(fmax(fabs(minx), fabs(maxx)), fmax(fabs(miny), fabs(maxy))), R(1));
-  const R M = R(2) * D * (R(2.e-8) + R(2) * rel) + R(16) * eps * qmax;
+  const R M = (R(5.8e-8) * emax + R(12) * g * D * e2 * fabs(P.ainv)) * R(1.00000095367431640625) + R(16) * eps * qmax;
   if (!(M < pitch)) return;
   const R ulo = (minx - M + R(0.5)) * v.fw - R(0.5), uhi = (maxx + M + R(0.5)) * v.fw - R(0.5);
   const R vlo = (miny - M + R(0.5)) * v.fh - R(0.5), vhi = (maxy + M + R(0.5)) * v.fh - R(0.5);

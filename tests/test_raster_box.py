@@ -35,7 +35,8 @@ def test_narrowed_boxes_hold_every_accepted_pixel(tmp_path):
 
 def test_check_sees_a_missing_margin(tmp_path):
     hdr = open(os.path.join(CSRC, "c3d_geom.h")).read()
-    line = "const R M = R(2) * D * (R(2.e-8) + R(2) * rel) + R(16) * eps * qmax;"
+    line = ("const R M = (R(5.8e-8) * emax + R(12) * g * D * e2 * fabs(P.ainv)) * R(1.00000095367431640625) + "
+            "R(16) * eps * qmax;")
     assert line in hdr
     mut = tmp_path / "mut"
     mut.mkdir()
