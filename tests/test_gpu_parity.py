@@ -341,3 +341,31 @@ def test_geometry_cache_follows_grid_values(orc):
         assert (ref_dep > -1e300).sum() > 100
         assert same_bits(dep, ref_dep)
         assert np.array_equal(rgb, ref_rgb)
+
+
+def test_dense_cells_and_many_thresholds(orc):
+    """A checkerboard field crosses every cell at every threshold near 0, so
+    a 256-cell chunk yields thousands of triangles (many queue drains per
+    chunk), and six thresholds on one field take two field passes.  FP64
+    images are bit-exact with the oracle; the FP32 image equals the same
+    image rendered in a batch."""
+    banner("dense cells, six thresholds on one field")
+    n = 18
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    f = np.where((i + j + k) % 2 == 0, 1.0, -1.0) + 0.01 * np.sin(i + 2.0 * j + 3.0 * k)
+    cols = [(250, 40, 40), (40, 250, 40), (40, 40, 250), (250, 250, 40), (250, 40, 250), (40, 250, 250)]
+    cont = [dict(field=f, threshold=t, color=c) for t, c in zip((-0.6, -0.3, -0.05, 0.05, 0.3, 0.6), cols)]
+    v = view_for(160, 120)
+    args = (v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], cont)
+    rgb, dep = api.execute_rgb(*args, precision=api.FP64, want_depth=True)
+    ref_rgb, ref_dep = orc.execute(*args)
+    assert (ref_dep > -1e300).sum() > 2000
+    assert same_bits(dep, ref_dep)
+    assert np.array_equal(rgb, ref_rgb)
+    rgb32, dep32 = api.execute_rgb(*args, precision=api.FP32, want_depth=True)
+    im = dict(camera=v["camera"], light=v["light"], size=v["size"], center=v["center"], local=v["local"],
+              bg=v["bg"], contours=cont, want_depth=True)
+    other = dict(im, contours=cont[:2])
+    got = api.execute_batch([other, im], precision=api.FP32)
+    assert np.array_equal(got[1][0], rgb32)
+    assert same_bits(got[1][1], dep32)
