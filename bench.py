@@ -473,7 +473,7 @@ def main():
     ap.add_argument("--precision-n", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
     if args.calibrate_copy:
         import torch
