@@ -795,7 +795,7 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
   tj.clear();
   F.obj_job.assign(n, 0);
   F.obj_q.assign(n, 0);
-  uint64_t jt = 0;
+  uint64_t jt = 0, jx = 0;
   for (size_t o = 0; o < n; o++) {
     if (group[o] != int(o)) continue;
     TessJob job{};
@@ -810,14 +810,18 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
       job.ts.n++;
       if (job.ts.n == kMaxThr) {
         job.tile0 = uint32_t(jt);
+        job.xoff = uint32_t(jx);
         jt += fd[o].nblocks;
+        jx += (fd[o].nblocks + 7) / 8;
         tj.push_back(job);
         job.ts = ThrSet{};
       }
     }
     if (job.ts.n) {
       job.tile0 = uint32_t(jt);
+      job.xoff = uint32_t(jx);
       jt += fd[o].nblocks;
+      jx += (fd[o].nblocks + 7) / 8;
       tj.push_back(job);
     }
   }

@@ -51,11 +51,13 @@ struct ThrSet {
 
 // One pass over one field: the unit of work of the tessellation kernels.
 // Jobs are stored in a device table; their tiles are numbered consecutively
-// from tile0.
+// from tile0.  xoff is the sum of ceil(tiles / 8) over the earlier jobs: the
+// job's first position in each XCD's walk over the launch's tiles.
 struct TessJob {
   FieldDev f;
   ThrSet ts;
   uint32_t tile0;
+  uint32_t xoff;
 };
 
 // Triangle storage, structure of arrays with capacity cap: component c of

@@ -29,24 +29,6 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
   return uint32_t((uint64_t(__umulhi(n, f.m)) + n) >> f.s);
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs.  Work items
-// (tiles) are split into 8 contiguous segments, one per XCD, and the XCD's
-// workgroups stride through their segment, so neighbouring z-planes meet in
-// that XCD's L2.  The grid must be a multiple of 8.
-struct XcdRange {
-  uint32_t begin, end, step;
-};
-
-__device__ __forceinline__ XcdRange xcd_range(uint32_t n) {
-  const uint32_t x = blockIdx.x % 8u, w = blockIdx.x / 8u, per_xcd = gridDim.x / 8u;
-  const uint32_t seg = (n + 7u) / 8u;
-  XcdRange r;
-  r.begin = x * seg + w;
-  r.end = min((x + 1u) * seg, n);
-  r.step = per_xcd;
-  return r;
-}
-
 // ----------------------------------------------------------------------
 // block-wide exclusive scan of one value per thread (256 threads)
 
@@ -195,14 +177,44 @@ __device__ __forceinline__ uint64_t packed_counts(const ThrSet& ts, const double
 // ----------------------------------------------------------------------
 // tessellation: count pass
 
-__device__ __forceinline__ uint32_t find_tjob(const TessJob* __restrict__ jobs, int n, uint32_t tile) {
-  int lo = 0, hi = n - 1;
+// Workgroups are dispatched round-robin over the 8 XCDs.  The tiles of each
+// field pass are split into 8 contiguous segments, one per XCD, so that
+// neighbouring z-planes meet in that XCD's L2 and every XCD gets an equal
+// share of every pass, however unequal the passes of a batch are.  XCD x
+// walks positions 0 .. end-1: position p lies in the pass jb with
+// jobs[jb].xoff <= p < jobs[jb + 1].xoff, at tile x * seg + p - xoff of it
+// (seg = ceil(tiles of the pass / 8)); the XCD's workgroups stride through
+// the positions.  The grid must be a multiple of 8.
+struct XcdWalk {
+  uint32_t x, begin, end, step;
+};
+
+__device__ __forceinline__ XcdWalk xcd_walk(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles) {
+  const TessJob& last = jobs[njobs - 1];
+  XcdWalk w;
+  w.x = blockIdx.x % 8u;
+  w.begin = blockIdx.x / 8u;
+  w.end = last.xoff + (ntiles - last.tile0 + 7u) / 8u;
+  w.step = gridDim.x / 8u;
+  return w;
+}
+
+// The pass and the launch-wide tile at position p of XCD x's walk; false
+// where the XCD's segment of that pass is shorter than seg (the last ones).
+__device__ __forceinline__ bool xcd_tile(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles, uint32_t x,
+                                         uint32_t p, uint32_t& jb, uint32_t& gt) {
+  int lo = 0, hi = njobs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].tile0 <= tile) lo = mid;
+    if (jobs[mid].xoff <= p) lo = mid;
     else hi = mid - 1;
   }
-  return uint32_t(lo);
+  jb = uint32_t(lo);
+  const uint32_t t0 = jobs[lo].tile0;
+  const uint32_t nt = (lo + 1 < njobs ? jobs[lo + 1].tile0 : ntiles) - t0;
+  const uint32_t t = x * ((nt + 7u) / 8u) + (p - jobs[lo].xoff);
+  gt = t0 + t;
+  return t < nt;
 }
 
 __global__ __launch_bounds__(kThreads) void k_count(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
@@ -211,9 +223,10 @@ __global__ __launch_bounds__(kThreads) void k_count(const TessJob* __restrict__ 
   __shared__ uint8_t s_ntri[256];
   s_ntri[threadIdx.x] = c_ntri[threadIdx.x];
   __syncthreads();
-  const XcdRange R = xcd_range(ntiles);
-  for (uint32_t gt = R.begin; gt < R.end; gt += R.step) {
-    const uint32_t jb = find_tjob(jobs, njobs, gt);
+  const XcdWalk W = xcd_walk(jobs, njobs, ntiles);
+  for (uint32_t p = W.begin; p < W.end; p += W.step) {
+    uint32_t jb, gt;
+    if (!xcd_tile(jobs, njobs, ntiles, W.x, p, jb, gt)) continue;
     const FieldDev f = jobs[jb].f;
     const ThrSet ts = jobs[jb].ts;
     const uint32_t tile = gt - jobs[jb].tile0;
@@ -941,9 +954,10 @@ __device__ __forceinline__ void ring_tiles(FusedLds& S, const TessJob* __restric
   // ring state, the same in every thread: oldest item, items queued, and
   // the field pass they belong to
   uint32_t head = 0, npend = 0, cur = 0;
-  const XcdRange XR = xcd_range(ntiles);
-  for (uint32_t gt = XR.begin; gt < XR.end; gt += XR.step) {
-    const uint32_t jb = find_tjob(jobs, njobs, gt);
+  const XcdWalk W = xcd_walk(jobs, njobs, ntiles);
+  for (uint32_t p = W.begin; p < W.end; p += W.step) {
+    uint32_t jb, gt;
+    if (!xcd_tile(jobs, njobs, ntiles, W.x, p, jb, gt)) continue;
     // the queue holds one field pass at a time
     while (npend && jb != cur) {
       const uint32_t n = npend < kThreads ? npend : kThreads;
