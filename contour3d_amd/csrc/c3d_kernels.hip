@@ -1331,17 +1331,18 @@ __device__ V3<R> vertex_normal_field(const FieldDev& f, double thr, uint32_t c, 
 }
 
 // k_resolve for a frame without a stored mesh: the nearest triangle is
-// found from its serial id (object by base, cell from the stored cell
-// index, slot by the cell's first triangle) and recomputed from its field.
+// found from its serial id (object among the image's objects by base, cell
+// from the stored cell index, slot from the cell store's neighbouring
+// entries) and recomputed from its field.  The kernel is latency-bound (a
+// chain of dependent gathers per pixel), so occupancy pays: the FP32 form
+// runs at 4 waves per SIMD (128 VGPRs, a small spill), the FP64 form at 3.
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_resolve_field(const TessJob* __restrict__ jobs,
-                                                            const uint32_t* __restrict__ cell,
-                                                            const RasterJob<R>* __restrict__ rjobs, int njobs,
-                                                            uint64_t npx_total, const ObjMeta* __restrict__ meta,
-                                                            int nobj, const uint8_t* __restrict__ frag_rgb,
-                                                            const FrameHdr* __restrict__ hdr,
-                                                            const uint64_t* __restrict__ key,
-                                                            const uint32_t* __restrict__ idbuf) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 3 : 4)))
+void k_resolve_field(const TessJob* __restrict__ jobs, const uint32_t* __restrict__ cell,
+                     const RasterJob<R>* __restrict__ rjobs, int njobs, uint64_t npx_total,
+                     const ObjMeta* __restrict__ meta, int nobj, const uint8_t* __restrict__ frag_rgb,
+                     const FrameHdr* __restrict__ hdr, const uint64_t* __restrict__ key,
+                     const uint32_t* __restrict__ idbuf) {
   // a frame whose triangles outgrew the cell store is rendered again: skip it
   if (hdr->overflow & 1u) return;
   const uint64_t ntri = hdr->ntri;
@@ -1374,9 +1375,10 @@ __global__ __launch_bounds__(kThreads) void k_resolve_field(const TessJob* __res
       dep = id ? double(unord32(uint32_t(k >> 32))) : -DBL_MAX;
     }
     uint8_t r = J.bg[0], gg = J.bg[1], b = J.bg[2];
-    if (id != 0 && id <= ntri) {
+    if (id != 0 && id <= ntri && J.obj0 < J.obj1) {
       const uint64_t t = id - 1;
-      int lo = 0, hi = nobj - 1;
+      // the winner is one of the image's own objects
+      int lo = int(J.obj0), hi = int(J.obj1) - 1;
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (meta[mid].base <= t) lo = mid;
@@ -1387,20 +1389,26 @@ __global__ __launch_bounds__(kThreads) void k_resolve_field(const TessJob* __res
       double thr = jobs[om.job].ts.thr[0];
       for (uint32_t z = 1; z < kMaxThr; z++) thr = om.q == z ? jobs[om.job].ts.thr[z] : thr;
       const uint32_t c = cell[t];
-      // a cell index outside the object's field or its interior, or a slot
-      // outside its cell, can only come from a damaged cell store: such a pixel keeps
-      // the background rather than read out of bounds
+      // Within an object the cells ascend with the serial position and a
+      // cell has at most 12 triangles, so the slot of t is the number of
+      // the 11 positions before it, inside the object, that hold the same
+      // cell.  The 11 loads are independent and share one or two lines.
+      uint32_t sl = 0;
+#pragma unroll
+      for (uint32_t d = 1; d < 12; d++) sl += t >= om.base + d && cell[t - d] == c ? 1u : 0u;
+      // a position outside the object, a cell index outside the object's
+      // field or its interior, or a slot outside its cell, can only come
+      // from a damaged cell store: such a pixel keeps the background rather
+      // than read out of bounds
       do {
-        if (c >= f.ncells) break;
-        const uint32_t tl = c / kCellsPerBlock;
-        const uint64_t first = lower_bound_cell(cell, om.bstart[tl], om.bstart[tl + 1], c);
+        if (t < om.base || t - om.base >= om.count || c >= f.ncells) break;
         const CellLoc L = locate(f, c);
         if (!interior_cell(f, L)) break;
         double cv[8];
         load_corners(f, L, cv);
         const uint32_t code = corner_code(thr, cv);
-        if (t < first || t - first >= c_ntri[code]) break;
-        const uint64_t ent = c_cslot[code * 12u + uint32_t(t - first)];
+        if (sl >= c_ntri[code]) break;
+        const uint64_t ent = c_cslot[code * 12u + sl];
         V3<R> wpos[3];
         slot_positions<R>(f, L, thr, ent, cv, wpos);
         Projected<R> P;
