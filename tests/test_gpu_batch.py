@@ -58,6 +58,42 @@ def test_uneven_batch_equals_images_alone(orc, precision):
             assert same_bits(got[q][1], ref_dep), q
 
 
+def small_pass_batch():
+    """Field passes of 1 to 26 tiles (1024 cells each): each pass's tiles are
+    split over the 8 XCDs, so most of these leave some XCDs nothing of that
+    pass.  The last field has six thresholds, two passes over one field."""
+    out = []
+    shapes = [(3, 3, 3), (9, 7, 5), (12, 11, 10), (20, 18, 16), (33, 30, 29), (6, 40, 7), (24, 22, 20)]
+    for q, shape in enumerate(shapes):
+        f = fields.blobs(shape, seed=20 + q)
+        nthr = 6 if q == len(shapes) - 1 else 1 + q % 4
+        im = view_for(48 + 8 * q, 40 + 4 * q)
+        im.update(contours=[dict(field=f, threshold=float(t), color=(40 + 30 * k, 200 - 20 * k, 90))
+                            for k, t in enumerate(np.linspace(0.25, 0.75, nthr))],
+                  lines=[], want_depth=True)
+        out.append(im)
+    return out
+
+
+@pytest.mark.parametrize("precision", [api.FP64, api.FP32], ids=["fp64", "fp32"])
+def test_many_small_passes_equal_images_alone(orc, precision):
+    banner(f"batch of small field passes precision={precision}")
+    batch = small_pass_batch()
+    got = api.execute_batch(batch, precision=precision)
+    drawn = 0
+    for q, im in enumerate(batch):
+        rgb, dep = alone(im, precision)
+        assert np.array_equal(got[q][0], rgb), q
+        assert same_bits(got[q][1], dep), q
+        drawn += int((dep > -1e300).sum())
+        if precision == api.FP64:
+            ref_rgb, ref_dep = orc.execute(im["camera"], im["light"], im["size"], im["center"], im["local"],
+                                           im["bg"], im["contours"], ())
+            assert np.array_equal(got[q][0], ref_rgb), q
+            assert same_bits(got[q][1], ref_dep), q
+    assert drawn > 500
+
+
 @pytest.mark.parametrize("precision", [api.FP64, api.FP32], ids=["fp64", "fp32"])
 def test_cameras_sharing_one_scene_tessellate_once(orc, precision):
     banner(f"batch shared scene precision={precision}")
