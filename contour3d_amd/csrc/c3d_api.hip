@@ -4,6 +4,7 @@
 #include "contour3d.h"
 #include "contour3d_amd.h"
 #include "c3d_internal.h"
+#include "c3d_wait.h"
 
 #include <cerrno>
 #include <cfloat>
@@ -1553,14 +1554,14 @@ void server_loop(Lib& L, Server& S) {
       return (!S.flights.empty() && S.flights.front()->done) || can_start() ||
              (S.stop && S.queue.empty() && S.flights.empty());
     };
-    if (!S.queue.empty() && wait_s > 0 && S.flights.empty()) {
-      const auto until = std::chrono::steady_clock::now() +
-                         std::chrono::duration_cast<std::chrono::steady_clock::duration>(std::chrono::duration<double>(
-                             std::max(0.0, S.queue.front()->t_submit + wait_s - now_s())));
-      S.cv_work.wait_until(lk, until, pred);
-    } else {
-      S.cv_work.wait(lk, pred);
-    }
+    // while the device is idle, the oldest request's window bounds the wait
+    wait_work(lk, S.cv_work, pred, [&](std::chrono::steady_clock::time_point& until) {
+      if (S.queue.empty() || wait_s <= 0 || !S.flights.empty()) return false;
+      until = std::chrono::steady_clock::now() +
+              std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                  std::chrono::duration<double>(std::max(0.0, S.queue.front()->t_submit + wait_s - now_s())));
+      return true;
+    });
     if (!S.flights.empty() && S.flights.front()->done) {
       std::unique_ptr<Flight> f = std::move(S.flights.front());
       S.flights.pop_front();

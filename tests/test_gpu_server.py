@@ -56,6 +56,27 @@ def test_served_images_equal_images_alone(max_images, budget):
     assert st["peak_admitted_bytes"] > 0
 
 
+def test_batching_window_starts_lone_requests_and_bursts():
+    """With max_wait_ms > 0 an idle worker holds a request until its window
+    ends, for company; a lone request and the last of a burst must still
+    start then, with nothing else arriving."""
+    banner("server: batching window")
+    reqs = [(im, api.FP32) for im, _ in mixed_requests()[:5]]
+    expect = [alone(im, p) for im, p in reqs]
+    with api.Server(max_images=16, max_wait_ms=20.0) as srv:
+        rgb, dep, lat = srv.wait(srv.submit(*reqs[0]))
+        assert np.array_equal(rgb, expect[0][0]) and same_bits(dep, expect[0][1])
+        assert 15.0 <= lat < 5000.0  # held for its window, then rendered
+        tickets = [srv.submit(im, p) for im, p in reqs[1:]]
+        got = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    for q, ((rgb, dep, _), (ergb, edep)) in enumerate(zip(got, expect[1:])):
+        assert np.array_equal(rgb, ergb), q
+        assert same_bits(dep, edep), q
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert st["batches"] <= 3 and st["mean_batch"] > 1.0  # the burst shared frames
+
+
 def test_device_fields_updated_in_place_between_requests():
     banner("server: device field updated in place")
     import torch
