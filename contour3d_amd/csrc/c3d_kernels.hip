@@ -1166,7 +1166,10 @@ __global__ void k_frag_keys(const Frag* __restrict__ frags, uint32_t cap, const 
   const uint32_t n = min(hdr->nfrag, cap);
   for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < n; f += gridDim.x * blockDim.x) {
     const Frag fr = frags[f];
-    atomicMax(reinterpret_cast<unsigned long long*>(key + fr.px), (unsigned long long)fr.key);
+    // keys only grow, so a value read here (however stale) that is at least
+    // this key means the atomic could not change the pixel: skip it
+    if (fr.key > key[fr.px])
+      atomicMax(reinterpret_cast<unsigned long long*>(key + fr.px), (unsigned long long)fr.key);
   }
 }
 
