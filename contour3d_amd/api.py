@@ -32,6 +32,7 @@ EXPORTS = (
     "contour3d_amd_execute_batch",
     "contour3d_amd_server_start",
     "contour3d_amd_server_submit",
+    "contour3d_amd_server_submit_ppm",
     "contour3d_amd_server_wait",
     "contour3d_amd_server_stats",
     "contour3d_amd_server_stop",
@@ -78,6 +79,8 @@ def lib():
         L.contour3d_amd_tune.argtypes = [C.c_int, C.c_int64]
         L.contour3d_amd_server_start.argtypes = [C.POINTER(_abi.ServerOpts)]
         L.contour3d_amd_server_submit.argtypes = [C.POINTER(_abi.Image), C.c_int, C.POINTER(C.c_uint64)]
+        L.contour3d_amd_server_submit_ppm.argtypes = [C.POINTER(_abi.Image), C.c_int, C.c_char_p,
+                                                      C.POINTER(C.c_uint64)]
         L.contour3d_amd_server_wait.argtypes = [C.c_uint64, C.POINTER(C.c_double)]
         L.contour3d_amd_server_stats.argtypes = [C.POINTER(_abi.ServerStats)]
         L.contour3d_amd_init.argtypes = [C.c_int]
@@ -303,6 +306,25 @@ class Server:
         if lib().contour3d_amd_server_submit(C.byref(e), int(precision), C.byref(t)) != 0:
             raise RuntimeError("contour3d_amd_server_submit failed")
         self._pending[t.value] = (e, keep, out)
+        return t.value
+
+    def submit_ppm(self, image, fname, precision=FP64):
+        """As submit, but the server writes the image to `fname` as a binary
+        PPM (the bytes contour3d_execute writes) on its writer thread, while
+        it renders the next frames; wait(ticket) returns (None, depth,
+        latency_ms) once the file is written."""
+        keep = _Keep()
+        e = _abi.Image()
+        made = {}
+        # the server renders into a pinned image of its own
+        out = _fill_image(e, dict(image, rgb=np.zeros((1, 1, 3), np.uint8)), keep, made)
+        keep.items.append(made)
+        path = os.fsencode(fname)
+        keep.items.append(path)
+        t = C.c_uint64()
+        if lib().contour3d_amd_server_submit_ppm(C.byref(e), int(precision), C.c_char_p(path), C.byref(t)) != 0:
+            raise RuntimeError("contour3d_amd_server_submit_ppm failed")
+        self._pending[t.value] = (e, keep, (None, out[1]))
         return t.value
 
     def wait(self, ticket):

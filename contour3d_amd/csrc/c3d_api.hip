@@ -17,6 +17,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <string>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -1439,6 +1440,9 @@ struct Request {
   uint64_t bytes = 0;  // admission estimate
   double t_submit = 0, t_done = 0;
   int status = -1;  // -1 pending, 0 done, 1 failed
+  std::string fname;          // PPM output, written by the server's writer thread
+  uint8_t* pinned = nullptr;  // the pinned host image it is rendered into
+  size_t pinned_bytes = 0;
 };
 
 struct Flight {
@@ -1468,7 +1472,45 @@ struct Server {
   std::vector<double> lat_ms;
   uint64_t completed = 0, failed = 0, batches = 0, images = 0, peak = 0;
   double t_first = 0, t_last = 0;
+  // PPM requests: rendered frames wait here for the writer thread
+  std::thread writer;
+  std::condition_variable cv_write;
+  std::deque<std::shared_ptr<Request>> to_write;
+  bool writer_stop = false;
+  // pinned host images of PPM requests, reused (allocated with hipHostMalloc
+  // by submit, freed by stop)
+  std::mutex pool_mu;
+  std::vector<std::pair<uint8_t*, size_t>> pinned_free, pinned_all;
 };
+
+uint8_t* pinned_take(Server& S, size_t bytes, size_t& got) {
+  {
+    std::lock_guard<std::mutex> g(S.pool_mu);
+    size_t best = S.pinned_free.size();
+    for (size_t k = 0; k < S.pinned_free.size(); k++)
+      if (S.pinned_free[k].second >= bytes && (best == S.pinned_free.size() || S.pinned_free[k].second <
+                                                                                    S.pinned_free[best].second))
+        best = k;
+    if (best < S.pinned_free.size()) {
+      const auto b = S.pinned_free[best];
+      S.pinned_free.erase(S.pinned_free.begin() + long(best));
+      got = b.second;
+      return b.first;
+    }
+  }
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p) return nullptr;
+  std::lock_guard<std::mutex> g(S.pool_mu);
+  S.pinned_all.push_back({static_cast<uint8_t*>(p), bytes});
+  got = bytes;
+  return static_cast<uint8_t*>(p);
+}
+
+void pinned_give(Server& S, uint8_t* p, size_t bytes) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(S.pool_mu);
+  S.pinned_free.push_back({p, bytes});
+}
 
 std::mutex g_server_mu;  // server start/stop against submit/wait/stats
 
@@ -1516,6 +1558,14 @@ void complete(Server& S, Flight& f, int rc, double tri_total) {
 void complete_locked(Server& S, Flight& f, int rc, double tri_total) {
   const double t = now_s();
   for (auto& r : f.reqs) {
+    if (!rc && !r->fname.empty()) {
+      // rendered into its pinned image: completes once the file is written
+      S.to_write.push_back(r);
+      S.cv_write.notify_one();
+      continue;
+    }
+    if (r->pinned) pinned_give(S, r->pinned, r->pinned_bytes);
+    r->pinned = nullptr;
     r->status = rc ? 1 : 0;
     r->t_done = t;
     if (rc) {
@@ -1633,6 +1683,36 @@ void server_loop(Lib& L, Server& S) {
       continue;
     }
     if (S.stop && S.queue.empty() && S.flights.empty()) break;
+  }
+}
+
+int write_ppm(const char* fname, size_t W, size_t H, const uint8_t* rgb);
+
+// Writes the PPM files of rendered requests, in completion order, while the
+// worker renders the next frames; each request completes when its file is
+// written.  Calls no HIP function.
+void writer_loop(Server& S) {
+  std::unique_lock<std::mutex> lk(S.m);
+  for (;;) {
+    S.cv_write.wait(lk, [&] { return !S.to_write.empty() || S.writer_stop; });
+    if (S.to_write.empty()) break;
+    std::shared_ptr<Request> r = S.to_write.front();
+    S.to_write.pop_front();
+    lk.unlock();
+    const int rc = write_ppm(r->fname.c_str(), r->img.screen_sizes[0], r->img.screen_sizes[1], r->pinned);
+    pinned_give(S, r->pinned, r->pinned_bytes);
+    lk.lock();
+    r->pinned = nullptr;
+    r->status = rc ? 1 : 0;
+    r->t_done = now_s();
+    if (rc) {
+      S.failed++;
+    } else {
+      S.completed++;
+      S.lat_ms.push_back((r->t_done - r->t_submit) * 1e3);
+    }
+    S.t_last = r->t_done;
+    S.cv_done.notify_all();
   }
 }
 
@@ -2027,33 +2107,52 @@ int contour3d_amd_server_start(const contour3d_amd_server_opts_t* opts) {
     L.server = S;
   }
   S->worker = std::thread(c3d::server_loop, std::ref(L), std::ref(*S));
+  S->writer = std::thread(c3d::writer_loop, std::ref(*S));
   return 0;
 }
 
-int contour3d_amd_server_submit(const contour3d_amd_image_t* image, int precision, uint64_t* ticket) {
+namespace c3d {
+
+// submit and submit_ppm: queues one request (fname: PPM output)
+int server_submit(const contour3d_amd_image_t* image, int precision, const char* fname, uint64_t* ticket) {
   auto& L = lib();
-  std::lock_guard<std::mutex> gs(c3d::g_server_mu);
-  c3d::Server* S = L.server;
+  std::lock_guard<std::mutex> gs(g_server_mu);
+  Server* S = L.server;
   if (!S || !image || !ticket) {
-    c3d::log_error("server_submit: server not running or NULL argument");
+    log_error("server_submit: server not running or NULL argument");
     return 1;
   }
   if (precision != CONTOUR3D_AMD_FP64 && precision != CONTOUR3D_AMD_FP32) {
-    c3d::log_error("unknown precision %d", precision);
+    log_error("unknown precision %d", precision);
     return 1;
   }
   if (image->num_contours && !image->contour_objs) {
-    c3d::log_error("server_submit: contour objects pointer is NULL");
+    log_error("server_submit: contour objects pointer is NULL");
     return 1;
   }
-  auto r = std::make_shared<c3d::Request>();
+  auto r = std::make_shared<Request>();
   r->img = *image;
+  if (fname) {
+    const size_t bytes = size_t(image->screen_sizes[0]) * size_t(image->screen_sizes[1]) * 3;
+    if (!*fname || bytes == 0) {
+      log_error("server_submit_ppm: empty file name or image");
+      return 1;
+    }
+    r->fname = fname;
+    r->pinned = pinned_take(*S, bytes, r->pinned_bytes);
+    if (!r->pinned) {
+      log_error("server_submit_ppm: cannot allocate a pinned image of %zu bytes", bytes);
+      return 1;
+    }
+    r->img.rgb = r->pinned;
+  }
   r->precision = precision;
-  r->bytes = c3d::estimate_bytes(*S, *image, precision);
-  r->t_submit = c3d::now_s();
+  r->bytes = estimate_bytes(*S, *image, precision);
+  r->t_submit = now_s();
   std::lock_guard<std::mutex> g(S->m);
   if (S->stop) {
-    c3d::log_error("server_submit: server stopping");
+    log_error("server_submit: server stopping");
+    pinned_give(*S, r->pinned, r->pinned_bytes);
     return 1;
   }
   r->ticket = S->next_ticket++;
@@ -2063,6 +2162,21 @@ int contour3d_amd_server_submit(const contour3d_amd_image_t* image, int precisio
   *ticket = r->ticket;
   S->cv_work.notify_all();
   return 0;
+}
+
+}  // namespace c3d
+
+int contour3d_amd_server_submit(const contour3d_amd_image_t* image, int precision, uint64_t* ticket) {
+  return c3d::server_submit(image, precision, nullptr, ticket);
+}
+
+int contour3d_amd_server_submit_ppm(const contour3d_amd_image_t* image, int precision, const char* fname,
+                                    uint64_t* ticket) {
+  if (!fname) {
+    c3d::log_error("server_submit_ppm: NULL file name");
+    return 1;
+  }
+  return c3d::server_submit(image, precision, fname, ticket);
 }
 
 int contour3d_amd_server_wait(uint64_t ticket, double* latency_ms) {
@@ -2129,12 +2243,20 @@ int contour3d_amd_server_stop(void) {
     S->cv_work.notify_all();
   }
   S->worker.join();
+  // every rendered PPM request is queued for the writer by now
+  {
+    std::lock_guard<std::mutex> g(S->m);
+    S->writer_stop = true;
+    S->cv_write.notify_all();
+  }
+  S->writer.join();
   {
     std::lock_guard<std::mutex> gs(c3d::g_server_mu);
     std::lock_guard<std::mutex> g(L.mu);
     L.server = nullptr;
   }
   for (auto& F : S->frames) F.release();
+  for (auto& b : S->pinned_all) (void)hipHostFree(b.first);
   delete S;
   return 0;
 }

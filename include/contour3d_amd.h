@@ -133,6 +133,17 @@ typedef struct {
  */
 extern int contour3d_amd_server_start(const contour3d_amd_server_opts_t * opts);
 extern int contour3d_amd_server_submit(const contour3d_amd_image_t * image, int precision, uint64_t * ticket);
+/* As _submit, but the image goes to `fname` as the binary PPM that
+ * contour3d_execute writes (P6, output.c:125-140); image->rgb is ignored.
+ * The server renders into a pinned host image of its own, and a writer
+ * thread writes the file once the frame's copies are done, so the file I/O
+ * of one frame overlaps the rendering of the next.  The wait returns once
+ * the file is written; 1 if it could not be.  No counterpart in the
+ * reference, whose contour3d_execute writes the file before it returns
+ * (output.c:113-149): it is the asynchronous output path for a serving
+ * process. */
+extern int contour3d_amd_server_submit_ppm(const contour3d_amd_image_t * image, int precision, const char * fname,
+                                           uint64_t * ticket);
 /* Blocks until the request is done; 0 = rendered, 1 = failed.  Each ticket
  * is waited for once. */
 extern int contour3d_amd_server_wait(uint64_t ticket, double * latency_ms);

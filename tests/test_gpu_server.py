@@ -77,6 +77,29 @@ def test_batching_window_starts_lone_requests_and_bursts():
     assert st["batches"] <= 3 and st["mean_batch"] > 1.0  # the burst shared frames
 
 
+def test_ppm_requests_write_the_files_contour3d_execute_writes(tmp_path):
+    """submit_ppm: the writer thread writes each image as contour3d_execute
+    writes its file (P6 header, then the rows top first, output.c:125-140);
+    a file that cannot be written fails that request only."""
+    banner("server: PPM output")
+    reqs = mixed_requests()[:6]
+    expect = [alone(im, p) for im, p in reqs]
+    paths = [tmp_path / f"r{q}.ppm" for q in range(len(reqs))]
+    with api.Server(max_images=4) as srv:
+        tickets = [srv.submit_ppm(im, str(path), p) for (im, p), path in zip(reqs, paths)]
+        bad = srv.submit_ppm(reqs[0][0], str(tmp_path / "missing" / "x.ppm"), reqs[0][1])
+        got = [srv.wait(t) for t in tickets]
+        with pytest.raises(RuntimeError):
+            srv.wait(bad)
+        st = srv.stats()
+    for q, ((im, _), path, (rgb, dep, lat), (ergb, edep)) in enumerate(zip(reqs, paths, got, expect)):
+        W, H = im["size"]
+        assert path.read_bytes() == f"P6\n{W} {H}\n255\n".encode() + np.ascontiguousarray(ergb).tobytes(), q
+        assert rgb is None and same_bits(dep, edep), q
+        assert lat > 0
+    assert st["completed"] == len(reqs) and st["failed"] == 1
+
+
 def test_device_fields_updated_in_place_between_requests():
     banner("server: device field updated in place")
     import torch
