@@ -20,6 +20,9 @@ Beside it, in the same JSON line:
               steps), 1-4 thresholds, images 512^2..2048^2, random cameras;
               p50/p99 latency and frames/s of the serving loop, next to a
               plain loop that renders the same arrivals one at a time
+  served_ppm  the output path: images written as PPM files by a loop of
+              contour3d_execute against PPM requests to the serving loop,
+              whose writer thread overlaps file I/O with the next frames
   roofline    HBM roofline of the dominant kernel of the FP64 frame: the
               fused tessellation + raster stage, charged with the bytes of
               the stages it replaces (B_tess + B_raster below)
@@ -193,6 +196,49 @@ def shared_scene(api, fields, contours, reps):
     res["batched_fp32_vs_plain_fp64"] = round(res["batched_fp32_fps"] / res["plain_fp64_fps"], 3)
     res["batched_fp32_vs_plain_fp32"] = round(res["batched_fp32_fps"] / res["plain_fp32_fps"], 3)
     return res
+
+
+def ppm_output(api, fields, contours, rounds=4):
+    """The output path: every image written as a binary PPM file.  A loop of
+    contour3d_execute (render, copy back, write the file, then return)
+    against PPM requests to the serving loop (rendered into pinned images;
+    the writer thread writes each file while the next frames render), both
+    FP64, the four cameras of the 512^3 scene `rounds` times.  Files go to a
+    temporary directory, removed afterwards; one pair is compared byte for
+    byte."""
+    import shutil
+    import tempfile
+
+    views = [fields.example_view(W, H)] + [fields.sphere_view(e, a, W, H) for e, a in ((0.5, 0.7), (0.2, 2.6),
+                                                                                      (0.9, 4.4))]
+    n = rounds * len(views)
+    d = tempfile.mkdtemp(prefix="c3d_ppm_")
+    try:
+        def plain():
+            for q in range(n):
+                v = views[q % len(views)]
+                api.execute(v["camera"], v["look_at"], v["light"], v["size"], v["center"], v["local"],
+                            v.get("bg", (0, 0, 0)), contours, fname=os.path.join(d, f"p{q}.ppm"))
+
+        imgs = [_img(v, contours) for v in views]
+        api.set_precision(api.FP64)
+        plain()
+        t_plain = min(_timed(plain, 1) for _ in range(2))
+        with api.Server(max_images=16) as srv:
+            def served():
+                ts = [srv.submit_ppm(imgs[q % len(imgs)], os.path.join(d, f"s{q}.ppm"), api.FP64) for q in range(n)]
+                for t in ts:
+                    srv.wait(t)
+            served()
+            t_served = min(_timed(served, 1) for _ in range(2))
+        with open(os.path.join(d, "p0.ppm"), "rb") as a, open(os.path.join(d, "s0.ppm"), "rb") as b:
+            same = a.read() == b.read()
+        header = len(f"P6\n{W} {H}\n255\n")
+        return {"files": n, "image": [W, H], "precision": "fp64", "bytes_per_file": 3 * W * H + header,
+                "contour3d_execute_files_per_s": round(n / t_plain, 2),
+                "served_ppm_files_per_s": round(n / t_served, 2), "identical_files": same}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
 
 
 def uneven_batch_frame(api, fields, nfields, side, reps, seed=0):
@@ -579,6 +625,8 @@ def main():
     if "served" in forms:
         log(f"served: {args.served_requests} Poisson arrivals at 50/s")
         out["served"] = form("served", lambda: served_stream(api, fields, args.served_requests, 50.0, 8))
+        log("served: PPM files, contour3d_execute loop and served PPM requests")
+        out["served_ppm"] = form("served_ppm", lambda: ppm_output(api, fields, contours))
     if "precision" in forms:
         log(f"precision study: {args.precision_n}^3, 2048^2")
         out["precision_study"] = form("precision", lambda: precision_study(api, fields, args.precision_n, 2048))
