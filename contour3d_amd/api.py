@@ -447,7 +447,7 @@ def tune(param, value):
         raise RuntimeError("contour3d_amd_tune failed")
 
 
-def autotune(run, tess=(0, 2048, 4096, 8192), raster=(0, 4096, 16384, 32768), reps=2):
+def autotune(run, tess=(0, 8192, 65536, 1 << 30), raster=(0, 4096, 16384, 32768), reps=2):
     """Times run() under each tessellation and raster workgroup cap (results
     are the same bits under every setting) and keeps the fastest pair.
     Returns ({"tess": t, "raster": r}, best_ms)."""

@@ -222,7 +222,9 @@ extern int contour3d_amd_mesh_free(contour3d_amd_mesh_t * mesh);
 /* Launch-shape and capacity parameters.  None changes a result: every
  * setting renders the same bits. */
 enum {
-  CONTOUR3D_AMD_TUNE_TESS_GRID = 0,     /* max workgroups of count/emit (0 = one per tile) */
+  CONTOUR3D_AMD_TUNE_TESS_GRID = 0,     /* max workgroups of the tessellation kernels (count, the fused
+                                           tessellation + raster, emit); each walks several tiles
+                                           (0: the defaults, 128 per CU, and one per tile for emit) */
   CONTOUR3D_AMD_TUNE_RASTER_GRID = 1,   /* max workgroups of the raster passes (0 = one per 256 triangles) */
   CONTOUR3D_AMD_TUNE_FRAG_CAPACITY = 2, /* fragment list entries, 16 B each (0: the default, 2^24; grows by
                                            itself after an overflow; an overflowing frame takes the slower
