@@ -224,6 +224,7 @@ struct Frame {
   Prof prof;
   bool fp64 = true;         // form of the frame in flight
   FrameHdr last{};          // header of the last finished frame
+  bool unread = false;      // the frame in flight was planned with no read back of its totals
   template <typename R>
   std::vector<RasterJob<R>>& rjobs_host();
   hipError_t create() {
@@ -1078,6 +1079,12 @@ int frame_finish(Frame& F) {
   C3D_HIP(e);
   FrameHdr& fin = F.last;
   std::memcpy(&fin, F.hstage.p, sizeof(FrameHdr));
+  if (F.unread && !(fin.overflow & 1u)) {
+    // totals of a frame planned without a read back, from its final header
+    F.mesh.total = fin.ntri;
+    F.prof.ntri += fin.ntri;
+  }
+  F.unread = false;
   if ((fin.overflow & 2u) && fin.nfrag > F.frag_cap) {
     const uint64_t grow = uint64_t(fin.nfrag) + fin.nfrag / 4;
     F.frag_cap = uint32_t(grow > 0xffffffffull ? 0xffffffffull : grow);
@@ -1291,13 +1298,16 @@ int graph_frame(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& img
   }
   C3D_HIP(hipGraphLaunch(F.gexec, F.st));
   replayed = true;
+  F.unread = true;
   return raster_copies(F, imgs, fp64);
 }
 
 // Tessellate the objects of every image and render them all.
 template <typename R>
-int run_frame_t(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& imgs, bool graph, bool& replayed) {
+int run_frame_t(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& imgs, bool graph, bool& replayed,
+                bool speculate) {
   replayed = false;
+  F.unread = false;
   uint64_t npx = 0;
   if (frame_setup(F, imgs, npx)) return 1;
   // the frame's object list: each image's objects, unless an earlier image
@@ -1333,6 +1343,17 @@ int run_frame_t(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& img
   FrameHdr hh{};
   if (tess_prepare<R>(L, F, M, objs)) return 1;
   if (upload_pass_images(F, imgs)) return 1;
+  if (speculate && M.cap > 0 && M.cell.p && M.cap < kMaxFrameTriangles) {
+    // planned against the cell store that earlier frames reserved, with no
+    // read back of the totals (no synchronisation mid-frame); the plan
+    // flags a frame that needs more, and run_frame renders it again
+    if (tess_head<R>(L, F, M, rjd, njobs, M.cap)) return 1;
+    if (raster_prepare<R>(F, npx)) return 1;
+    if (fused_launch<R>(L, F, M, rjd, njobs, npx, F.h_nline)) return 1;
+    replayed = true;
+    F.unread = true;
+    return raster_copies(F, imgs, sizeof(R) == 8);
+  }
   if (tess_head<R>(L, F, M, rjd, njobs, ~uint64_t(0))) return 1;
   if (tess_readback<R>(F, M, hh, true)) return 1;
   if (raster_prepare<R>(F, npx)) return 1;
@@ -1354,8 +1375,11 @@ int run_frame(Lib& L, Frame& F, std::vector<Image>& imgs, int precision, bool wa
   for (int attempt = 0; attempt < 2; attempt++) {
     bool replayed = false;
     const bool g = graph && attempt == 0;
-    const int rc = precision == CONTOUR3D_AMD_FP64 ? run_frame_t<double>(L, F, F.mesh, imgs, g, replayed)
-                                                   : run_frame_t<float>(L, F, F.mesh, imgs, g, replayed);
+    // a waited frame skips the mid-frame read back while the reserved
+    // capacity holds it; a frame that outgrows it is rendered again
+    const bool spec = wait && attempt == 0;
+    const int rc = precision == CONTOUR3D_AMD_FP64 ? run_frame_t<double>(L, F, F.mesh, imgs, g, replayed, spec)
+                                                   : run_frame_t<float>(L, F, F.mesh, imgs, g, replayed, spec);
     if (rc) {
       (void)hipStreamSynchronize(F.st);
       release_geoms(F);
@@ -1368,7 +1392,7 @@ int run_frame(Lib& L, Frame& F, std::vector<Image>& imgs, int precision, bool wa
       return 1;
     }
     if (!(replayed && (F.last.overflow & 1u))) return 0;
-    F.drop_graph();  // the mesh must grow: render again eagerly
+    F.drop_graph();  // the cell store must grow: render again, with the read back
   }
   return 0;
 }
