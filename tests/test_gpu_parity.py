@@ -294,6 +294,38 @@ def test_errors_return_one():
                     [dict(field=g, threshold=0.5, color=(1, 1, 1))])
 
 
+def test_frame_outgrowing_the_reserved_cells_renders_again(orc):
+    """A waited frame plans against the cell store that earlier frames
+    reserved, with no read back of its totals; a frame that needs more is
+    flagged by the plan and rendered again with the read back.  After a
+    finalize (nothing reserved), a small frame, then one with far more
+    triangles: it must equal the oracle, and the profiled triangle count of
+    the re-rendered frame must equal that of the next, which fits."""
+    banner("frame outgrowing the reserved cell store")
+    api.finalize()
+    v = view_for(96, 72)
+    small = [dict(field=fields.blobs((12, 12, 12), seed=3), threshold=0.5, color=(200, 60, 30))]
+    big = [dict(field=fields.turbulence((40, 40, 40), kmax=12, seed=8), threshold=t, color=c)
+           for t, c in ((-0.8, (255, 0, 0)), (0.0, (0, 255, 0)), (0.8, (0, 0, 255)))]
+    api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], small,
+                    precision=api.FP64)
+    api.profile_enable(True)
+    try:
+        counts = []
+        for _ in range(2):
+            api.profile_reset()
+            rgb, dep = api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], big,
+                                       precision=api.FP64, want_depth=True)
+            counts.append(api.profile_read()[2])
+            ref_rgb, ref_dep = orc.execute(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"],
+                                           big)
+            assert np.array_equal(rgb, ref_rgb)
+            assert same_bits(dep, ref_dep)
+    finally:
+        api.profile_enable(False)
+    assert counts[0] == counts[1] > 10000
+
+
 def test_large_triangles_fill_the_fragment_lists(orc):
     """A 9^3 field at 320x240: each triangle covers hundreds of pixels, so
     the workgroups' LDS fragment lists overflow into the frame's list.  FP64
