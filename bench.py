@@ -92,23 +92,32 @@ def mesh_tessellation(api, contours, precision, reps):
 
 
 def run_form(api, contours, view, precision, steps, warmup, rgb_dev):
+    """`warmup` untimed frames, then `steps` timed frames with the stage
+    profiler off (as a caller runs them), then `steps` more with it on for
+    the stage times and the triangle count.  Returns (seconds for the timed
+    frames, ms per frame by stage, triangles per frame)."""
     import torch
 
-    for _ in range(warmup):
+    def frame():
         api.execute_rgb(view["camera"], view["light"], view["size"], view["center"], view["local"], view["bg"],
                         contours, precision=precision, rgb=rgb_dev)
-    api.profile_reset()
-    api.profile_enable(True)
+
+    for _ in range(warmup):
+        frame()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        api.execute_rgb(view["camera"], view["light"], view["size"], view["center"], view["local"], view["bg"],
-                        contours, precision=precision, rgb=rgb_dev)
+        frame()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    api.profile_reset()
+    api.profile_enable(True)
+    for _ in range(steps):
+        frame()
+    torch.cuda.synchronize()
     api.profile_enable(False)
     ms, calls, ntri = api.profile_read()
-    return dt, ms, calls, ntri
+    return dt, {k: v / max(1, steps) for k, v in ms.items()}, ntri / max(1, steps)
 
 
 def _slab_rank(path, z0, planes, view, barrier, out):
@@ -601,13 +610,12 @@ def main():
     results = {}
     for name, prec in (("fp64", api.FP64), ("fp32", api.FP32)):
         print(f"[bench] {name}: {args.warmup} warmup + {args.steps} timed frames", file=sys.stderr, flush=True)
-        dt, ms, calls, ntri = run_form(api, contours, view, prec, args.steps, args.warmup, rgb_dev)
-        T = ntri / max(1, args.steps)
+        dt, ms, T = run_form(api, contours, view, prec, args.steps, args.warmup, rgb_dev)
         stages = {}
         for s in api.STAGES:
             if s in ("smooth", "emit"):
                 continue  # frames store no mesh: the fused raster stage tessellates
-            per = ms[s] / max(1, args.steps)
+            per = ms[s]
             b = algorithmic_bytes("fused" if s == "raster" else s, T, npts, npx, prec == api.FP64)
             stages[s] = dict(ms=round(per, 4), gbs=round(b / (per * 1e-3) / 1e9, 1) if per > 0 and b else None)
         results[name] = dict(dt=dt, T=T, stages=stages, frame_ms=dt / args.steps * 1e3)
