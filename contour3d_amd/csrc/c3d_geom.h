@@ -73,6 +73,43 @@ C3D_HD void set_pitch(ViewDev<R>& v) {
   v.pitch = fmax(R(1) / v.fw, R(1) / v.fh);
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// a / b in FP64 for a divisor b shared by many numerators (a view
+// constant).  The compiler's IEEE division is v_div_scale of the
+// denominator, v_rcp and two Newton steps on it, then v_div_scale of the
+// numerator, a product, an FMA, v_div_fmas and v_div_fixup.  Here the
+// denominator half is recip_shared(b), computed once for all numerators,
+// and the rest is the same instructions.  The denominator's v_div_scale
+// depends on a too: it leaves b unchanged except at extreme exponent
+// ratios, which is checked per numerator, and any other case takes the
+// plain division.  Every result therefore equals a / b bit for bit.
+__device__ __forceinline__ double recip_shared(double b) {
+  const double r0 = __builtin_amdgcn_rcp(b);
+  const double e0 = __builtin_fma(-b, r0, 1.0);
+  const double r1 = __builtin_fma(r0, e0, r0);
+  const double e1 = __builtin_fma(-b, r1, 1.0);
+  return __builtin_fma(r1, e1, r1);
+}
+
+__device__ __forceinline__ double div_shared(double a, double b, double rb) {
+  bool fd, fn;
+  if (__builtin_amdgcn_div_scale(a, b, false, &fd) != b) return a / b;
+  const double n = __builtin_amdgcn_div_scale(a, b, true, &fn);
+  const double q0 = n * rb;
+  const double rem = __builtin_fma(-b, q0, n);
+  return __builtin_amdgcn_div_fixup(__builtin_amdgcn_div_fmas(rem, rb, q0, fn), b, a);
+}
+#endif
+
+// a / b, b a view constant (see div_shared)
+template <typename R>
+C3D_HD R div_view(R a, R b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(R) == 8) return div_shared(a, b, recip_shared(b));
+#endif
+  return a / b;
+}
+
 // perspective projection (project.c:9-67); false when rejected
 template <typename R>
 C3D_HD bool project(const ViewDev<R>& v, V3<R> p, V3<R>& q) {
@@ -87,8 +124,8 @@ C3D_HD bool project(const ViewDev<R>& v, V3<R> p, V3<R>& q) {
                        v.cam[2] + t * ray.z - v.ctr[2]};
   const V3<R> lx = {v.lx[0], v.lx[1], v.lx[2]};
   const V3<R> ly = {v.ly[0], v.ly[1], v.ly[2]};
-  q.x = vdot(delta, lx) / v.lxx;
-  q.y = vdot(delta, ly) / v.lyy;
+  q.x = div_view(vdot(delta, lx), v.lxx);
+  q.y = div_view(vdot(delta, ly), v.lyy);
   q.z = nr;
   return true;
 }
