@@ -45,7 +45,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch past L2 of the FP64 frame's stages at 512^3 (rocprofv3
 # --pmc FETCH_SIZE / WRITE_SIZE, scaled by a plain copy's readings); filled
 # in from the profile summary under profiles/
-MEASURED_TRAFFIC_512 = {"raster": 5.256e9, "count": 2.257e9, "resolve": 0.351e9}
+MEASURED_TRAFFIC_512 = {"raster": 4.846e9, "count": 2.258e9, "resolve": 0.359e9}
 N = 512
 THRESHOLDS = (-1.0, 0.0, 1.0)
 COLORS = ((255, 80, 40), (40, 220, 90), (60, 120, 255))
