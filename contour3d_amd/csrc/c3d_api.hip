@@ -2163,11 +2163,15 @@ int server_submit(const contour3d_amd_image_t* image, int precision, const char*
   auto r = std::make_shared<Request>();
   r->img = *image;
   if (fname) {
-    const size_t bytes = size_t(image->screen_sizes[0]) * size_t(image->screen_sizes[1]) * 3;
-    if (!*fname || bytes == 0) {
-      log_error("server_submit_ppm: empty file name or image");
+    // the frame's image limits, checked before the pinned image is taken
+    const size_t W = image->screen_sizes[0], H = image->screen_sizes[1];
+    if (!*fname || W == 0 || H == 0 || W > (size_t(1) << 16) || H > (size_t(1) << 16) ||
+        uint64_t(W) * uint64_t(H) >= (uint64_t(1) << 31)) {
+      log_error("server_submit_ppm: empty file name, or image size %zux%zu outside 1..65536 per axis and "
+                "2^31 pixels", W, H);
       return 1;
     }
+    const size_t bytes = W * H * 3;
     r->fname = fname;
     r->pinned = pinned_take(*S, bytes, r->pinned_bytes);
     if (!r->pinned) {

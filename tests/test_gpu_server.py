@@ -88,6 +88,8 @@ def test_ppm_requests_write_the_files_contour3d_execute_writes(tmp_path):
     with api.Server(max_images=4) as srv:
         tickets = [srv.submit_ppm(im, str(path), p) for (im, p), path in zip(reqs, paths)]
         bad = srv.submit_ppm(reqs[0][0], str(tmp_path / "missing" / "x.ppm"), reqs[0][1])
+        with pytest.raises(RuntimeError):  # over the frame's limits: rejected before any pinned image is taken
+            srv.submit_ppm(dict(reqs[0][0], size=(70000, 8)), str(tmp_path / "big.ppm"), reqs[0][1])
         got = [srv.wait(t) for t in tickets]
         with pytest.raises(RuntimeError):
             srv.wait(bad)
