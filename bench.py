@@ -689,7 +689,7 @@ def main():
         out["precision_study"] = form("precision", lambda: precision_study(api, fields, args.precision_n, 2048))
     if not args.no_cpu:
         log("cpu oracle baseline")
-        out["cpu_baseline"] = cpu_baseline(field, view)
+        out["cpu_baseline"] = form("cpu_baseline", lambda: cpu_baseline(field, view))
     print(json.dumps(out), flush=True)
 
 
