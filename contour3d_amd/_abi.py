@@ -76,7 +76,8 @@ class Image(C.Structure):
 class ServerOpts(C.Structure):
     """contour3d_amd_server_opts_t (no reference counterpart)."""
 
-    _fields_ = [("max_images", C.c_size_t), ("hbm_budget", C.c_uint64), ("max_wait_ms", C.c_double)]
+    _fields_ = [("max_images", C.c_size_t), ("hbm_budget", C.c_uint64), ("max_wait_ms", C.c_double),
+                ("snapshot_fields", C.c_int)]
 
 
 class ServerStats(C.Structure):

@@ -290,8 +290,8 @@ class Server:
     (rgb, depth, latency_ms).  The buffers behind a request are held here
     until it has been waited for."""
 
-    def __init__(self, max_images=16, hbm_budget=200e9, max_wait_ms=0.0):
-        o = _abi.ServerOpts(int(max_images), int(hbm_budget), float(max_wait_ms))
+    def __init__(self, max_images=16, hbm_budget=200e9, max_wait_ms=0.0, snapshot_fields=False):
+        o = _abi.ServerOpts(int(max_images), int(hbm_budget), float(max_wait_ms), 1 if snapshot_fields else 0)
         if lib().contour3d_amd_server_start(C.byref(o)) != 0:
             raise RuntimeError("contour3d_amd_server_start failed")
         self._pending = {}
@@ -438,6 +438,7 @@ TUNE_TESS_GRID = 0
 TUNE_RASTER_GRID = 1
 TUNE_FRAG_CAPACITY = 2
 TUNE_GRAPHS = 3
+TUNE_FAIL_ALLOC = 4
 
 
 def tune(param, value):

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 #include <atomic>
@@ -48,6 +49,19 @@ void log_error(const char* fmt, ...) {
     }                                                                                  \
   } while (0)
 
+// Failure injection for the error contract (CONTOUR3D_AMD_TUNE_FAIL_ALLOC):
+// while positive, every device or pinned allocation counts it down, and the
+// allocation that brings it from 1 to 0 fails as an out-of-memory would.
+std::atomic<int64_t> g_fail_alloc{0};
+
+bool injected_alloc_failure() {
+  int64_t v = g_fail_alloc.load();
+  while (v > 0) {
+    if (g_fail_alloc.compare_exchange_weak(v, v - 1)) return v == 1;
+  }
+  return false;
+}
+
 // grow-only device buffer
 struct DevBuf {
   void* p = nullptr;
@@ -65,6 +79,7 @@ struct DevBuf {
       if (e != hipSuccess) return e;
     }
     size_t n = exact ? want : want + want / 4 + 256;
+    if (injected_alloc_failure()) return hipErrorOutOfMemory;
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) {
       p = nullptr;
@@ -93,6 +108,7 @@ struct HostBuf {
     if (p) (void)hipHostFree(p);
     p = nullptr;
     bytes = 0;
+    if (injected_alloc_failure()) return hipErrorOutOfMemory;
     hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
     if (e != hipSuccess) {
       p = nullptr;
@@ -175,8 +191,6 @@ struct contour3d_amd_mesh {
 namespace c3d {
 namespace {
 
-// triangle ids are u32 and line fragments take ids after the triangles
-constexpr uint64_t kMaxFrameTriangles = (uint64_t(1) << 32) - (uint64_t(1) << 24);
 // initial fragment-list entries of a frame (16 B each): every covered pixel
 // of a 512^3 field's frame at 1920x1080 fits
 constexpr uint32_t kDefaultFragCap = 1u << 24;
@@ -424,6 +438,16 @@ int get_geometry(Lib& L, const contour3d_contour_obj_t& o, Geom** out) {
   g->host.assign(gp[0], gp[0] + nx);
   g->host.insert(g->host.end(), gp[1], gp[1] + ny);
   g->host.insert(g->host.end(), gp[2], gp[2] + nz);
+  // a geometry that fails half-built returns its device memory
+  struct Unbuilt {
+    Geom* g;
+    ~Unbuilt() {
+      if (g) {
+        g->grids.release();
+        g->table.release();
+      }
+    }
+  } unbuilt{g.get()};
   C3D_HIP(g->grids.ensure((nx + ny + nz) * sizeof(double)));
   C3D_HIP(hipMemcpy(g->grids.as<double>(), gp[0], nx * sizeof(double), hipMemcpyHostToDevice));
   C3D_HIP(hipMemcpy(g->grids.as<double>() + nx, gp[1], ny * sizeof(double), hipMemcpyHostToDevice));
@@ -464,6 +488,7 @@ int get_geometry(Lib& L, const contour3d_contour_obj_t& o, Geom** out) {
       C3D_HIP(hipMemcpy(g->table.p, tab.data(), 3 * npts * sizeof(double), hipMemcpyHostToDevice));
     }
   }
+  unbuilt.g = nullptr;
   g->stamp = ++L.stamp;
   // Keep up to kMaxGeoms geometries and kGeomTableBytes of coordinate
   // tables; evict the least recently used ones that no frame in flight
@@ -901,7 +926,7 @@ int tess_emit(Lib& L, Frame& F, contour3d_amd_mesh& M) {
 // Reads the plan's totals back (one synchronisation) and sizes the mesh
 // (only its cell indices for a frame).
 template <typename R>
-int tess_readback(Frame& F, contour3d_amd_mesh& M, FrameHdr& hh, bool cells_only = false) {
+int tess_readback(Frame& F, contour3d_amd_mesh& M, FrameHdr& hh, uint64_t nline, bool cells_only = false) {
   const size_t n = size_t(F.tl.nobj);
   unsigned char* hs = static_cast<unsigned char*>(F.hstage.p);
   C3D_HIP(hipMemcpyAsync(hs, F.hdr.p, sizeof(FrameHdr), hipMemcpyDeviceToHost, F.st));
@@ -917,8 +942,9 @@ int tess_readback(Frame& F, contour3d_amd_mesh& M, FrameHdr& hh, bool cells_only
   }
   M.total = hh.ntri;
   F.prof.ntri += M.total;
-  if (M.total >= kMaxFrameTriangles) {
-    log_error("%llu triangles exceed the 2^32 triangle ids of one frame", (unsigned long long)M.total);
+  if (!frame_ids_fit(M.total, nline)) {
+    log_error("%llu triangles and %llu line fragments exceed the 2^32 ids of one frame",
+              (unsigned long long)M.total, (unsigned long long)nline);
     return 1;
   }
   return cells_only ? ensure_cells(M) : ensure_capacity<R>(M);
@@ -932,8 +958,10 @@ template <typename R>
 int tessellate(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<const contour3d_contour_obj_t*>& objs,
                RasterJob<R>* rjobs, int njobs, FrameHdr& hh) {
   if (tess_prepare<R>(L, F, M, objs)) return 1;
+  // a mesh has no line fragments (the plan checks the frame's ids)
+  C3D_HIP(hipMemsetAsync(F.hdr.as<unsigned char>() + offsetof(FrameHdr, nline), 0, sizeof(uint32_t), F.st));
   if (tess_head<R>(L, F, M, rjobs, njobs, ~uint64_t(0))) return 1;
-  if (tess_readback<R>(F, M, hh)) return 1;
+  if (tess_readback<R>(F, M, hh, 0)) return 1;
   return tess_emit<R>(L, F, M);
 }
 
@@ -1060,7 +1088,6 @@ int raster_copies(Frame& F, const std::vector<Image>& imgs, bool fp64) {
 template <typename R>
 int raster_frame(Lib& L, Frame& F, const contour3d_amd_mesh& M, std::vector<Image>& imgs, const RasterJob<R>* rjobs,
                  uint64_t npx, const FrameHdr& hh) {
-  if (raster_prepare<R>(F, npx)) return 1;
   const uint32_t rgrid = raster_grid(L, sizeof(R) == 8, hh.nwork, false);
   if (raster_launch<R>(L, F, M, rjobs, int(imgs.size()), npx, rgrid, F.h_nline)) return 1;
   return raster_copies(F, imgs, sizeof(R) == 8);
@@ -1258,7 +1285,7 @@ int graph_frame(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& img
     F.drop_graph();
     FrameHdr hh{};
     if (tess_head<R>(L, F, M, rjd, njobs, ~uint64_t(0))) return 1;
-    if (tess_readback<R>(F, M, hh, true)) return 1;
+    if (tess_readback<R>(F, M, hh, F.h_nline, true)) return 1;
     if (fused_launch<R>(L, F, M, rjd, njobs, npx, F.h_nline)) return 1;
     return raster_copies(F, imgs, fp64);
   }
@@ -1343,20 +1370,20 @@ int run_frame_t(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& img
   FrameHdr hh{};
   if (tess_prepare<R>(L, F, M, objs)) return 1;
   if (upload_pass_images(F, imgs)) return 1;
+  // the line fragments' count goes to the header before the plan reads it
+  if (raster_prepare<R>(F, npx)) return 1;
   if (speculate && M.cap > 0 && M.cell.p && M.cap < kMaxFrameTriangles) {
     // planned against the cell store that earlier frames reserved, with no
     // read back of the totals (no synchronisation mid-frame); the plan
     // flags a frame that needs more, and run_frame renders it again
     if (tess_head<R>(L, F, M, rjd, njobs, M.cap)) return 1;
-    if (raster_prepare<R>(F, npx)) return 1;
     if (fused_launch<R>(L, F, M, rjd, njobs, npx, F.h_nline)) return 1;
     replayed = true;
     F.unread = true;
     return raster_copies(F, imgs, sizeof(R) == 8);
   }
   if (tess_head<R>(L, F, M, rjd, njobs, ~uint64_t(0))) return 1;
-  if (tess_readback<R>(F, M, hh, true)) return 1;
-  if (raster_prepare<R>(F, npx)) return 1;
+  if (tess_readback<R>(F, M, hh, F.h_nline, true)) return 1;
   if (fused_launch<R>(L, F, M, rjd, njobs, npx, F.h_nline)) return 1;
   return raster_copies(F, imgs, sizeof(R) == 8);
 }
@@ -1414,6 +1441,13 @@ int render_mesh_t(Lib& L, Frame& F, const contour3d_amd_mesh& M, std::vector<Ima
   RasterJob<R>* rjd = F.rjobs.as<RasterJob<R>>();
   C3D_HIP(F.hdr.ensure(sizeof(FrameHdr)));
   C3D_HIP(F.hstage.ensure(sizeof(FrameHdr)));
+  if (!frame_ids_fit(M.total, F.h_frags.pix.size())) {
+    log_error("%llu triangles and %zu line fragments exceed the 2^32 ids of one frame",
+              (unsigned long long)M.total, F.h_frags.pix.size());
+    return 1;
+  }
+  // the line fragments' count goes to the header before the plan reads it
+  if (raster_prepare<R>(F, npx)) return 1;
   // the plan kernel recomputes the ranges from the mesh's tile starts
   C3D_HIP(launch_plan<R>(M.bstart.as<uint64_t>(), M.ntiles, M.meta.as<ObjMeta>(), int(M.base.size()), rjd,
                          int(imgs.size()), F.hdr.as<FrameHdr>(), ~uint64_t(0), F.st));
@@ -1473,6 +1507,10 @@ struct Request {
   std::string fname;          // PPM output, written by the server's writer thread
   uint8_t* pinned = nullptr;  // the pinned host image it is rendered into
   size_t pinned_bytes = 0;
+  // snapshot_fields: the request's own contour objects, pointing at device
+  // copies of its fields taken at submit (returned to the pool at completion)
+  std::vector<contour3d_contour_obj_t> objs;
+  std::vector<std::pair<void*, size_t>> snaps;
 };
 
 struct Flight {
@@ -1496,7 +1534,6 @@ struct Server {
   uint64_t next_ticket = 1;
   bool stop = false;
   Frame frames[2];
-  int next_slot = 0;
   double tri_per_cell = 0.5;  // running estimate of triangles per cell and threshold
   uint64_t admitted = 0;      // estimated bytes of the frames in flight
   std::vector<double> lat_ms;
@@ -1511,6 +1548,12 @@ struct Server {
   // by submit, freed by stop)
   std::mutex pool_mu;
   std::vector<std::pair<uint8_t*, size_t>> pinned_free, pinned_all;
+  // device snapshots of fields (snapshot_fields), reused: allocated by
+  // submit, taken and returned under pool_mu, freed by stop; snap_held counts
+  // the bytes that requests hold, within the HBM budget
+  std::vector<std::pair<void*, size_t>> snap_free, snap_all;
+  uint64_t snap_held = 0;
+  hipStream_t copy_st = nullptr;  // the snapshot copies
 };
 
 uint8_t* pinned_take(Server& S, size_t bytes, size_t& got) {
@@ -1529,7 +1572,7 @@ uint8_t* pinned_take(Server& S, size_t bytes, size_t& got) {
     }
   }
   void* p = nullptr;
-  if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p) return nullptr;
+  if (injected_alloc_failure() || hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p) return nullptr;
   std::lock_guard<std::mutex> g(S.pool_mu);
   S.pinned_all.push_back({static_cast<uint8_t*>(p), bytes});
   got = bytes;
@@ -1540,6 +1583,47 @@ void pinned_give(Server& S, uint8_t* p, size_t bytes) {
   if (!p) return;
   std::lock_guard<std::mutex> g(S.pool_mu);
   S.pinned_free.push_back({p, bytes});
+}
+
+// A device buffer of at least `bytes` for a field snapshot: the smallest
+// free one that fits, else a new one; null when the snapshots held would
+// exceed the HBM budget or the allocation fails.
+void* snap_take(Server& S, size_t bytes, size_t& got) {
+  {
+    std::lock_guard<std::mutex> g(S.pool_mu);
+    if (S.snap_held + bytes > S.opts.hbm_budget) return nullptr;
+    size_t best = S.snap_free.size();
+    for (size_t k = 0; k < S.snap_free.size(); k++)
+      if (S.snap_free[k].second >= bytes &&
+          (best == S.snap_free.size() || S.snap_free[k].second < S.snap_free[best].second))
+        best = k;
+    if (best < S.snap_free.size()) {
+      const auto b = S.snap_free[best];
+      S.snap_free.erase(S.snap_free.begin() + long(best));
+      S.snap_held += b.second;
+      got = b.second;
+      return b.first;
+    }
+  }
+  void* p = nullptr;
+  if (injected_alloc_failure() || hipMalloc(&p, bytes) != hipSuccess || !p) return nullptr;
+  std::lock_guard<std::mutex> g(S.pool_mu);
+  S.snap_all.push_back({p, bytes});
+  S.snap_held += bytes;
+  got = bytes;
+  return p;
+}
+
+// Returns a request's snapshots to the pool (no HIP call: runs in the
+// stream's host function too).
+void snap_give(Server& S, Request& r) {
+  if (r.snaps.empty()) return;
+  std::lock_guard<std::mutex> g(S.pool_mu);
+  for (auto& b : r.snaps) {
+    S.snap_free.push_back(b);
+    S.snap_held -= b.second;
+  }
+  r.snaps.clear();
 }
 
 std::mutex g_server_mu;  // server start/stop against submit/wait/stats
@@ -1588,6 +1672,7 @@ void complete(Server& S, Flight& f, int rc, double tri_total) {
 void complete_locked(Server& S, Flight& f, int rc, double tri_total) {
   const double t = now_s();
   for (auto& r : f.reqs) {
+    snap_give(S, *r);  // the frame that read the snapshots is done
     if (!rc && !r->fname.empty()) {
       // rendered into its pinned image: completes once the file is written
       S.to_write.push_back(r);
@@ -1671,8 +1756,8 @@ void server_loop(Lib& L, Server& S) {
       f->bytes = bytes;
       S.admitted += bytes;
       S.peak = std::max(S.peak, S.admitted);
-      f->slot = S.next_slot;
-      S.next_slot ^= 1;
+      // the frame slot that no flight holds (at most one is in flight here)
+      f->slot = S.flights.empty() ? 0 : (S.flights.front()->slot ^ 1);
       Flight* fp = f.get();
       S.flights.push_back(std::move(f));
       lk.unlock();
@@ -1771,6 +1856,28 @@ int write_ppm(const char* fname, size_t W, size_t H, const uint8_t* rgb) {
   return 0;
 }
 
+// The reference runs one pencil per MPI rank and reads each rank's local
+// extent from SimpleDecomp (extend.c:158-181); this library renders every
+// contour object's array as the whole field.  Under an MPI launch of more
+// than one rank the arrays are pencils, so contour3d_execute refuses the
+// call instead of reading past them.  Ranks are taken from the launchers'
+// environment (Open MPI, MPICH/Hydra and PMI, MVAPICH);
+// CONTOUR3D_AMD_WHOLE_FIELD=1 declares that every rank passes whole fields.
+int mpi_ranks_from_env() {
+  const char* wf = std::getenv("CONTOUR3D_AMD_WHOLE_FIELD");
+  if (wf && std::strcmp(wf, "1") == 0) return 1;
+  const char* vars[] = {"OMPI_COMM_WORLD_SIZE", "PMI_SIZE", "PMIX_SIZE", "MV2_COMM_WORLD_SIZE"};
+  int n = 1;
+  for (const char* v : vars) {
+    const char* s = std::getenv(v);
+    if (!s || !*s) continue;
+    char* end = nullptr;
+    const long r = std::strtol(s, &end, 10);
+    if (end && *end == '\0' && r > n) n = int(std::min<long>(r, 1 << 30));
+  }
+  return n;
+}
+
 bool args_ok(const void* cam, const void* light, const size_t* sizes, const void* ctr, const void* local,
              const void* bg) {
   if (!cam || !light || !sizes || !ctr || !local || !bg) {
@@ -1834,6 +1941,9 @@ int contour3d_amd_tune(int param, int64_t value) {
     case CONTOUR3D_AMD_TUNE_GRAPHS:
       L.graphs = value != 0;
       if (!L.graphs) L.main.drop_graph();
+      return 0;
+    case CONTOUR3D_AMD_TUNE_FAIL_ALLOC:
+      c3d::g_fail_alloc.store(value);
       return 0;
     default:
       c3d::log_error("tune: unknown parameter %d", param);
@@ -1901,6 +2011,13 @@ int contour3d_execute(const sdecomp_info_t* const sdecomp_info, const contour3d_
                       const contour3d_line_obj_t* const contour3d_line_objs, const char fname[]) {
   (void)sdecomp_info;    // one GPU holds the whole field
   (void)camera_look_at;  // unused by the reference too (struct.h:53-58)
+  const int ranks = c3d::mpi_ranks_from_env();
+  if (ranks > 1) {
+    c3d::log_error("%d MPI ranks: each rank's array is a pencil of the field, and this library renders whole "
+                   "fields on one GPU (set CONTOUR3D_AMD_WHOLE_FIELD=1 if every rank passes its whole field)",
+                   ranks);
+    return 1;
+  }
   auto& L = lib();
   std::lock_guard<std::mutex> g(L.mu);
   if (c3d::ensure_init(L, 0)) return 1;
@@ -1944,9 +2061,9 @@ int contour3d_amd_mesh_build(const contour3d_contour_obj_t* contour_objs, size_t
   const int rc = c3d::build_mesh(L, L.main, *M, contour_objs, num_contours, precision);
   const hipError_t e = hipStreamSynchronize(L.main.st);
   c3d::release_geoms(L.main);
-  if (rc) return 1;
-  if (e != hipSuccess) {
-    c3d::log_error("mesh build failed on the device");
+  if (rc || e != hipSuccess) {
+    if (!rc) c3d::log_error("mesh build failed on the device");
+    M->release();  // a failed build keeps no device memory
     return 1;
   }
   if (c3d::prof_collect(L.main)) return 1;
@@ -2132,6 +2249,12 @@ int contour3d_amd_server_start(const contour3d_amd_server_opts_t* opts) {
       return 1;
     }
   }
+  if (hipStreamCreateWithFlags(&S->copy_st, hipStreamNonBlocking) != hipSuccess) {
+    c3d::log_error("server: stream creation failed");
+    for (auto& G : S->frames) G.release();
+    delete S;
+    return 1;
+  }
   {
     std::lock_guard<std::mutex> g(L.mu);
     L.server = S;
@@ -2142,6 +2265,38 @@ int contour3d_amd_server_start(const contour3d_amd_server_opts_t* opts) {
 }
 
 namespace c3d {
+
+// The checks run_frame makes of one image (frame_setup, tess_prepare,
+// frame_lines), made at submit.
+int validate_request(const contour3d_amd_image_t& im, bool ppm) {
+  const size_t W = im.screen_sizes[0], H = im.screen_sizes[1];
+  if (W == 0 || H == 0 || W > (size_t(1) << 16) || H > (size_t(1) << 16) ||
+      uint64_t(W) * uint64_t(H) >= (uint64_t(1) << 31)) {
+    log_error("server_submit: image size %zu x %zu outside 1..65536 per axis and 2^31 pixels", W, H);
+    return 1;
+  }
+  if (!ppm && !im.rgb) {
+    log_error("server_submit: rgb output pointer is NULL");
+    return 1;
+  }
+  if (im.num_contours && !im.contour_objs) {
+    log_error("server_submit: contour objects pointer is NULL");
+    return 1;
+  }
+  for (size_t o = 0; o < im.num_contours; o++)
+    if (validate_obj(im.contour_objs[o], o)) return 1;
+  if (im.num_lines && !im.line_objs) {
+    log_error("server_submit: line objects pointer is NULL");
+    return 1;
+  }
+  for (size_t l = 0; l < im.num_lines; l++) {
+    if (im.line_objs[l].nitems < 2 || !im.line_objs[l].converter) {
+      log_error("line object %zu: needs at least two points and a converter", l);
+      return 1;
+    }
+  }
+  return 0;
+}
 
 // submit and submit_ppm: queues one request (fname: PPM output)
 int server_submit(const contour3d_amd_image_t* image, int precision, const char* fname, uint64_t* ticket) {
@@ -2156,37 +2311,73 @@ int server_submit(const contour3d_amd_image_t* image, int precision, const char*
     log_error("unknown precision %d", precision);
     return 1;
   }
-  if (image->num_contours && !image->contour_objs) {
-    log_error("server_submit: contour objects pointer is NULL");
-    return 1;
-  }
+  // what run_frame would reject fails here, at submit, so that a frame in
+  // flight never carries a request that cannot render
+  if (validate_request(*image, fname != nullptr)) return 1;
   auto r = std::make_shared<Request>();
   r->img = *image;
-  if (fname) {
-    // the frame's image limits, checked before the pinned image is taken
-    const size_t W = image->screen_sizes[0], H = image->screen_sizes[1];
-    if (!*fname || W == 0 || H == 0 || W > (size_t(1) << 16) || H > (size_t(1) << 16) ||
-        uint64_t(W) * uint64_t(H) >= (uint64_t(1) << 31)) {
-      log_error("server_submit_ppm: empty file name, or image size %zux%zu outside 1..65536 per axis and "
-                "2^31 pixels", W, H);
+  if (S->opts.snapshot_fields && image->num_contours) {
+    // copy every distinct field to a device snapshot before returning, so
+    // the caller may update its fields in place at once
+    r->objs.assign(image->contour_objs, image->contour_objs + image->num_contours);
+    std::vector<std::pair<const double*, double*>> done;
+    int bad = 0;
+    for (size_t o = 0; o < r->objs.size() && !bad; o++) {
+      contour3d_contour_obj_t& ob = r->objs[o];
+      double* dst = nullptr;
+      for (const auto& m : done)
+        if (m.first == ob.array) dst = m.second;
+      if (!dst) {
+        const size_t bytes = ob.glsizes[0] * ob.glsizes[1] * ob.glsizes[2] * sizeof(double);
+        size_t got = 0;
+        void* p = snap_take(*S, bytes, got);
+        if (!p) {
+          log_error("server_submit: no device snapshot of %zu bytes (allocation failed, or the snapshots "
+                    "held would exceed the HBM budget)", bytes);
+          bad = 1;
+          break;
+        }
+        r->snaps.push_back({p, got});
+        if (hipMemcpyAsync(p, ob.array, bytes, hipMemcpyDefault, S->copy_st) != hipSuccess) {
+          log_error("server_submit: snapshot copy failed");
+          bad = 1;
+          break;
+        }
+        dst = static_cast<double*>(p);
+        done.push_back({ob.array, dst});
+      }
+      ob.array = dst;
+    }
+    if (hipStreamSynchronize(S->copy_st) != hipSuccess && !bad) {
+      log_error("server_submit: snapshot copy failed");
+      bad = 1;
+    }
+    if (bad) {
+      snap_give(*S, *r);
       return 1;
     }
+    r->img.contour_objs = r->objs.data();
+  }
+  if (fname) {
+    const size_t W = image->screen_sizes[0], H = image->screen_sizes[1];
     const size_t bytes = W * H * 3;
     r->fname = fname;
     r->pinned = pinned_take(*S, bytes, r->pinned_bytes);
     if (!r->pinned) {
       log_error("server_submit_ppm: cannot allocate a pinned image of %zu bytes", bytes);
+      snap_give(*S, *r);
       return 1;
     }
     r->img.rgb = r->pinned;
   }
   r->precision = precision;
-  r->bytes = estimate_bytes(*S, *image, precision);
+  r->bytes = estimate_bytes(*S, r->img, precision);
   r->t_submit = now_s();
   std::lock_guard<std::mutex> g(S->m);
   if (S->stop) {
     log_error("server_submit: server stopping");
     pinned_give(*S, r->pinned, r->pinned_bytes);
+    snap_give(*S, *r);
     return 1;
   }
   r->ticket = S->next_ticket++;
@@ -2291,6 +2482,8 @@ int contour3d_amd_server_stop(void) {
   }
   for (auto& F : S->frames) F.release();
   for (auto& b : S->pinned_all) (void)hipHostFree(b.first);
+  for (auto& b : S->snap_all) (void)hipFree(b.first);
+  if (S->copy_st) (void)hipStreamDestroy(S->copy_st);
   delete S;
   return 0;
 }

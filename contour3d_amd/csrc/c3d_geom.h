@@ -16,6 +16,14 @@
 
 namespace c3d {
 
+// Every triangle of a frame takes a 32-bit id (1..ntri, its serial position
+// plus one) and every line fragment one after them (ntri + 1 + fragment);
+// 0 marks an empty pixel.  A frame holds fewer than 2^32 - 2^24 triangles.
+constexpr uint64_t kMaxFrameTriangles = (uint64_t(1) << 32) - (uint64_t(1) << 24);
+C3D_HD bool frame_ids_fit(uint64_t ntri, uint64_t nline) {
+  return ntri < kMaxFrameTriangles && ntri + nline <= uint64_t(0xffffffffu);
+}
+
 using std::ceil;
 using std::fabs;
 using std::floor;

@@ -117,7 +117,8 @@ struct RasterJob {
 };
 
 // Frame-wide values on the device.  k_plan writes ntri, nwork and the
-// overflow bit 1; k_zinit clears nfrag and bit 2; the host writes nline.
+// overflow bit 1; k_zinit clears nfrag and bit 2; the host writes nline
+// before the plan runs.
 struct FrameHdr {
   uint64_t ntri;        // triangles in the mesh
   uint64_t nwork;       // raster work items (sum of the images' triangle ranges)

@@ -340,7 +340,10 @@ __global__ void k_plan(const uint64_t* __restrict__ start, uint64_t ntiles, ObjM
     hdr->ntri = ntri;
     hdr->nwork = wk;
     hdr->nfrag = 0;
-    hdr->overflow = ntri > cap ? 1u : 0u;
+    // triangles take ids 1..ntri and line fragments the ids after them, all
+    // in 32 bits; a frame beyond that is flagged like one beyond `cap` (the
+    // host then rejects it after its read back)
+    hdr->overflow = (ntri > cap || !frame_ids_fit(ntri, hdr->nline)) ? 1u : 0u;
   }
 }
 

@@ -105,8 +105,12 @@ extern int contour3d_amd_execute_batch(size_t num_images, const contour3d_amd_im
 
 typedef struct {
   size_t max_images;    /* images per batch at most (0: 16) */
-  uint64_t hbm_budget;  /* estimated device bytes of the frames in flight at most (0: 200e9) */
+  uint64_t hbm_budget;  /* estimated device bytes of the frames in flight at most, and separately the
+                           bytes of field snapshots held (0: 200e9) */
   double max_wait_ms;   /* while the device is idle, a request waits this long for others (0: none) */
+  int snapshot_fields;  /* 1: submit copies every field of the request (host or device) into a device
+                           snapshot before it returns, so a solver may update its fields in place at
+                           once; the request renders the snapshot.  0: fields are read in place. */
 } contour3d_amd_server_opts_t;
 
 typedef struct {
@@ -127,9 +131,13 @@ typedef struct {
  *
  * Every buffer an image points to (contour object array, fields, grids,
  * line objects, rgb, depth) must stay valid and unchanged from submit until
- * its wait returns.  Fields updated in place on the device must be complete
- * (the updating stream synchronised) before the submit that reads them.
- * start/stop must not run concurrently with submit, wait or stats.
+ * its wait returns; with snapshot_fields, the contour object array and the
+ * fields only until submit returns.  Fields updated in place on the device
+ * must be complete (the updating stream synchronised) before the submit
+ * that reads them.  Submit checks each request as a frame would (sizes,
+ * objects, lines) and returns 1 for one that could not render; a request
+ * whose frame fails on the device fails alone.  start/stop must not run
+ * concurrently with submit, wait or stats.
  */
 extern int contour3d_amd_server_start(const contour3d_amd_server_opts_t * opts);
 extern int contour3d_amd_server_submit(const contour3d_amd_image_t * image, int precision, uint64_t * ticket);
@@ -229,11 +237,15 @@ enum {
   CONTOUR3D_AMD_TUNE_FRAG_CAPACITY = 2, /* fragment list entries, 16 B each (0: the default, 2^24; grows by
                                            itself after an overflow; an overflowing frame takes the slower
                                            z-test passes over the geometry) */
-  CONTOUR3D_AMD_TUNE_GRAPHS = 3         /* 1: contour3d_execute, _execute_rgb and _execute_batch capture
+  CONTOUR3D_AMD_TUNE_GRAPHS = 3,        /* 1: contour3d_execute, _execute_rgb and _execute_batch capture
                                            their launch sequence as a HIP graph and replay it while the
                                            frame layout (image sizes, objects' fields and lattices,
                                            buffer capacities) repeats; thresholds, colours, cameras and
                                            lines may change between replays.  Default 0. */
+  CONTOUR3D_AMD_TUNE_FAIL_ALLOC = 4     /* failure injection for testing the error contract: n > 0 makes
+                                           the n-th device or pinned allocation from now fail as an
+                                           out-of-memory would (once); the call that needed it returns 1
+                                           with the error line, and the library stays usable.  0 disarms. */
 };
 
 extern int contour3d_amd_tune(int param, int64_t value);

@@ -57,6 +57,7 @@ int main(void) {
   P(contour3d_amd_image_t, rgb); P(contour3d_amd_image_t, depth);
   printf("sopts %zu\nsstats %zu\n", sizeof(contour3d_amd_server_opts_t), sizeof(contour3d_amd_server_stats_t));
   P(contour3d_amd_server_opts_t, hbm_budget); P(contour3d_amd_server_opts_t, max_wait_ms);
+  P(contour3d_amd_server_opts_t, snapshot_fields);
   P(contour3d_amd_server_stats_t, mean_batch); P(contour3d_amd_server_stats_t, p99_ms);
   P(contour3d_amd_server_stats_t, frames_per_s); P(contour3d_amd_server_stats_t, peak_admitted_bytes);
   P(contour3d_contour_obj_t, glsizes); P(contour3d_contour_obj_t, grids);
@@ -84,7 +85,7 @@ int main(void) {
         assert got[f"contour3d_line_obj_t.{f}"] == getattr(_abi.LineObj, f).offset, f
     assert got["sopts"] == C.sizeof(_abi.ServerOpts)
     assert got["sstats"] == C.sizeof(_abi.ServerStats)
-    for f in ("hbm_budget", "max_wait_ms"):
+    for f in ("hbm_budget", "max_wait_ms", "snapshot_fields"):
         assert got[f"contour3d_amd_server_opts_t.{f}"] == getattr(_abi.ServerOpts, f).offset, f
     for f in ("mean_batch", "p99_ms", "frames_per_s", "peak_admitted_bytes"):
         assert got[f"contour3d_amd_server_stats_t.{f}"] == getattr(_abi.ServerStats, f).offset, f

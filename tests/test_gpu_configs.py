@@ -1,0 +1,252 @@
+"""GPU: the benchmark configurations and the z-buffer tie rule against the
+CPU oracle.
+
+- Ties.  The reference draws every contour object in order, then every
+  line, and writes a pixel when the new depth is at least the stored one
+  (render.c:125-132, line.c:109-115, contour/main.c:49-112): on equal depths
+  the later triangle or line wins.  The device keeps "largest depth, then
+  largest serial id"; these tests force exact ties.
+- C1: the plain reference method, 128^3 Gaussian blobs, threshold 0.5,
+  800x600.
+- The reference example itself (src/main.c:23-306): 128^3 sine field on a
+  Chebyshev x grid, thresholds -0.25 / +0.25, eight edge lines, 1440x1080,
+  written by contour3d_execute; the PPM bytes must equal the oracle's.
+- C4: the FP32 form against the oracle's FP64 directly on a field the oracle
+  renders in seconds, and a 1024^3 property run of both device forms.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from contour3d_amd import api, fields
+
+from test_gpu_parity import same_bits, view_for
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def render(v, contours, lines=(), precision=api.FP64):
+    return api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], contours, lines,
+                           precision=precision, want_depth=True)
+
+
+def test_equal_depths_go_to_the_later_object(orc):
+    """The same field and threshold three times in three colours: twice in
+    one field pass (same array) and once from a copy of the array (a second
+    field pass).  Every covered pixel is an exact tie between the three; the
+    last object must win all of them.  FP64 is bit-exact with the oracle;
+    FP32 renders exactly what the last object renders alone."""
+    banner("ties: one surface drawn three times")
+    f = fields.turbulence((28, 28, 28), kmax=7, seed=21)
+    g = f.copy()
+    cont = [dict(field=f, threshold=0.1, color=(250, 30, 30)),
+            dict(field=f, threshold=0.1, color=(30, 250, 30)),
+            dict(field=g, threshold=0.1, color=(30, 30, 250))]
+    v = view_for(160, 120)
+    rgb, dep = render(v, cont)
+    ref_rgb, ref_dep = orc.execute(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], cont)
+    cov = ref_dep > -1e300
+    assert cov.sum() > 1000
+    assert same_bits(dep, ref_dep)
+    assert np.array_equal(rgb, ref_rgb)
+    last = render(v, cont[2:])
+    assert np.array_equal(rgb, last[0]) and same_bits(dep, last[1])
+    # the later object wins inside one field pass too
+    two = render(v, cont[:2])
+    assert np.array_equal(two[0], render(v, cont[1:2])[0])
+    # FP32: the packed key (depth, id) picks the same winners
+    rgb32, dep32 = render(v, cont, precision=api.FP32)
+    alone32, adep32 = render(v, cont[2:], precision=api.FP32)
+    assert np.array_equal(rgb32, alone32)
+    assert same_bits(dep32, adep32)
+    two32 = render(v, cont[:2], precision=api.FP32)
+    assert np.array_equal(two32[0], render(v, cont[1:2], precision=api.FP32)[0])
+
+
+def test_lines_drawn_after_contours_on_a_plane(orc):
+    """f = z on a cell-centred grid: threshold 0 cuts every z-edge at its
+    midpoint, so the surface is the plane z = 0 exactly.  Lines lying in
+    that plane (along its diagonal and an axis) compete with the surface on
+    nearly or exactly equal depths; the image is bit-exact with the oracle,
+    which draws lines after contours with the same >= rule."""
+    banner("ties: lines lying on a contour plane")
+    n = 16
+    gz = fields.uniform_grid(n)
+    f = np.broadcast_to(gz[:, None, None], (n, n, n)).copy()
+    cont = [dict(field=f, threshold=0.0, color=(40, 200, 220))]
+    lines = [dict(edges=((-0.5, -0.5, 0.0), (0.5, 0.5, 0.0)), nitems=9, color=(255, 255, 255), width=3.0),
+             dict(edges=((-0.4, 0.0, 0.0), (0.4, 0.0, 0.0)), nitems=5, color=(255, 0, 0), width=2.0)]
+    v = view_for(200, 150)
+    for prec in (api.FP64, api.FP32):
+        rgb, dep = render(v, cont, lines, precision=prec)
+        if prec == api.FP64:
+            ref_rgb, ref_dep = orc.execute(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"],
+                                           cont, lines)
+            assert same_bits(dep, ref_dep)
+            assert np.array_equal(rgb, ref_rgb)
+            white = np.all(ref_rgb == (255, 255, 255), axis=2).sum()
+            assert white > 50  # the lines show over the plane
+        else:
+            assert np.isfinite(dep[dep > -1e300]).all()
+
+
+def test_c1_plain_method_128_blobs_800x600(orc):
+    banner("C1: 128^3 blobs, threshold 0.5, 800x600")
+    f = fields.blobs((128, 128, 128), seed=0)
+    v = fields.example_view(800, 600)
+    cont = [dict(field=f, threshold=0.5, color=(255, 200, 60))]
+    rgb, dep = render(v, cont)
+    ref_rgb, ref_dep = orc.execute(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], cont)
+    assert (ref_dep > -1e300).sum() > 20000
+    assert same_bits(dep, ref_dep)
+    assert np.array_equal(rgb, ref_rgb)
+    rgb32, dep32 = render(v, cont, precision=api.FP32)
+    cov = ref_dep > -1e300
+    mism = np.any(rgb32 != ref_rgb, axis=2).sum()
+    print(f"C1 FP32: {mism} of {cov.sum()} pixels differ from the FP64 oracle")
+    assert mism <= 0.01 * cov.sum()
+
+
+def reference_example_lines(n):
+    """The eight edge lines of src/main.c:221-306: white, width 4, nitems =
+    glsizes[0], y- and z-direction edges of the unit cube."""
+    out = []
+    for a, b in (((-0.5, -0.5, -0.5), (-0.5, 0.5, -0.5)), ((0.5, -0.5, -0.5), (0.5, 0.5, -0.5)),
+                 ((-0.5, -0.5, 0.5), (-0.5, 0.5, 0.5)), ((0.5, -0.5, 0.5), (0.5, 0.5, 0.5)),
+                 ((-0.5, -0.5, -0.5), (-0.5, -0.5, 0.5)), ((0.5, -0.5, -0.5), (0.5, -0.5, 0.5)),
+                 ((-0.5, 0.5, -0.5), (-0.5, 0.5, 0.5)), ((0.5, 0.5, -0.5), (0.5, 0.5, 0.5))):
+        out.append(dict(edges=(a, b), nitems=n, color=(255, 255, 255), width=4.0))
+    return out
+
+
+def test_reference_example_at_full_size(orc, tmp_path):
+    banner("reference example: 128^3, 1440x1080, 8 lines, contour3d_execute")
+    f, g = fields.example_field(128)
+    conv = orc.identity_converter_address()  # a caller's own identity map, as src/main.c:12-21
+    cont = [dict(field=f, grids=g, converter=conv, threshold=-0.25, color=(0x00, 0xFF, 0xFF)),
+            dict(field=f, grids=g, converter=conv, threshold=+0.25, color=(0xFF, 0xFF, 0x00))]
+    lines = [dict(ln, converter=conv) for ln in reference_example_lines(128)]
+    cam = fields.EXAMPLE_CAMERA
+    out, ref = tmp_path / "gpu.ppm", tmp_path / "cpu.ppm"
+    api.set_precision(api.FP64)
+    api.execute(cam["camera"], cam["look_at"], cam["light"], (1440, 1080), cam["center"], cam["local"], cam["bg"],
+                cont, lines, fname=out)
+    orc.execute(cam["camera"], cam["light"], (1440, 1080), cam["center"], cam["local"], cam["bg"], cont, lines,
+                fname=ref)
+    a, b = out.read_bytes(), ref.read_bytes()
+    assert a[:16] == b"P6\n1440 1080\n255"
+    assert len(a) == len(b) == len(b"P6\n1440 1080\n255\n") + 1440 * 1080 * 3
+    assert a == b
+    body = np.frombuffer(b[len(b"P6\n1440 1080\n255\n"):], np.uint8).reshape(1080, 1440, 3)
+    assert np.all(body == (255, 255, 255), axis=2).sum() > 1000  # the edge lines are drawn
+    assert np.all(body == 0, axis=2).sum() < 0.9 * 1440 * 1080   # and the surfaces
+
+
+def vertex_angles(a, b):
+    d = np.clip(np.sum(a * b, axis=-1), -1, 1)
+    ok = np.isfinite(d)
+    return np.degrees(np.arccos(d[ok]))
+
+
+def test_c4_fp32_form_against_the_fp64_oracle(orc):
+    """C4's precision study at a size the oracle renders in seconds: 96^3
+    spectral turbulence, thresholds {-1.5, 0, 1.5}, 512^2.  The FP32 form's
+    triangle set equals the oracle's (cells and lattice edges of every
+    triangle, in serial order); its vertices, vertex normals and pixels are
+    compared with the oracle's FP64 directly.  Bounds sit an order of
+    magnitude above what the device measures (printed) and far below what a
+    wrong index or sign gives."""
+    banner("C4: FP32 form vs the FP64 oracle, 96^3, 512^2")
+    n = 96
+    f = fields.turbulence((n, n, n), kmax=n // 2 - 1, seed=7)
+    thr = (-1.5, 0.0, 1.5)
+    cols = ((255, 80, 40), (40, 220, 90), (60, 120, 255))
+    cont = [dict(field=f, threshold=t, color=c) for t, c in zip(thr, cols)]
+    m32 = api.Mesh(cont, precision=api.FP32)
+    try:
+        for q, t in enumerate(thr):
+            ref = orc.mesh(f, t)
+            got = m32.copy(q)
+            T = len(ref["cell"])
+            assert T > 1000 and m32.count(q) == T
+            assert np.array_equal(got["cell"].astype(np.int64), ref["cell"])
+            assert np.array_equal(got["edge"], ref["edge"])
+            verr = np.abs(got["pos"] - ref["pos"]).max()
+            ins = ref["interior"]
+            ang = vertex_angles(got["vn"][ins], ref["vn"][ins])
+            print(f"thr {t}: T={T} max vertex err {verr:.3g}, normal angle mean {ang.mean():.3g} "
+                  f"max {ang.max():.3g} deg, over 1 deg {np.mean(ang > 1):.3g}")
+            assert verr < 1e-6
+            assert ang.mean() < 0.05 and np.mean(ang > 1.0) < 1e-3
+    finally:
+        m32.free()
+    v = fields.example_view(512, 512)
+    ref_rgb, ref_dep = orc.execute(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], cont)
+    rgb, dep = render(v, cont, precision=api.FP32)
+    cov = ref_dep > -1e300
+    cov32 = dep > -1e300
+    diff = np.abs(rgb.astype(int) - ref_rgb.astype(int)).max(axis=2)
+    both = cov & cov32
+    derr = np.abs(dep[both] - ref_dep[both]).max()
+    print(f"pixels: covered {cov.sum()}, differ {np.sum(diff > 0)}, by more than one level {np.sum(diff > 1)}, "
+          f"coverage differences {np.sum(cov ^ cov32)}, max depth err {derr:.3g}")
+    assert cov.sum() > 50000
+    assert np.sum(diff > 0) <= 0.01 * cov.sum()
+    assert np.sum(diff > 1) <= 0.001 * cov.sum()
+    assert np.sum(cov ^ cov32) <= 0.001 * cov.sum()
+    assert np.isfinite(dep[cov32]).all()
+
+
+def test_c4_1024_property_run():
+    """C4 at its full size, 1024^3 spectral turbulence (|k| 1..128),
+    thresholds {-1.5, 0, 1.5}, 2048^2, too large for the oracle: the FP32
+    and FP64 device forms have the same triangle set (no topology mismatch),
+    vertex errors of an FP32 rounding, and no non-finite depth in either
+    image."""
+    banner("C4: 1024^3 property run")
+    import torch
+
+    api.finalize()
+    n = 1024
+    f = fields.turbulence_fast_device((n, n, n), kmax=128, seed=7)
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    thr = (-1.5, 0.0, 1.5)
+    cont = [dict(field=f, threshold=t, color=(200, 200, 200)) for t in thr]
+    v = fields.example_view(2048, 2048)
+    try:
+        for prec in (api.FP64, api.FP32):
+            rgb = torch.zeros((2048, 2048, 3), dtype=torch.uint8, device="cuda")
+            dep = torch.zeros((2048, 2048), dtype=torch.float64, device="cuda")
+            api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], cont,
+                            precision=prec, rgb=rgb, depth=dep)
+            torch.cuda.synchronize()
+            cov = dep > -1e300
+            assert int(cov.sum()) > 500000
+            assert bool(torch.isfinite(dep[cov]).all()), prec
+            del rgb, dep
+        api.finalize()
+        for t in thr:
+            one = [dict(field=f, threshold=t, color=(1, 1, 1))]
+            m64 = api.Mesh(one, precision=api.FP64)
+            m32 = api.Mesh(one, precision=api.FP32)
+            try:
+                st = m64.compare(m32, 0)
+            finally:
+                m64.free()
+                m32.free()
+            print(f"thr {t}: {st}")
+            assert st["triangles"] > 10 ** 8
+            assert st["topology_mismatches"] == 0
+            assert st["max_vertex_err"] < 1e-7
+            assert st["mean_normal_angle_deg"] < 0.05
+    finally:
+        api.finalize()
+        del f
+        torch.cuda.empty_cache()
