@@ -9,25 +9,37 @@ per frame.  `value` is the FP64 form (bit-identical to the reference).
 
 Beside it, in the same JSON line:
   fp32        the FP32-geometry form of the same frame
+  roofline    the dominant kernel of the FP64 frame, the fused tessellation
+              + raster kernel (k_tess_raster), timed alone with HIP events
+              on its stream, charged with the graded tessellation bytes
+              B_tess = 8 N + 96 T (the field once, the FP64 triangles); its
+              measured HBM traffic and VALU figures come from the rocprofv3
+              summary under profiles/
+  tessellation  the stored-mesh tessellation (count + scan + emit of
+              contour3d_amd_mesh_build) on the same field, FP32 and FP64,
+              against B_tess (48 B per FP32 triangle, 96 B per FP64 one),
+              and the smoothing stage (contour3d_amd_mesh_normals of every
+              object) against B_smooth = 76 T (FP32)
+  c1          the plain reference method: 128^3 Gaussian blobs, threshold
+              0.5, 800x600
   batched     (1) four cameras on the same 512^3 scene: plain single-image
               calls (FP64 and FP32) against one batched call (FP32, eager
               and graph-replayed) and the serving loop; (2) 64 fields of
               uneven size (each axis log-uniform in 64..384), 1-4 thresholds
               and a camera each, 1024^2 images, in one frame, eager and
-              graph-replayed, with autotuned workgroup caps
+              graph-replayed, after an autotuner sweep of the tessellation
+              brick (rows per tile, planes per work item) and workgroup cap
   served      Poisson arrivals (50/s) of FP32 requests against a pool of
-              device-resident fields (64^3..768^3, updated in place between
-              steps), 1-4 thresholds, images 512^2..2048^2, random cameras;
-              p50/p99 latency and frames/s of the serving loop, next to a
-              plain loop that renders the same arrivals one at a time
+              device-resident fields (64^3..768^3) that a solver advances in
+              place after every submit (the server snapshots each request's
+              fields), 1-4 thresholds, images 512^2..2048^2, random cameras;
+              p50/p99 latency and frames/s of the serving loop without and
+              with a batching window, next to a plain loop that renders the
+              same arrivals one at a time
   served_ppm  the output path: images written as PPM files by a loop of
               contour3d_execute against PPM requests to the serving loop,
               whose writer thread overlaps file I/O with the next frames
-  roofline    HBM roofline of the dominant kernel of the FP64 frame: the
-              fused tessellation + raster stage, charged with the bytes of
-              the stages it replaces (B_tess + B_raster below)
-  tessellation  the stored-mesh tessellation (count + scan + emit of
-              contour3d_amd_mesh_build) on the same field, against B_tess
+  precision_study  1024^3, 2048^2: the FP32 form against the FP64 form
   cpu_baseline  the CPU oracle timed on a bounded slab of the same field
 """
 import argparse
@@ -42,10 +54,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch past L2 of the FP64 frame's stages at 512^3 (rocprofv3
-# --pmc FETCH_SIZE / WRITE_SIZE, scaled by a plain copy's readings); filled
-# in from the profile summary under profiles/
-MEASURED_TRAFFIC_512 = {"raster": 4.846e9, "count": 2.258e9, "resolve": 0.359e9}
+# the rocprofv3 summary of the headline workload: per-kernel HBM traffic
+# (FETCH_SIZE / WRITE_SIZE runs scaled by a plain copy's readings) and VALU
+# counters, in a fenced json block
+PROFILE_MD = os.path.join(ROOT, "profiles", "turbulence_512cube_3thr_1920x1080_kernels.md")
 N = 512
 THRESHOLDS = (-1.0, 0.0, 1.0)
 COLORS = ((255, 80, 40), (40, 220, 90), (60, 120, 255))
@@ -58,10 +70,12 @@ def algorithmic_bytes(stage, T, npts, npx, fp64):
     count    the field, read once for all thresholds (8 B per lattice point)
     emit     B_tess: the field once + per triangle 3 positions and the
              weighted face normal (96 B in FP64, 48 B in FP32)
-    fused    the frame's tessellation + raster stage, charged with the work
-             of the stages it replaces: B_tess + B_raster, where B_raster is
-             72 B per triangle (positions and vertex normals, FP64) plus
-             19 B per pixel (key init, key read, RGB)
+    fused    the fused tessellation + raster kernel, charged with the
+             graded tessellation bytes B_tess of the same triangles (it
+             tessellates them and draws them without storing them)
+    smooth   B_smooth: per triangle 3 edge keys (24 B), the face normal and
+             area (12 + 4 B) in and 3 vertex normals (36 B) out in FP32
+             (76 B); the same in FP64 is 24 + 32 + 72 = 128 B
     resolve  the keys read and the RGB bytes written per pixel (the vertex
              normals of the nearest triangles are gathered here and add
              O(pixels) bytes, not counted)
@@ -69,26 +83,56 @@ def algorithmic_bytes(stage, T, npts, npx, fp64):
     r = 8 if fp64 else 4
     if stage == "count":
         return 8 * npts
-    if stage == "emit":
+    if stage in ("emit", "fused"):
         return 8 * npts + 12 * r * T
-    if stage == "fused":
-        return 8 * npts + 12 * r * T + 9 * r * T + 19 * npx
+    if stage == "smooth":
+        return (128 if fp64 else 76) * T
     if stage == "resolve":
         return 8 * npx + 3 * npx
     return 0
 
 
+def profile_summary():
+    """The json block of the committed rocprofv3 summary (per-kernel HBM
+    bytes and VALU figures of the headline workload), or {}."""
+    try:
+        text = open(PROFILE_MD).read()
+        return json.loads(text.split("```json", 1)[1].split("```", 1)[0])
+    except (OSError, IndexError, ValueError):
+        return {}
+
+
 def mesh_tessellation(api, contours, precision, reps):
     """Stage times of contour3d_amd_mesh_build (the stored mesh) on the
-    frame's objects: one warm-up build, then `reps` timed ones."""
+    frame's objects, one warm-up build then `reps` timed ones, and of the
+    smoothing stage (contour3d_amd_mesh_normals of every object, into a
+    device buffer) on the last build."""
+    import torch
+
     api.Mesh(contours, precision).free()
     api.profile_reset()
     api.profile_enable(True)
-    for _ in range(reps):
+    for _ in range(reps - 1):
         api.Mesh(contours, precision).free()
+    m = api.Mesh(contours, precision)
     api.profile_enable(False)
     ms, calls, ntri = api.profile_read()
-    return {k: ms[k] / reps for k in ("count", "scan", "emit", "smooth")}, ntri / reps
+    out = {k: ms[k] / reps for k in ("count", "scan", "emit")}
+    try:
+        dt = torch.float64 if precision == api.FP64 else torch.float32
+        vn = torch.empty((max(m.count(o) for o in range(len(contours))), 3, 3), dtype=dt, device="cuda")
+        for o in range(len(contours)):
+            m.normals(o, vn)  # warm-up
+        api.profile_reset()
+        api.profile_enable(True)
+        for o in range(len(contours)):
+            m.normals(o, vn)
+        api.profile_enable(False)
+        out["smooth"] = api.profile_read()[0]["smooth"]
+        del vn
+    finally:
+        m.free()
+    return out, ntri / reps
 
 
 def run_form(api, contours, view, precision, steps, warmup, rgb_dev):
@@ -329,7 +373,8 @@ def uneven_batch_frame(api, fields, nfields, side, reps, seed=0):
     torch.cuda.synchronize()
     run = lambda: api.execute_batch(imgs, precision=api.FP32)  # noqa: E731
     run()
-    tuned, _ = api.autotune(run, reps=1)
+    tuned, best_ms, sweep = api.autotune(run, reps=2)
+    ms_all = [m for _, m in sweep]
     api.profile_reset()
     api.profile_enable(True)
     eager = _timed(run, reps)
@@ -342,22 +387,29 @@ def uneven_batch_frame(api, fields, nfields, side, reps, seed=0):
         graph = _timed(run, reps)
     finally:
         api.tune(api.TUNE_GRAPHS, 0)
-        api.tune(api.TUNE_TESS_GRID, 0)
-        api.tune(api.TUNE_RASTER_GRID, 0)
+        for p in api.AUTOTUNE_KNOBS:
+            api.tune(p, 0)
     return {"fields": nfields, "images": nfields, "image": [side, side], "precision": "fp32",
             "cell_thresholds": cells, "triangles": int(ntri / max(1, reps)),
             "eager_ms": round(eager * 1e3, 3), "graph_ms": round(graph * 1e3, 3),
             "frames_per_s_eager": round(nfields / eager, 1), "frames_per_s_graph": round(nfields / graph, 1),
-            "gcells_per_s_graph": round(cells / graph / 1e9, 2), "autotuned_workgroup_caps": tuned}
+            "gcells_per_s_graph": round(cells / graph / 1e9, 2),
+            "autotune": {"best": tuned, "best_ms": round(best_ms, 3), "worst_ms": round(max(ms_all), 3),
+                         "default_ms": round(next(m for k, m in sweep if k == {"tess_grid": 0, "tess_rows": 0,
+                                                                                "tess_zplanes": 8}), 3),
+                         "sweep": [dict(k, ms=round(m, 3)) for k, m in sweep]}}
 
 
-def served_stream(api, fields, nreq, rate, nfield_pool, seed=0):
+def served_stream(api, fields, nreq, rate, nfield_pool, seed=0, windows=(0.0, 2.0)):
     """Poisson arrivals against a pool of device-resident turbulence fields
     (sides log-uniform in 64..768) that a simulated solver advances in place
-    (sign flip, a half-period phase shift) whenever no request reads them;
-    1-4 thresholds, square images with log-uniform side in 512..2048,
-    cameras uniform on the sphere, FP32 form.  The same arrival times are
-    then replayed through a plain loop of single execute_rgb calls."""
+    (sign flip) right after every submit: the server snapshots each
+    request's fields at submit (snapshot_fields), so no request waits for
+    the solver and none sees its later steps.  1-4 thresholds, square images
+    with log-uniform side in 512..2048, cameras uniform on the sphere, FP32
+    form.  The same arrivals are served once per batching window in
+    `windows`, then replayed through a plain loop of single execute_rgb
+    calls (with the same solver steps)."""
     import math
     import threading
 
@@ -385,6 +437,10 @@ def served_stream(api, fields, nreq, rate, nfield_pool, seed=0):
         cont = [dict(field=pool[fi], threshold=x, color=cols[k]) for k, x in enumerate(thr)]
         return _img(v, cont, np.zeros((side, side, 3), np.uint8))
 
+    def solver_step(fi):
+        pool[fi].neg_()
+        torch.cuda.synchronize()
+
     # both paths first render the largest requests once, so that neither
     # times the growth of its buffers
     big = sorted(reqs, key=lambda r: -(pool[r[1]].numel() * len(r[2]) + r[3] ** 2))[:3]
@@ -393,81 +449,56 @@ def served_stream(api, fields, nreq, rate, nfield_pool, seed=0):
         api.execute_rgb(im["camera"], im["light"], im["size"], im["center"], im["local"], im["bg"], im["contours"],
                         precision=api.FP32, rgb=im["rgb"])
 
-    # serving loop
-    lock = threading.Lock()
-    inflight = [0] * nfield_pool
-    lat = []
-    srv = api.Server(max_images=16, hbm_budget=200e9)
-    order = []
-    done = threading.Event()
+    def serve(window):
+        lat = []
+        order = []
+        lock = threading.Lock()
+        done = threading.Event()
+        srv = api.Server(max_images=16, hbm_budget=200e9, max_wait_ms=window, snapshot_fields=True)
 
-    def reaper():
-        k = 0
-        while True:
-            with lock:
-                if k >= len(order):
-                    if done.is_set():
+        def reaper():
+            k = 0
+            while True:
+                with lock:
+                    item = order[k] if k < len(order) else None
+                    if item is None and done.is_set():
                         return
-                    item = None
-                else:
-                    item = order[k]
-            if item is None:
-                time.sleep(0.0005)
-                continue
-            ticket, fi = item
-            _, _, ms = srv.wait(ticket)
-            with lock:
-                inflight[fi] -= 1
-                lat.append(ms)
-            k += 1
+                if item is None:
+                    time.sleep(0.0005)
+                    continue
+                lat.append(srv.wait(item)[2])
+                k += 1
 
-    for r in big:
-        srv.wait(srv.submit(image_of(r), api.FP32))
-    th = threading.Thread(target=reaper)
-    th.start()
-    t0 = time.perf_counter()
-    updates = 0
-    for q, r in enumerate(reqs):
-        while time.perf_counter() - t0 < r[0]:
-            time.sleep(0.0002)
-        if q % 8 == 7:
-            # a solver step: advance one field that no queued request reads
-            with lock:
-                free = [i for i in range(nfield_pool) if inflight[i] == 0]
-            if free:
-                pool[free[q % len(free)]].neg_()
-                torch.cuda.synchronize()
-                updates += 1
-        im = image_of(r)
-        with lock:
-            inflight[r[1]] += 1
-        ticket = srv.submit(im, api.FP32)
-        with lock:
-            order.append((ticket, r[1]))
-    done.set()
-    th.join()
-    st = srv.stats()
-    # saturation: every request submitted at once; the images (with their
-    # host buffers) are built once, and each path keeps its best of three
-    # passes (the first sizes the buffers for the larger batches)
-    nsat = min(48, nreq)
-    sat_imgs = [image_of(r) for r in reqs[:nsat]]
-    st0 = srv.stats()
-    sat_served = 0.0
-    for rep in range(3):
-        t1 = time.perf_counter()
-        ts = [srv.submit(im, api.FP32) for im in sat_imgs]
-        for t_ in ts:
-            srv.wait(t_)
-        sat_served = max(sat_served, nsat / (time.perf_counter() - t1))
-    st1 = srv.stats()
-    sat_batch = (st1["completed"] - st0["completed"]) / max(1, st1["batches"] - st0["batches"])
-    srv.stop()
-    lat.sort()
+        try:
+            for r in big:
+                srv.wait(srv.submit(image_of(r), api.FP32))
+            st0 = srv.stats()
+            th = threading.Thread(target=reaper)
+            th.start()
+            t0 = time.perf_counter()
+            for r in reqs:
+                while time.perf_counter() - t0 < r[0]:
+                    time.sleep(0.0002)
+                ticket = srv.submit(image_of(r), api.FP32)
+                with lock:
+                    order.append(ticket)
+                solver_step(r[1])  # at once: the request renders its snapshot
+            done.set()
+            th.join()
+            span = time.perf_counter() - t0
+            st = srv.stats()
+        finally:
+            srv.stop()
+        lat.sort()
+        pct = lambda v, q: v[min(len(v) - 1, int(q * (len(v) - 1) + 0.5))]  # noqa: E731
+        return {"max_wait_ms": window, "p50_ms": round(pct(lat, 0.5), 2), "p99_ms": round(pct(lat, 0.99), 2),
+                "frames_per_s": round(nreq / span, 2),
+                "mean_batch": round((st["completed"] - st0["completed"]) / max(1, st["batches"] - st0["batches"]),
+                                    2),
+                "peak_admitted_gb": round(st["peak_admitted_bytes"] / 1e9, 2), "failed": st["failed"]}
+
     served = {"requests": nreq, "rate_per_s": rate, "precision": "fp32", "field_pool": nfield_pool,
-              "solver_updates": updates, "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
-              "frames_per_s": round(st["frames_per_s"], 2), "mean_batch": round(st["mean_batch"], 2),
-              "peak_admitted_gb": round(st["peak_admitted_bytes"] / 1e9, 2)}
+              "solver_steps": nreq, "snapshot_fields": True, "runs": [serve(w) for w in windows]}
     # the same arrivals through plain single-image calls, one at a time
     plat = []
     t0 = time.perf_counter()
@@ -478,11 +509,30 @@ def served_stream(api, fields, nreq, rate, nfield_pool, seed=0):
         api.execute_rgb(im["camera"], im["light"], im["size"], im["center"], im["local"], im["bg"], im["contours"],
                         precision=api.FP32, rgb=im["rgb"])
         plat.append((time.perf_counter() - t0 - r[0]) * 1e3)
+        solver_step(r[1])
     span = time.perf_counter() - t0
     plat.sort()
     pct = lambda v, q: v[min(len(v) - 1, int(q * (len(v) - 1) + 0.5))]  # noqa: E731
     served["plain"] = {"p50_ms": round(pct(plat, 0.5), 2), "p99_ms": round(pct(plat, 0.99), 2),
                        "frames_per_s": round(nreq / span, 2)}
+    # saturation: every request submitted at once; the images (with their
+    # host buffers) are built once, and each path keeps its best of three
+    # passes (the first sizes the buffers for the larger batches)
+    nsat = min(48, nreq)
+    sat_imgs = [image_of(r) for r in reqs[:nsat]]
+    sat_served, sat_batch = 0.0, 0.0
+    with api.Server(max_images=16, hbm_budget=200e9) as srv:
+        for rep in range(3):
+            st0 = srv.stats()
+            t1 = time.perf_counter()
+            ts = [srv.submit(im, api.FP32) for im in sat_imgs]
+            for t_ in ts:
+                srv.wait(t_)
+            rate_ = nsat / (time.perf_counter() - t1)
+            st1 = srv.stats()
+            if rate_ > sat_served:
+                sat_served = rate_
+                sat_batch = (st1["completed"] - st0["completed"]) / max(1, st1["batches"] - st0["batches"])
     sat_plain = 0.0
     for rep in range(3):
         t1 = time.perf_counter()
@@ -493,6 +543,23 @@ def served_stream(api, fields, nreq, rate, nfield_pool, seed=0):
     served["saturated"] = {"requests": nsat, "served_frames_per_s": round(sat_served, 2),
                            "plain_frames_per_s": round(sat_plain, 2), "mean_batch": round(sat_batch, 2)}
     return served
+
+
+def c1_frames(api, fields, steps, warmup):
+    """The plain reference method at its own size (C1): 128^3 Gaussian
+    blobs, threshold 0.5, one 800x600 image, FP64, in HBM."""
+    import torch
+
+    f = torch.from_numpy(fields.blobs((128, 128, 128), seed=0)).cuda()
+    v = fields.example_view(800, 600)
+    v["bg"] = (0, 0, 0)
+    cont = [dict(field=f, threshold=0.5, color=(255, 200, 60))]
+    rgb = torch.zeros((600, 800, 3), dtype=torch.uint8, device="cuda")
+    dt, ms, T = run_form(api, cont, v, api.FP64, steps, warmup, rgb)
+    cells = 127 ** 3
+    return {"lattice": [128, 128, 128], "threshold": 0.5, "image": [800, 600], "precision": "fp64",
+            "triangles": int(T), "ms_per_frame": round(dt / steps * 1e3, 4),
+            "frames_per_s": round(steps / dt, 1), "gcells_per_s": round(cells * steps / dt / 1e9, 3)}
 
 
 def precision_study(api, fields, n, side):
@@ -570,8 +637,9 @@ def main():
     ap.add_argument("--n", type=int, default=N, help="lattice points per axis (default 512)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 1 GiB device-to-device three times (reference kernel for HBM counters)")
-    ap.add_argument("--forms", default="batched,served,precision",
-                    help="comma list of extra forms beside the headline frame: batched, served, precision (or none)")
+    ap.add_argument("--forms", default="c1,batched,served,precision",
+                    help="comma list of extra forms beside the headline frame: c1, batched, served, precision "
+                         "(or none)")
     ap.add_argument("--batch-fields", type=int, default=64)
     ap.add_argument("--served-requests", type=int, default=150)
     ap.add_argument("--precision-n", type=int, default=1024)
@@ -612,27 +680,36 @@ def main():
         print(f"[bench] {name}: {args.warmup} warmup + {args.steps} timed frames", file=sys.stderr, flush=True)
         dt, ms, T = run_form(api, contours, view, prec, args.steps, args.warmup, rgb_dev)
         stages = {}
-        for s in api.STAGES:
-            if s in ("smooth", "emit"):
-                continue  # frames store no mesh: the fused raster stage tessellates
-            per = ms[s]
-            b = algorithmic_bytes("fused" if s == "raster" else s, T, npts, npx, prec == api.FP64)
-            stages[s] = dict(ms=round(per, 4), gbs=round(b / (per * 1e-3) / 1e9, 1) if per > 0 and b else None)
+        for s_ in api.STAGES:
+            if s_ in ("smooth", "emit"):
+                continue  # frames store no mesh: the fused kernel tessellates
+            per = ms[s_]
+            b = algorithmic_bytes(s_, T, npts, npx, prec == api.FP64) if s_ != "raster" else 0
+            stages[s_] = dict(ms=round(per, 4), gbs=round(b / (per * 1e-3) / 1e9, 1) if per > 0 and b else None)
         results[name] = dict(dt=dt, T=T, stages=stages, frame_ms=dt / args.steps * 1e3)
 
     r = results["fp64"]
-    dom = max(("count", "raster", "resolve"), key=lambda s: r["stages"][s]["ms"])
-    dom_b = algorithmic_bytes("fused" if dom == "raster" else dom, r["T"], npts, npx, True)
-    achieved = dom_b / (r["stages"][dom]["ms"] * 1e-3) / 1e9
-    log("stored-mesh tessellation (mesh build), FP64 and FP32")
+    # the dominant kernel of the FP64 frame: the fused tessellation + raster
+    # kernel, timed alone (HIP events around its launch on the frame stream)
+    fused_ms = r["stages"]["fused"]["ms"]
+    dom_b = algorithmic_bytes("fused", r["T"], npts, npx, True)
+    achieved = dom_b / (fused_ms * 1e-3) / 1e9
+    prof = profile_summary() if n == 512 else {}
+    kprof = prof.get("kernels", {}).get("k_tess_raster<double,3>", {})
+    log("stored-mesh tessellation (mesh build) and smoothing, FP64 and FP32")
     tess = {}
     for name, prec in (("fp64", api.FP64), ("fp32", api.FP32)):
         st, tri = mesh_tessellation(api, contours, prec, 3)
         t_ms = st["count"] + st["scan"] + st["emit"]
         b = algorithmic_bytes("emit", tri, npts, npx, prec == api.FP64)
-        tess[name] = {"ms": round(t_ms, 4), "count_ms": round(st["count"], 4), "emit_ms": round(st["emit"], 4),
-                      "smooth_ms": round(st["smooth"], 4), "gcells_per_s": round(cells / (t_ms * 1e-3) / 1e9, 2),
-                      "algorithmic_bytes": round(b), "hbm_frac": round(b / (t_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+        bs = algorithmic_bytes("smooth", tri, npts, npx, prec == api.FP64)
+        tess[name] = {"ms": round(t_ms, 4), "count_ms": round(st["count"], 4), "scan_ms": round(st["scan"], 4),
+                      "emit_ms": round(st["emit"], 4), "triangles": int(tri),
+                      "gcells_per_s": round(cells / (t_ms * 1e-3) / 1e9, 2), "algorithmic_bytes": round(b),
+                      "hbm_frac": round(b / (t_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                      "smooth_ms": round(st["smooth"], 4), "smooth_algorithmic_bytes": round(bs),
+                      "smooth_hbm_frac": round(bs / (st["smooth"] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
+                      if st["smooth"] > 0 else None}
     out = {
         "metric": "Gcells/s + frames/s at 512^3 fp64; served p50/p99 latency vs plain GPU path",
         "value": round(cells * args.steps / r["dt"] / 1e9, 3),
@@ -651,11 +728,13 @@ def main():
                                f"thresholds {{-1,0,1}}, {W}x{H} image in HBM",
                    "lattice": [n, n, n], "thresholds": list(THRESHOLDS), "image": [W, H],
                    "triangles_per_frame": int(r["T"])},
-        "roofline": {"bound": "hbm", "kernel": "k_tess_raster (fused tessellation + raster stage)" if dom == "raster"
-                     else dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": MEASURED_TRAFFIC_512.get(dom) if n == 512 else None,
-                     "algorithmic_bytes": round(dom_b)},
+        "roofline": {"bound": "hbm", "kernel": "k_tess_raster<double> (fused tessellation + raster, FP64 frame)",
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": kprof.get("hbm_bytes"),
+                     "algorithmic_bytes": round(dom_b), "kernel_ms": fused_ms,
+                     "algorithmic": "B_tess = 8 B x lattice points + 96 B x triangles",
+                     "limiter": kprof.get("limiter"), "valu": kprof.get("valu"),
+                     "profile": os.path.relpath(PROFILE_MD, ROOT) if kprof else None},
         "tessellation": tess,
         "stages_fp64": r["stages"],
         "fp32": {"value": round(cells * args.steps / results["fp32"]["dt"] / 1e9, 3), "unit": "Gcells/s",
@@ -673,6 +752,9 @@ def main():
             log(f"{name} failed: {e!r}")
             return {"error": repr(e)}
 
+    if "c1" in forms:
+        log("c1: 128^3 blobs, 800x600")
+        out["c1"] = form("c1", lambda: c1_frames(api, fields, args.steps, args.warmup))
     if "batched" in forms:
         log("batched: four cameras on the shared 512^3 scene")
         out["batched"] = {"shared_scene": form("shared_scene", lambda: shared_scene(

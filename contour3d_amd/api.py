@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libcontour3d.so")
 
 FP64 = 0
 FP32 = 1
-STAGES = ("count", "scan", "emit", "smooth", "raster", "resolve")
+STAGES = ("count", "scan", "emit", "smooth", "raster", "resolve", "fused")
 
 # every entry point declared in include/contour3d.h and include/contour3d_amd.h
 EXPORTS = (
@@ -41,6 +41,7 @@ EXPORTS = (
     "contour3d_amd_mesh_copy",
     "contour3d_amd_mesh_render",
     "contour3d_amd_mesh_compare",
+    "contour3d_amd_mesh_normals",
     "contour3d_amd_mesh_free",
     "contour3d_amd_tune",
     "contour3d_amd_profile_enable",
@@ -94,6 +95,7 @@ def lib():
                                                 C.c_void_p, C.c_void_p]
         L.contour3d_amd_mesh_free.argtypes = [C.c_void_p]
         L.contour3d_amd_mesh_compare.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+        L.contour3d_amd_mesh_normals.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p]
         L.contour3d_amd_profile_enable.argtypes = [C.c_int]
         L.contour3d_amd_profile_read.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
@@ -363,6 +365,7 @@ class Mesh:
             raise RuntimeError("contour3d_amd_mesh_build failed")
         self.handle = h
         self.n = len(contours)
+        self.precision = int(precision)
 
     def count(self, obj):
         n = C.c_size_t()
@@ -396,6 +399,17 @@ class Mesh:
         if rc != 0:
             raise RuntimeError("mesh_render failed")
         return rgb, depth
+
+    def normals(self, obj, out=None):
+        """contour3d_amd_mesh_normals: the unit vertex normals of every
+        triangle of object `obj`, shape (T, 3, 3), in the mesh's precision;
+        `out` may be a device tensor (filled in place)."""
+        T = self.count(obj)
+        if out is None:
+            out = np.zeros((T, 3, 3), np.float64 if self.precision == FP64 else np.float32)
+        if T and lib().contour3d_amd_mesh_normals(self.handle, obj, _addr(out)) != 0:
+            raise RuntimeError("mesh_normals failed")
+        return out
 
     def compare(self, other, obj):
         """contour3d_amd_mesh_compare of this FP64 mesh with `other`."""
@@ -439,6 +453,8 @@ TUNE_RASTER_GRID = 1
 TUNE_FRAG_CAPACITY = 2
 TUNE_GRAPHS = 3
 TUNE_FAIL_ALLOC = 4
+TUNE_TESS_ROWS = 5
+TUNE_TESS_ZPLANES = 6
 
 
 def tune(param, value):
@@ -448,31 +464,44 @@ def tune(param, value):
         raise RuntimeError("contour3d_amd_tune failed")
 
 
-def autotune(run, tess=(0, 8192, 65536, 1 << 30), raster=(0, 4096, 16384, 32768), reps=2):
-    """Times run() under each tessellation and raster workgroup cap (results
-    are the same bits under every setting) and keeps the fastest pair.
-    Returns ({"tess": t, "raster": r}, best_ms)."""
+AUTOTUNE_KNOBS = {
+    TUNE_TESS_GRID: (0, 16384, 1 << 30),   # persistent workgroups (0: 128 per CU), or one per work item
+    TUNE_TESS_ROWS: (0, 2, 1),             # rows of cells per tile (0: as many as fit in 2048 cells)
+    TUNE_TESS_ZPLANES: (4, 8, 16, 32),     # z-planes per work item
+}
+KNOB_NAMES = {TUNE_TESS_GRID: "tess_grid", TUNE_TESS_ROWS: "tess_rows", TUNE_TESS_ZPLANES: "tess_zplanes"}
+
+
+def autotune(run, knobs=None, reps=2):
+    """Times run() under every combination of the launch-shape knobs (a dict
+    tune parameter -> values; default AUTOTUNE_KNOBS: the tessellation
+    workgroup cap and brick shape, which the frame path reads) and keeps the
+    fastest.  Every setting renders the same bits.  Returns (best settings
+    by knob name, best ms, the sweep as a list of (settings, ms))."""
+    import itertools
     import time
 
     import torch
 
-    best = None
-    for t in tess:
-        for r in raster:
-            tune(TUNE_TESS_GRID, t)
-            tune(TUNE_RASTER_GRID, r)
+    knobs = AUTOTUNE_KNOBS if knobs is None else knobs
+    params = list(knobs)
+    combos = list(itertools.product(*(knobs[p] for p in params)))
+    sweep = []
+    for values in combos:
+        for p, v in zip(params, values):
+            tune(p, v)
+        run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
             run()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(reps):
-                run()
-            torch.cuda.synchronize()
-            ms = (time.perf_counter() - t0) / reps * 1e3
-            if best is None or ms < best[1]:
-                best = ({"tess": t, "raster": r}, ms)
-    tune(TUNE_TESS_GRID, best[0]["tess"])
-    tune(TUNE_RASTER_GRID, best[0]["raster"])
-    return best
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        sweep.append(({KNOB_NAMES.get(p, str(p)): v for p, v in zip(params, values)}, ms))
+    best = min(range(len(sweep)), key=lambda q: sweep[q][1])
+    for p, v in zip(params, combos[best]):
+        tune(p, v)
+    return sweep[best][0], sweep[best][1], sweep
 
 
 def set_precision(p):

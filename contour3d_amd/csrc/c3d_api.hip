@@ -161,6 +161,8 @@ struct Prof {
   std::vector<hipEvent_t> pool;
   hipEvent_t cur_a = nullptr;
   int cur_stage = -1;
+  hipEvent_t in_a = nullptr;  // a stage timed inside the current one
+  int in_stage = -1;
 };
 
 }  // namespace
@@ -198,7 +200,7 @@ constexpr uint32_t kDefaultFragCap = 1u << 24;
 // Host-side layout of a frame's tessellation.
 struct TessLayout {
   uint64_t nbt = 0;  // tile-count entries (the objects' tiles)
-  uint32_t ntj = 0;  // tiles of the field passes
+  uint32_t ntj = 0;  // work items of the field passes
   int njobs = 0;     // field passes
   int nobj = 0;
 };
@@ -290,14 +292,18 @@ struct Lib {
   // workgroups of the tessellation and raster kernels (0: built-in choice)
   uint32_t tess_cap = 0;
   uint32_t raster_cap = 0;
+  // tessellation brick shape: rows of cells per tile (0: as many as fit in
+  // kTileCells) and planes per work item (0: kDefaultZPlanes)
+  uint32_t tess_rows = 0;
+  uint32_t tess_zc = 0;
   bool graphs = false;  // replay captured launch sequences of repeated frame layouts
   // Tessellation launches are persistent: by default 128 workgroups per CU,
   // each walking several tiles, so that its triangle queue carries items
   // across tiles and drains part-full once, at its end (measured at 512^3:
   // the fused stage 5.6 -> 5.3 ms against one workgroup per tile)
-  uint32_t tess_grid(uint32_t ntiles) const {
+  uint32_t tess_grid(uint32_t nitems) const {
     const uint32_t cap = tess_cap ? tess_cap : uint32_t(persistent_blocks()) * 16u;
-    const uint32_t g = cap < ntiles ? cap : ntiles;
+    const uint32_t g = cap < nitems ? cap : nitems;
     return g < 8 ? 8 : g;
   }
   Frame main;
@@ -354,6 +360,35 @@ int prof_end(Frame& F) {
   C3D_HIP(hipEventRecord(b, F.st));
   P.pending.push_back({P.cur_stage, P.cur_a, b});
   P.cur_stage = -1;
+  return 0;
+}
+
+// a stage timed within the open one (the fused kernel inside the raster stage)
+int prof_inner_begin(Frame& F, int stage) {
+  Prof& P = F.prof;
+  if (!P.on) return 0;
+  if (P.pool.size() < 2) {
+    for (int q = 0; q < 8; q++) {
+      hipEvent_t e;
+      C3D_HIP(hipEventCreate(&e));
+      P.pool.push_back(e);
+    }
+  }
+  P.in_a = P.pool.back();
+  P.pool.pop_back();
+  P.in_stage = stage;
+  C3D_HIP(hipEventRecord(P.in_a, F.st));
+  return 0;
+}
+
+int prof_inner_end(Frame& F) {
+  Prof& P = F.prof;
+  if (!P.on || P.in_stage < 0) return 0;
+  hipEvent_t b = P.pool.back();
+  P.pool.pop_back();
+  C3D_HIP(hipEventRecord(b, F.st));
+  P.pending.push_back({P.in_stage, P.in_a, b});
+  P.in_stage = -1;
   return 0;
 }
 
@@ -529,7 +564,7 @@ int validate_obj(const contour3d_contour_obj_t& o, size_t n) {
     return 1;
   }
   const uint64_t nc = uint64_t(o.glsizes[0] - 1) * uint64_t(o.glsizes[1] - 1) * uint64_t(o.glsizes[2] - 1);
-  if (nc >= (uint64_t(1) << 32) - kCellsPerBlock || o.glsizes[0] > (1u << 30) || o.glsizes[1] > (1u << 30) ||
+  if (nc >= (uint64_t(1) << 32) - kTileCells || o.glsizes[0] > (1u << 30) || o.glsizes[1] > (1u << 30) ||
       o.glsizes[2] > (1u << 30)) {
     log_error("contour object %zu: %zu x %zu x %zu lattice exceeds 2^32 cells", n, o.glsizes[0], o.glsizes[1],
               o.glsizes[2]);
@@ -766,9 +801,11 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
     f.ncy = f.ny - 1;
     f.ncz = f.nz - 1;
     f.ncells = f.ncx * f.ncy * f.ncz;
-    f.nblocks = (f.ncells + kCellsPerBlock - 1) / kCellsPerBlock;
     f.div_ncx = make_fastdiv(f.ncx);
     f.div_ncy = make_fastdiv(f.ncy);
+    f.tg = make_tile_geo(f.ncx, f.ncy, f.ncz, L.tess_rows, L.tess_zc);
+    f.ntiles = f.ncz * f.tg.cols;  // at most the cell count
+    f.nitems = f.tg.cols * f.tg.nzc;
     if (is_device_ptr(ob.array)) {
       f.f = ob.array;
     } else {
@@ -802,11 +839,11 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
       f.cy = f.cx + npts;
       f.cz = f.cy + npts;
     }
-    if (uint64_t(boff[o]) + f.nblocks >= (uint64_t(1) << 31)) {
+    if (uint64_t(boff[o]) + f.ntiles >= (uint64_t(1) << 31)) {
       log_error("frame exceeds 2^31 tessellation tiles");
       return 1;
     }
-    boff[o + 1] = boff[o] + f.nblocks;
+    boff[o + 1] = boff[o] + f.ntiles;
     for (size_t q = 0; q < o && group[o] < 0; q++) {
       if (fd[q].f == f.f && fd[q].nx == f.nx && fd[q].ny == f.ny && fd[q].nz == f.nz && fd[q].gx == f.gx &&
           fd[q].cx == f.cx)
@@ -842,24 +879,24 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
       F.obj_q[q] = uint32_t(job.ts.n);
       job.ts.n++;
       if (job.ts.n == kMaxThr) {
-        job.tile0 = uint32_t(jt);
+        job.item0 = uint32_t(jt);
         job.xoff = uint32_t(jx);
-        jt += fd[o].nblocks;
-        jx += (fd[o].nblocks + 7) / 8;
+        jt += fd[o].nitems;
+        jx += (fd[o].nitems + 7) / 8;
         tj.push_back(job);
         job.ts = ThrSet{};
       }
     }
     if (job.ts.n) {
-      job.tile0 = uint32_t(jt);
+      job.item0 = uint32_t(jt);
       job.xoff = uint32_t(jx);
-      jt += fd[o].nblocks;
-      jx += (fd[o].nblocks + 7) / 8;
+      jt += fd[o].nitems;
+      jx += (fd[o].nitems + 7) / 8;
       tj.push_back(job);
     }
   }
   if (jt >= (uint64_t(1) << 31)) {
-    log_error("frame exceeds 2^31 tessellation tiles");
+    log_error("frame exceeds 2^31 tessellation work items");
     return 1;
   }
   const uint32_t ntj = uint32_t(jt);
@@ -878,6 +915,9 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
     hm[o].ncx = fd[o].ncx;
     hm[o].ncy = fd[o].ncy;
     hm[o].ncz = fd[o].ncz;
+    hm[o].div_ncx = fd[o].div_ncx;
+    hm[o].div_ncy = fd[o].div_ncy;
+    hm[o].tg = fd[o].tg;
     hm[o].tile_off = boff[o];
     hm[o].job = F.obj_job[o];
     hm[o].q = F.obj_q[o];
@@ -1239,8 +1279,10 @@ int fused_launch(Lib& L, Frame& F, const contour3d_amd_mesh& M, const RasterJob<
     // overflow of the list (which grows for the next frame) the z-test
     // passes over the geometry run instead (each kernel decides on the
     // device)
+    if (prof_inner_begin(F, CONTOUR3D_AMD_STAGE_FUSED)) return 1;
     C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterAppend, key, id,
                                   frags, F.frag_cap, grid, F.st));
+    if (prof_inner_end(F)) return 1;
     C3D_HIP(launch_frag_keys(frags, F.frag_cap, hdr, key, fgrid, F.st));
     C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterKey64, key, id,
                                   nullptr, 0, grid, F.st));
@@ -1251,8 +1293,10 @@ int fused_launch(Lib& L, Frame& F, const contour3d_amd_mesh& M, const RasterJob<
     C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 1, 0, key, id, F.st));
   } else {
     // one pass: the packed key carries the id
+    if (prof_inner_begin(F, CONTOUR3D_AMD_STAGE_FUSED)) return 1;
     C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterFp32, key, id,
                                   nullptr, 0, grid, F.st));
+    if (prof_inner_end(F)) return 1;
     C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 1, key, id, F.st));
   }
   if (prof_end(F)) return 1;
@@ -1292,7 +1336,7 @@ int graph_frame(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& img
   const uint32_t lines = line_capacity(F);
   const std::vector<uint64_t> sig = {
       sizeof(R), uint64_t(njobs), npx, F.tl.nbt, F.tl.ntj, uint64_t(F.tl.njobs), uint64_t(F.tl.nobj),
-      L.tess_cap, L.tess_grid(F.tl.ntj), lines, M.cap, F.frag_cap,
+      L.tess_cap, L.tess_grid(F.tl.ntj), L.tess_rows, L.tess_zc, lines, M.cap, F.frag_cap,
       uint64_t(uintptr_t(F.cnt.p)), uint64_t(uintptr_t(F.tiles.p)), uint64_t(uintptr_t(F.tjobs.p)),
       uint64_t(uintptr_t(F.rjobs.p)), uint64_t(uintptr_t(F.hdr.p)), uint64_t(uintptr_t(F.jimg.p)),
       uint64_t(uintptr_t(F.key.p)), uint64_t(uintptr_t(F.id.p)), uint64_t(uintptr_t(F.frags.p)),
@@ -1856,6 +1900,30 @@ int write_ppm(const char* fname, size_t W, size_t H, const uint8_t* rgb) {
   return 0;
 }
 
+// Unit vertex normals of every triangle of object `obj` of mesh M into the
+// device buffer out (9 values of the mesh's precision per triangle), timed
+// as the SMOOTH stage, and waited for (normal.c:46-165).
+int smooth_object(Frame& F, const contour3d_amd_mesh& M, size_t obj, void* out) {
+  const uint64_t n = M.count[obj];
+  if (n == 0) return 0;
+  const ObjMeta* md = M.meta.as<ObjMeta>() + obj;
+  if (prof_begin(F, CONTOUR3D_AMD_STAGE_SMOOTH)) return 1;
+  hipError_t e = M.precision == CONTOUR3D_AMD_FP64
+                     ? launch_vertex_normals<double>(view_of<double>(M), md, n, static_cast<double*>(out), F.st)
+                     : launch_vertex_normals<float>(view_of<float>(M), md, n, static_cast<float*>(out), F.st);
+  if (e != hipSuccess) {
+    log_error("vertex normals: %s", hipGetErrorString(e));
+    return 1;
+  }
+  if (prof_end(F)) return 1;
+  e = hipStreamSynchronize(F.st);
+  if (e != hipSuccess) {
+    log_error("vertex normals failed on the device: %s", hipGetErrorString(e));
+    return 1;
+  }
+  return prof_collect(F);
+}
+
 // The reference runs one pencil per MPI rank and reads each rank's local
 // extent from SimpleDecomp (extend.c:158-181); this library renders every
 // contour object's array as the whole field.  Under an MPI launch of more
@@ -1944,6 +2012,12 @@ int contour3d_amd_tune(int param, int64_t value) {
       return 0;
     case CONTOUR3D_AMD_TUNE_FAIL_ALLOC:
       c3d::g_fail_alloc.store(value);
+      return 0;
+    case CONTOUR3D_AMD_TUNE_TESS_ROWS:
+      L.tess_rows = uint32_t(value);
+      return 0;
+    case CONTOUR3D_AMD_TUNE_TESS_ZPLANES:
+      L.tess_zc = uint32_t(value);
       return 0;
     default:
       c3d::log_error("tune: unknown parameter %d", param);
@@ -2121,24 +2195,43 @@ int contour3d_amd_mesh_copy(const contour3d_amd_mesh_t* mesh, size_t obj, double
     // the mesh keeps no per-triangle vertex normals; they are gathered now
     // with the same sums the resolve pass uses for nearest triangles
     if (c3d::ensure_init(L, 0)) return 1;
-    if (L.main.vntmp.ensure(n * 9 * sizeof(double)) != hipSuccess) {
+    if (L.main.vntmp.ensure(n * 9 * rs) != hipSuccess) {
       c3d::log_error("mesh_copy: allocation failed");
       return 1;
     }
-    const c3d::ObjMeta* md = mesh->meta.as<c3d::ObjMeta>() + obj;
-    hipError_t e = mesh->precision == CONTOUR3D_AMD_FP64
-                       ? c3d::launch_vertex_normals<double>(c3d::view_of<double>(*mesh), md, n,
-                                                            L.main.vntmp.as<double>(), L.main.st)
-                       : c3d::launch_vertex_normals<float>(c3d::view_of<float>(*mesh), md, n,
-                                                           L.main.vntmp.as<double>(), L.main.st);
-    if (e == hipSuccess) e = hipStreamSynchronize(L.main.st);
-    if (e == hipSuccess) e = hipMemcpy(vn, L.main.vntmp.p, n * 9 * sizeof(double), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) {
-      c3d::log_error("mesh_copy: vertex normals failed: %s", hipGetErrorString(e));
+    if (c3d::smooth_object(L.main, *mesh, obj, L.main.vntmp.p)) return 1;
+    std::vector<unsigned char> raw(n * 9 * rs);
+    if (hipMemcpy(raw.data(), L.main.vntmp.p, n * 9 * rs, hipMemcpyDeviceToHost) != hipSuccess) {
+      c3d::log_error("mesh_copy: device copy failed");
       return 1;
     }
+    for (uint64_t q = 0; q < n * 9; q++) vn[q] = real_at(raw.data() + q * rs);
   }
   if (cell && hipMemcpy(cell, mesh->cell.as<uint32_t>() + b, n * 4, hipMemcpyDeviceToHost) != hipSuccess) return 1;
+  return 0;
+}
+
+int contour3d_amd_mesh_normals(const contour3d_amd_mesh_t* mesh, size_t obj, void* vn) {
+  if (!mesh || !vn || obj >= mesh->count.size()) {
+    c3d::log_error("mesh_normals: bad arguments");
+    return 1;
+  }
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  if (c3d::ensure_init(L, 0)) return 1;
+  const uint64_t n = mesh->count[obj];
+  if (n == 0) return 0;
+  const size_t bytes = size_t(n) * 9 * (mesh->precision == CONTOUR3D_AMD_FP64 ? 8 : 4);
+  if (c3d::is_device_ptr(vn)) return c3d::smooth_object(L.main, *mesh, obj, vn);
+  if (L.main.vntmp.ensure(bytes) != hipSuccess) {
+    c3d::log_error("mesh_normals: allocation failed");
+    return 1;
+  }
+  if (c3d::smooth_object(L.main, *mesh, obj, L.main.vntmp.p)) return 1;
+  if (hipMemcpy(vn, L.main.vntmp.p, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+    c3d::log_error("mesh_normals: device copy failed");
+    return 1;
+  }
   return 0;
 }
 

@@ -8,20 +8,14 @@
 #include <cstdint>
 
 #include "c3d_geom.h"
+#include "c3d_tiles.h"
 
 namespace c3d {
 
 constexpr int kThreads = 256;           // one workgroup = 4 wave64
-constexpr int kCellsPerBlock = 1024;    // tessellation tile: 4 chunks of 256 cells
+constexpr int kTileChunks = kTileCells / kThreads;  // chunks of 256 cells per tile
 constexpr int kMaxThr = 4;              // thresholds classified per field read
 constexpr int kScanTile = 1024;         // entries per workgroup of the tile-count scan
-
-// Division by a runtime constant with a multiply-high (Granlund-Montgomery,
-// "round-up" variant): q = (umulhi(n, m) + n) >> s for every 32-bit n.
-struct FastDiv {
-  uint32_t d, m, s;
-};
-FastDiv make_fastdiv(uint32_t d);
 
 // One FP64 field on the device plus its lattice coordinates: either the
 // identity map of the three grids, or an SoA coordinate table (cx != null).
@@ -36,8 +30,10 @@ struct FieldDev {
   uint32_t nx, ny, nz;            // lattice points
   uint32_t ncx, ncy, ncz;         // cells
   uint32_t ncells;
-  uint32_t nblocks;               // ceil(ncells / kCellsPerBlock)
+  uint32_t ntiles;                // ncz * tg.cols
+  uint32_t nitems;                // tg.cols * tg.nzc
   FastDiv div_ncx, div_ncy;
+  TileGeo tg;
 };
 
 // Thresholds classified in one pass over a field, each with the offset of
@@ -50,13 +46,14 @@ struct ThrSet {
 };
 
 // One pass over one field: the unit of work of the tessellation kernels.
-// Jobs are stored in a device table; their tiles are numbered consecutively
-// from tile0.  xoff is the sum of ceil(tiles / 8) over the earlier jobs: the
-// job's first position in each XCD's walk over the launch's tiles.
+// Jobs are stored in a device table; their work items are numbered
+// consecutively from item0.  xoff is the sum of ceil(items / 8) over the
+// earlier jobs: the job's first position in each XCD's walk over the
+// launch's work items.
 struct TessJob {
   FieldDev f;
   ThrSet ts;
-  uint32_t tile0;
+  uint32_t item0;
   uint32_t xoff;
 };
 
@@ -76,8 +73,10 @@ struct MeshView {
 struct ObjMeta {
   uint64_t base;           // first triangle of the object in the mesh
   uint64_t count;
-  const uint64_t* bstart;  // triangle start of each tile of the object's field (nblocks + 1)
+  const uint64_t* bstart;  // triangle start of each tile of the object's field (ntiles + 1)
   uint32_t ncx, ncy, ncz;
+  FastDiv div_ncx, div_ncy;
+  TileGeo tg;              // the tiles of the object's field
   uint32_t tile_off;       // offset of the object's first tile in the frame's tile arrays
   uint32_t job, q;         // its field pass and threshold slot there
   uint8_t rgb[4];
@@ -134,7 +133,7 @@ struct FrameHdr {
 hipError_t upload_tables(const NbrTable& nt);
 int persistent_blocks();  // workgroups that fill the device once (multiple of 8)
 
-hipError_t launch_count(const TessJob* jobs, int njobs, uint32_t ntiles, uint32_t* cnt, uint32_t grid,
+hipError_t launch_count(const TessJob* jobs, int njobs, uint32_t nitems, uint32_t* cnt, uint32_t grid,
                         hipStream_t st);
 hipError_t launch_scan(const uint32_t* cnt, uint64_t* start, uint64_t n, uint64_t* tile_tmp,
                        hipStream_t st);
@@ -144,13 +143,14 @@ template <typename R>
 hipError_t launch_plan(const uint64_t* start, uint64_t ntiles, ObjMeta* meta, int nobj, RasterJob<R>* rjobs,
                        int njobs, FrameHdr* hdr, uint64_t cap, hipStream_t st);
 template <typename R>
-hipError_t launch_emit(const TessJob* jobs, int njobs, uint32_t ntiles, const uint64_t* start,
+hipError_t launch_emit(const TessJob* jobs, int njobs, uint32_t nitems, const uint64_t* start,
                        const MeshView<R>& m, FrameHdr* hdr, uint32_t grid, hipStream_t st);
-// unit vertex normals of every triangle of one object, 9 doubles per
-// triangle (NaN rows for triangles outside the rendered interior)
+// unit vertex normals of every triangle of one object, 9 values of the
+// mesh's precision per triangle (NaN rows for triangles outside the
+// rendered interior)
 template <typename R>
-hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, uint64_t count,
-                                 double* out, hipStream_t st);
+hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, uint64_t count, R* out,
+                                 hipStream_t st);
 // FP64 mesh a against mesh b (FP64, or FP32 when b_fp32; b points to its
 // MeshView) for one object: out[0..4] as k_mesh_compare documents
 hipError_t launch_mesh_compare(const MeshView<double>& a, const ObjMeta* ma, const void* b, int b_fp32,
@@ -161,7 +161,7 @@ hipError_t launch_mesh_compare(const MeshView<double>& a, const ObjMeta* ma, con
 // show any of the pass's objects) and only its cell index is stored
 // (cell[cap]).  `mode` is a raster mode.
 template <typename R>
-hipError_t launch_tess_raster(const TessJob* jobs, int njobs, uint32_t ntiles, const uint64_t* start, uint32_t* cell,
+hipError_t launch_tess_raster(const TessJob* jobs, int njobs, uint32_t nitems, const uint64_t* start, uint32_t* cell,
                               uint64_t cap, const RasterJob<R>* rjobs, const uint32_t* jimg, FrameHdr* hdr, int mode,
                               uint64_t* key, uint32_t* id, Frag* frags, uint32_t fcap, uint32_t grid, hipStream_t st);
 // resolve of a fused frame: the nearest triangles and their neighbours'

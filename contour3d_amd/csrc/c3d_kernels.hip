@@ -15,20 +15,6 @@ namespace c3d {
 // ----------------------------------------------------------------------
 // small helpers
 
-FastDiv make_fastdiv(uint32_t d) {
-  FastDiv r;
-  r.d = d;
-  uint32_t s = 0;
-  while ((uint64_t(1) << s) < d) s++;
-  r.s = s;
-  r.m = uint32_t(((uint64_t(1) << 32) * ((uint64_t(1) << s) - d)) / d + 1);
-  return r;
-}
-
-__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
-  return uint32_t((uint64_t(__umulhi(n, f.m)) + n) >> f.s);
-}
-
 // ----------------------------------------------------------------------
 // block-wide exclusive scan of one value per thread (256 threads)
 
@@ -175,34 +161,86 @@ __device__ __forceinline__ uint64_t packed_counts(const ThrSet& ts, const double
 }
 
 // ----------------------------------------------------------------------
-// tessellation: count pass
+// tessellation tiles and work items
 
-// Workgroups are dispatched round-robin over the 8 XCDs.  The tiles of each
-// field pass are split into 8 contiguous segments, one per XCD, so that
-// neighbouring z-planes meet in that XCD's L2 and every XCD gets an equal
-// share of every pass, however unequal the passes of a batch are.  XCD x
-// walks positions 0 .. end-1: position p lies in the pass jb with
-// jobs[jb].xoff <= p < jobs[jb + 1].xoff, at tile x * seg + p - xoff of it
-// (seg = ceil(tiles of the pass / 8)); the XCD's workgroups stride through
+// the tile arithmetic of c3d_tiles.h on a field
+__device__ __forceinline__ TileSpan tile_span(const FieldDev& f, uint32_t k, uint32_t col) {
+  return tile_span(f.tg, f.ncx, f.ncy, f.nx, f.ny, k, col);
+}
+
+__device__ __forceinline__ uint32_t tile_corner_off(const FieldDev& f, const TileSpan&, uint32_t p) {
+  return tile_corner_off(f.tg, f.ncx, f.nx, f.div_ncx, p);
+}
+
+__device__ __forceinline__ void item_range(const FieldDev& f, uint32_t item, uint32_t& col, uint32_t& k0,
+                                           uint32_t& k1) {
+  item_range(f.tg, f.ncz, item, col, k0, k1);
+}
+
+// the tile holding cell c of an object's field (the stored mesh's lookup)
+__device__ __forceinline__ uint32_t tile_of_cell(const ObjMeta& om, uint32_t c) {
+  return tile_of_cell(om.tg, om.ncx, om.ncy, om.div_ncx, om.div_ncy, c);
+}
+
+// The 4-bit code of four corners against threshold thr: bit n = thr < v[n]
+// (triangulate.c:176-179).  A cell's 8-bit corner code is the code of its
+// lower plane (corners 0-3) | the code of its upper plane (corners 4-7) << 4,
+// and a cell's upper code is the lower code of the cell above it.
+__device__ __forceinline__ uint32_t plane_code(double thr, const double v[4]) {
+  uint32_t c = 0;
+#pragma unroll
+  for (int n = 0; n < 4; n++) c |= uint32_t(thr < v[n]) << n;
+  return c;
+}
+
+// the four corners of one plane of a cell, from its corner 0 at p
+__device__ __forceinline__ void load_plane4(const C3D_GLOBAL double* p, uint32_t nx, double v[4]) {
+  v[0] = p[0];
+  v[1] = p[1];
+  v[2] = p[nx];
+  v[3] = p[nx + 1];
+}
+
+// plane codes of up to four thresholds, 4 bits apart
+__device__ __forceinline__ uint32_t plane_codes(const ThrSet& ts, const double v[4]) {
+  uint32_t c = 0;
+#pragma unroll
+  for (int q = 0; q < kMaxThr; q++)
+    if (q < ts.n) c |= plane_code(ts.thr[q], v) << (4 * q);
+  return c;
+}
+
+// ----------------------------------------------------------------------
+// the walk over work items
+//
+// Workgroups are dispatched round-robin over the 8 XCDs.  The work items of
+// each field pass are split into 8 contiguous segments, one per XCD, so that
+// every XCD gets an equal share of every pass, however unequal the passes of
+// a batch are.  Work items are numbered column-fastest within a z-range, so
+// the workgroups an XCD runs at once march neighbouring columns over the
+// same planes (the rows they share meet in that XCD's L2).  XCD x walks
+// positions 0 .. end-1: position p lies in the pass jb with
+// jobs[jb].xoff <= p < jobs[jb + 1].xoff, at item x * seg + p - xoff of it
+// (seg = ceil(items of the pass / 8)); the XCD's workgroups stride through
 // the positions.  The grid must be a multiple of 8.
 struct XcdWalk {
   uint32_t x, begin, end, step;
 };
 
-__device__ __forceinline__ XcdWalk xcd_walk(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles) {
+__device__ __forceinline__ XcdWalk xcd_walk(const TessJob* __restrict__ jobs, int njobs, uint32_t nitems) {
   const TessJob& last = jobs[njobs - 1];
   XcdWalk w;
   w.x = blockIdx.x % 8u;
   w.begin = blockIdx.x / 8u;
-  w.end = last.xoff + (ntiles - last.tile0 + 7u) / 8u;
+  w.end = last.xoff + (nitems - last.item0 + 7u) / 8u;
   w.step = gridDim.x / 8u;
   return w;
 }
 
-// The pass and the launch-wide tile at position p of XCD x's walk; false
-// where the XCD's segment of that pass is shorter than seg (the last ones).
-__device__ __forceinline__ bool xcd_tile(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles, uint32_t x,
-                                         uint32_t p, uint32_t& jb, uint32_t& gt) {
+// The pass and its work item at position p of XCD x's walk; false where the
+// XCD's segment of that pass is shorter than seg (the last ones).
+__device__ __forceinline__ bool xcd_item(const TessJob* __restrict__ jobs, int njobs, uint32_t nitems, uint32_t x,
+                                         uint32_t p, uint32_t& jb, uint32_t& item) {
   int lo = 0, hi = njobs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -210,40 +248,69 @@ __device__ __forceinline__ bool xcd_tile(const TessJob* __restrict__ jobs, int n
     else hi = mid - 1;
   }
   jb = uint32_t(lo);
-  const uint32_t t0 = jobs[lo].tile0;
-  const uint32_t nt = (lo + 1 < njobs ? jobs[lo + 1].tile0 : ntiles) - t0;
-  const uint32_t t = x * ((nt + 7u) / 8u) + (p - jobs[lo].xoff);
-  gt = t0 + t;
-  return t < nt;
+  const uint32_t t0 = jobs[lo].item0;
+  const uint32_t nt = (lo + 1 < njobs ? jobs[lo + 1].item0 : nitems) - t0;
+  item = x * ((nt + 7u) / 8u) + (p - jobs[lo].xoff);
+  return item < nt;
 }
 
-__global__ __launch_bounds__(kThreads) void k_count(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
+// ----------------------------------------------------------------------
+// tessellation: count pass
+//
+// Per work item, each thread keeps its cells' lower-plane codes (4 bits per
+// threshold and chunk) in registers and loads only the four upper corners of
+// each cell per plane: the field is read about once.
+__global__ __launch_bounds__(kThreads) void k_count(const TessJob* __restrict__ jobs, int njobs, uint32_t nitems,
                                                     uint32_t* __restrict__ cnt) {
   __shared__ uint64_t s_wave[kThreads / 64];
   __shared__ uint8_t s_ntri[256];
   s_ntri[threadIdx.x] = c_ntri[threadIdx.x];
   __syncthreads();
-  const XcdWalk W = xcd_walk(jobs, njobs, ntiles);
-  for (uint32_t p = W.begin; p < W.end; p += W.step) {
-    uint32_t jb, gt;
-    if (!xcd_tile(jobs, njobs, ntiles, W.x, p, jb, gt)) continue;
+  const XcdWalk W = xcd_walk(jobs, njobs, nitems);
+  for (uint32_t pp = W.begin; pp < W.end; pp += W.step) {
+    uint32_t jb, item;
+    if (!xcd_item(jobs, njobs, nitems, W.x, pp, jb, item)) continue;
     const FieldDev f = jobs[jb].f;
     const ThrSet ts = jobs[jb].ts;
-    const uint32_t tile = gt - jobs[jb].tile0;
-    uint64_t acc = 0;
+    uint32_t col, k0, k1;
+    item_range(f, item, col, k0, k1);
+    const TileSpan t = tile_span(f, k0, col);
+    const uint64_t nxy = uint64_t(f.nx) * f.ny;
+    uint32_t off[kTileChunks], low[kTileChunks];
 #pragma unroll
-    for (int chunk = 0; chunk < kCellsPerBlock / kThreads; chunk++) {
-      const uint32_t c = tile * kCellsPerBlock + chunk * kThreads + threadIdx.x;
-      if (c < f.ncells) {
-        double v[8];
-        load_corners(f, locate(f, c), v);
-        uint32_t code[kMaxThr];
-        acc += packed_counts(ts, v, code, s_ntri);
+    for (int ch = 0; ch < kTileChunks; ch++) {
+      const uint32_t p = ch * kThreads + threadIdx.x;
+      off[ch] = p < t.n ? tile_corner_off(f, t, p) : 0u;
+      low[ch] = 0;
+      if (p < t.n) {
+        double v[4];
+        load_plane4(gptr(f.f) + t.l0 + off[ch], f.nx, v);
+        low[ch] = plane_codes(ts, v);
       }
     }
-    uint64_t tot;
-    block_excl_scan64(acc, s_wave, tot);
-    if (threadIdx.x < uint32_t(ts.n)) cnt[ts.boff[threadIdx.x] + tile] = uint32_t(tot >> (16 * threadIdx.x)) & 0xffffu;
+    for (uint32_t k = k0; k < k1; k++) {
+      const C3D_GLOBAL double* up = gptr(f.f) + t.l0 + uint64_t(k - k0 + 1) * nxy;
+      uint64_t acc = 0;
+#pragma unroll
+      for (int ch = 0; ch < kTileChunks; ch++) {
+        const uint32_t p = ch * kThreads + threadIdx.x;
+        if (p < t.n) {
+          double v[4];
+          load_plane4(up + off[ch], f.nx, v);
+          const uint32_t hi = plane_codes(ts, v);
+#pragma unroll
+          for (int q = 0; q < kMaxThr; q++) {
+            const uint32_t cd = ((low[ch] >> (4 * q)) & 15u) | ((hi >> (4 * q)) & 15u) << 4;
+            acc += uint64_t(q < ts.n ? s_ntri[cd] : 0u) << (16 * q);
+          }
+          low[ch] = hi;
+        }
+      }
+      uint64_t tot;
+      block_excl_scan64(acc, s_wave, tot);
+      const uint32_t tile = k * f.tg.cols + col;
+      if (threadIdx.x < uint32_t(ts.n)) cnt[ts.boff[threadIdx.x] + tile] = uint32_t(tot >> (16 * threadIdx.x)) & 0xffffu;
+    }
   }
 }
 
@@ -469,7 +536,7 @@ __device__ V3<R> vertex_normal(const MeshView<R>& m, const ObjMeta& om, uint32_t
     const int dk = c_nbr.v[e][q][0], dj = c_nbr.v[e][q][1], di = c_nbr.v[e][q][2];
     const uint32_t want = uint32_t(c_nbr.v[e][q][3]);
     const uint32_t c2 = uint32_t(int64_t(c) + dk * sz + dj * sy + di);
-    const uint32_t tl = c2 / kCellsPerBlock;
+    const uint32_t tl = tile_of_cell(om, c2);
     const uint64_t hi = om.bstart[tl + 1];
     for (uint64_t u = lower_bound_cell(m.cell, om.bstart[tl], hi, c2); u < hi && m.cell[u] == c2; u++) {
       const uint32_t eu = m.edges[u];
@@ -482,20 +549,17 @@ __device__ V3<R> vertex_normal(const MeshView<R>& m, const ObjMeta& om, uint32_t
 
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_vertex_normals(MeshView<R> m, const ObjMeta* __restrict__ meta,
-                                                             uint64_t count, double* __restrict__ out) {
+                                                             uint64_t count, R* __restrict__ out) {
   const uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
   if (t >= count) return;
   const ObjMeta om = *meta;
   const uint64_t g = om.base + t;
   const uint32_t ed = m.edges[g];
   const bool in = (ed >> 15) & 1u;
+  const R nan = R(__longlong_as_double(0x7ff8000000000000ll));
   for (int vv = 0; vv < 3; vv++) {
-    V3<double> n = {__longlong_as_double(0x7ff8000000000000ll), __longlong_as_double(0x7ff8000000000000ll),
-                    __longlong_as_double(0x7ff8000000000000ll)};
-    if (in) {
-      const V3<R> r = vertex_normal<R>(m, om, m.cell[g], (ed >> (5 * vv)) & 31u);
-      n = {double(r.x), double(r.y), double(r.z)};
-    }
+    V3<R> n = {nan, nan, nan};
+    if (in) n = vertex_normal<R>(m, om, m.cell[g], (ed >> (5 * vv)) & 31u);
     out[9 * t + 3 * vv + 0] = n.x;
     out[9 * t + 3 * vv + 1] = n.y;
     out[9 * t + 3 * vv + 2] = n.z;
@@ -830,6 +894,7 @@ constexpr uint32_t kRing = 512;          // a power of two, at least 2 * kThread
 struct FusedLds {
   uint8_t code[kMaxThr][kThreads];
   uint16_t pref[kMaxThr][kThreads];
+  uint16_t low[kTileChunks][kThreads];  // each thread's lower-plane codes, per chunk of the tile
   uint64_t wave[kThreads / 64];
   uint8_t ntri[256];
   uint32_t rg[kRing];  // serial position of the queued triangle
@@ -944,23 +1009,25 @@ __device__ __forceinline__ void drain_ring(FusedLds& S, Frag* s_frag, uint32_t h
   flush_frags<MODE>(S, s_frag, hdr, frags, fcap, false);
 }
 
-// The tile loop shared by the queue kernels: classifies each tile's cells
-// in chunks of 256, queues the chunk's triangles in the LDS ring in serial
-// order and calls drain(head, n, jb) for every full 256 items (and for the
-// rest when the field pass changes and at the end).  With kStoreCell the
-// cell index of each triangle is stored at its serial position as it is
-// queued.  Expects S.ntri loaded and a barrier since.
+// The work-item loop shared by the queue kernels: marches each work item's
+// column up its planes, classifies each tile's cells in chunks of 256
+// (loading only the upper corners: the lower-plane codes carry over from the
+// plane below, in LDS), queues the chunk's triangles in the LDS ring in
+// serial order and calls drain(head, n, jb) for every full 256 items (and
+// for the rest when the field pass changes and at the end).  With
+// kStoreCell the cell index of each triangle is stored at its serial
+// position as it is queued.  Expects S.ntri loaded and a barrier since.
 template <bool kStoreCell, typename Drain>
-__device__ __forceinline__ void ring_tiles(FusedLds& S, const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
+__device__ __forceinline__ void ring_tiles(FusedLds& S, const TessJob* __restrict__ jobs, int njobs, uint32_t nitems,
                                            const uint64_t* __restrict__ start, uint32_t* __restrict__ cellout,
                                            uint64_t cap, Drain&& drain) {
   // ring state, the same in every thread: oldest item, items queued, and
   // the field pass they belong to
   uint32_t head = 0, npend = 0, cur = 0;
-  const XcdWalk W = xcd_walk(jobs, njobs, ntiles);
-  for (uint32_t p = W.begin; p < W.end; p += W.step) {
-    uint32_t jb, gt;
-    if (!xcd_tile(jobs, njobs, ntiles, W.x, p, jb, gt)) continue;
+  const XcdWalk W = xcd_walk(jobs, njobs, nitems);
+  for (uint32_t pp = W.begin; pp < W.end; pp += W.step) {
+    uint32_t jb, item;
+    if (!xcd_item(jobs, njobs, nitems, W.x, pp, jb, item)) continue;
     // the queue holds one field pass at a time
     while (npend && jb != cur) {
       const uint32_t n = npend < kThreads ? npend : kThreads;
@@ -971,76 +1038,105 @@ __device__ __forceinline__ void ring_tiles(FusedLds& S, const TessJob* __restric
     cur = jb;
     const FieldDev f = jobs[jb].f;
     const ThrSet ts = jobs[jb].ts;
-    const uint32_t tile = gt - jobs[jb].tile0;
-    uint64_t base[kMaxThr];
-#pragma unroll
-    for (int q = 0; q < kMaxThr; q++) base[q] = q < ts.n ? start[ts.boff[q] + tile] : 0;
-    for (int chunk = 0; chunk < kCellsPerBlock / kThreads; chunk++) {
-      const uint32_t c0 = tile * kCellsPerBlock + chunk * kThreads;
-      if (c0 >= f.ncells) break;
-      const uint32_t c = c0 + threadIdx.x;
-      const bool valid = c < f.ncells;
-      uint32_t code[kMaxThr];
-      uint64_t packed = 0;
-      if (valid) {
-        double v[8];
-        load_corners(f, locate(f, c), v);
-        packed = packed_counts(ts, v, code, S.ntri);
+    uint32_t col, k0, k1;
+    item_range(f, item, col, k0, k1);
+    const TileSpan t = tile_span(f, k0, col);
+    const uint32_t nch = (t.n + kThreads - 1) / kThreads;
+    const uint64_t nxy = uint64_t(f.nx) * f.ny;
+    const uint32_t ncxy = f.ncx * f.ncy;
+    // codes of the first plane's corners (each thread reads back only what
+    // it wrote: no barrier)
+    for (uint32_t ch = 0; ch < nch; ch++) {
+      const uint32_t p = ch * kThreads + threadIdx.x;
+      if (p < t.n) {
+        double v[4];
+        load_plane4(gptr(f.f) + t.l0 + tile_corner_off(f, t, p), f.nx, v);
+        S.low[ch][threadIdx.x] = uint16_t(plane_codes(ts, v));
       }
-      uint64_t tot;
-      const uint64_t ex = block_excl_scan64(packed, S.wave, tot);
-      uint32_t qoff[kMaxThr + 1];
-      qoff[0] = 0;
+    }
+    for (uint32_t k = k0; k < k1; k++) {
+      const uint32_t tile = k * f.tg.cols + col;
+      const uint32_t ctile = t.c0 + (k - k0) * ncxy;
+      const C3D_GLOBAL double* up = gptr(f.f) + t.l0 + uint64_t(k - k0 + 1) * nxy;
+      uint64_t base[kMaxThr];
 #pragma unroll
-      for (int q = 0; q < kMaxThr; q++) {
-        qoff[q + 1] = qoff[q] + (uint32_t(tot >> (16 * q)) & 0xffffu);
-        S.pref[q][threadIdx.x] = uint16_t(ex >> (16 * q));
-        if (valid) S.code[q][threadIdx.x] = uint8_t(code[q]);
-      }
-      __syncthreads();
-      const uint32_t total = qoff[kMaxThr];
-      for (uint32_t pos = 0; pos < total;) {
-        // queue as many of the chunk's triangles as the ring holds, in
-        // serial order (threshold, cell, slot)
-        const uint32_t room = kRing - npend;
-        const uint32_t take = total - pos < room ? total - pos : room;
-        for (uint32_t e = threadIdx.x; e < take; e += kThreads) {
-          const uint32_t it = pos + e;
-          int q = 0;
+      for (int q = 0; q < kMaxThr; q++) base[q] = q < ts.n ? start[ts.boff[q] + tile] : 0;
+      for (uint32_t ch = 0; ch < nch; ch++) {
+        const uint32_t c0 = ctile + ch * kThreads;
+        const uint32_t p = ch * kThreads + threadIdx.x;
+        const bool valid = p < t.n;
+        uint32_t code[kMaxThr];
+        uint64_t packed = 0;
 #pragma unroll
-          for (int z = 1; z < kMaxThr; z++) q += it >= qoff[z] ? 1 : 0;
-          uint32_t qsel = 0;
+        for (int q = 0; q < kMaxThr; q++) code[q] = 0;
+        if (valid) {
+          double v[4];
+          load_plane4(up + tile_corner_off(f, t, p), f.nx, v);
+          const uint32_t hi = plane_codes(ts, v);
+          const uint32_t lo = S.low[ch][threadIdx.x];
+          S.low[ch][threadIdx.x] = uint16_t(hi);
 #pragma unroll
-          for (int z = 1; z < kMaxThr; z++) qsel = q == z ? qoff[z] : qsel;
-          const uint32_t r = it - qsel;
-          uint32_t l = 0;
-#pragma unroll
-          for (uint32_t step = kThreads / 2; step; step >>= 1) l += S.pref[q][l + step] <= r ? step : 0u;
-          const uint32_t sl = r - S.pref[q][l];
-          uint64_t bq = base[0];
-#pragma unroll
-          for (int z = 1; z < kMaxThr; z++) bq = q == z ? base[z] : bq;
-          const uint64_t g = bq + r;
-          const uint32_t cc = c0 + l;
-          if (kStoreCell && g < cap) cellout[g] = cc;
-          const uint32_t cd = S.code[q][l];
-          const uint32_t s = (head + npend + e) & (kRing - 1u);
-          S.rg[s] = uint32_t(g);
-          S.rc[s] = cc;
-          S.re[s] = (uint32_t(c_cslot[cd * 12u + sl]) & 0x3ffffu) | uint32_t(q) << 18 | sl << 20 | cd << 24;
+          for (int q = 0; q < kMaxThr; q++) {
+            const uint32_t cd = ((lo >> (4 * q)) & 15u) | ((hi >> (4 * q)) & 15u) << 4;
+            code[q] = q < ts.n ? cd : 0u;
+            packed |= uint64_t(S.ntri[code[q]]) << (16 * q);
+          }
         }
-        pos += take;
-        npend += take;
+        uint64_t tot;
+        const uint64_t ex = block_excl_scan64(packed, S.wave, tot);
+        uint32_t qoff[kMaxThr + 1];
+        qoff[0] = 0;
+#pragma unroll
+        for (int q = 0; q < kMaxThr; q++) {
+          qoff[q + 1] = qoff[q] + (uint32_t(tot >> (16 * q)) & 0xffffu);
+          S.pref[q][threadIdx.x] = uint16_t(ex >> (16 * q));
+          if (valid) S.code[q][threadIdx.x] = uint8_t(code[q]);
+        }
         __syncthreads();
-        while (npend >= kThreads) {
-          drain(head, kThreads, jb);
-          head = (head + kThreads) & (kRing - 1u);
-          npend -= kThreads;
-        }
-      }
-      // the next chunk's triangles follow this chunk's in every object
+        const uint32_t total = qoff[kMaxThr];
+        for (uint32_t pos = 0; pos < total;) {
+          // queue as many of the chunk's triangles as the ring holds, in
+          // serial order (threshold, cell, slot)
+          const uint32_t room = kRing - npend;
+          const uint32_t take = total - pos < room ? total - pos : room;
+          for (uint32_t e = threadIdx.x; e < take; e += kThreads) {
+            const uint32_t it = pos + e;
+            int q = 0;
 #pragma unroll
-      for (int q = 0; q < kMaxThr; q++) base[q] += qoff[q + 1] - qoff[q];
+            for (int z = 1; z < kMaxThr; z++) q += it >= qoff[z] ? 1 : 0;
+            uint32_t qsel = 0;
+#pragma unroll
+            for (int z = 1; z < kMaxThr; z++) qsel = q == z ? qoff[z] : qsel;
+            const uint32_t r = it - qsel;
+            uint32_t l = 0;
+#pragma unroll
+            for (uint32_t step = kThreads / 2; step; step >>= 1) l += S.pref[q][l + step] <= r ? step : 0u;
+            const uint32_t sl = r - S.pref[q][l];
+            uint64_t bq = base[0];
+#pragma unroll
+            for (int z = 1; z < kMaxThr; z++) bq = q == z ? base[z] : bq;
+            const uint64_t g = bq + r;
+            const uint32_t cc = c0 + l;
+            if (kStoreCell && g < cap) cellout[g] = cc;
+            const uint32_t cd = S.code[q][l];
+            const uint32_t s = (head + npend + e) & (kRing - 1u);
+            S.rg[s] = uint32_t(g);
+            S.rc[s] = cc;
+            S.re[s] = (uint32_t(c_cslot[cd * 12u + sl]) & 0x3ffffu) | uint32_t(q) << 18 | sl << 20 | cd << 24;
+          }
+          pos += take;
+          npend += take;
+          __syncthreads();
+          while (npend >= kThreads) {
+            drain(head, kThreads, jb);
+            head = (head + kThreads) & (kRing - 1u);
+            npend -= kThreads;
+          }
+        }
+        // the next chunk's triangles follow this chunk's in every object
+#pragma unroll
+        for (int q = 0; q < kMaxThr; q++) base[q] += qoff[q + 1] - qoff[q];
+      }
     }
   }
   while (npend) {
@@ -1053,7 +1149,7 @@ __device__ __forceinline__ void ring_tiles(FusedLds& S, const TessJob* __restric
 
 template <typename R, int MODE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void
-k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles, const uint64_t* __restrict__ start,
+k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t nitems, const uint64_t* __restrict__ start,
               uint32_t* __restrict__ cellout, uint64_t cap, const RasterJob<R>* __restrict__ rjobs,
               const uint32_t* __restrict__ jimg, FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
               uint32_t* __restrict__ id, Frag* __restrict__ frags, uint32_t fcap) {
@@ -1068,7 +1164,7 @@ k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles, cons
   __syncthreads();
   // the first pass of a frame (Append in FP64, the packed-key z-test in
   // FP32) stores the cell indices
-  ring_tiles<MODE == kRasterAppend || MODE == kRasterFp32>(S, jobs, njobs, ntiles, start, cellout, cap,
+  ring_tiles<MODE == kRasterAppend || MODE == kRasterFp32>(S, jobs, njobs, nitems, start, cellout, cap,
                                     [&](uint32_t head, uint32_t n, uint32_t jb) {
                                       drain_ring<R, MODE>(S, s_frag, head, n, jobs, jb, rjobs, jimg, hdr, key, id,
                                                           frags, fcap);
@@ -1106,48 +1202,34 @@ __device__ __forceinline__ void emit_ring(FusedLds& S, uint32_t head, uint32_t n
     if (g >= cap) {
       hdr->overflow = 1u;  // capacity reserved ahead of a replayed frame was too small
     } else {
-      // the FP64 mesh (104 B per triangle) streams past the caches:
-      // non-temporal stores measured faster there, slower in FP32
+      // the mesh streams past the caches (non-temporal stores), so that the
+      // field planes the workgroups march through stay in L2
       R* p = m.pos + g;
       R* wp = m.wfn + g;
-      if constexpr (sizeof(R) == 8) {
 #pragma unroll
-        for (int vv = 0; vv < 3; vv++) {
-          __builtin_nontemporal_store(P[vv].x, p + (3 * vv + 0) * cap);
-          __builtin_nontemporal_store(P[vv].y, p + (3 * vv + 1) * cap);
-          __builtin_nontemporal_store(P[vv].z, p + (3 * vv + 2) * cap);
-        }
-        __builtin_nontemporal_store(w.x, wp);
-        __builtin_nontemporal_store(w.y, wp + cap);
-        __builtin_nontemporal_store(w.z, wp + 2 * cap);
-        __builtin_nontemporal_store(ed, m.edges + g);
-        __builtin_nontemporal_store(cc, m.cell + g);
-      } else {
-#pragma unroll
-        for (int vv = 0; vv < 3; vv++) {
-          p[(3 * vv + 0) * cap] = P[vv].x;
-          p[(3 * vv + 1) * cap] = P[vv].y;
-          p[(3 * vv + 2) * cap] = P[vv].z;
-        }
-        wp[0] = w.x;
-        wp[cap] = w.y;
-        wp[2 * cap] = w.z;
-        m.edges[g] = ed;
-        m.cell[g] = cc;
+      for (int vv = 0; vv < 3; vv++) {
+        __builtin_nontemporal_store(P[vv].x, p + (3 * vv + 0) * cap);
+        __builtin_nontemporal_store(P[vv].y, p + (3 * vv + 1) * cap);
+        __builtin_nontemporal_store(P[vv].z, p + (3 * vv + 2) * cap);
       }
+      __builtin_nontemporal_store(w.x, wp);
+      __builtin_nontemporal_store(w.y, wp + cap);
+      __builtin_nontemporal_store(w.z, wp + 2 * cap);
+      __builtin_nontemporal_store(ed, m.edges + g);
+      __builtin_nontemporal_store(cc, m.cell + g);
     }
   }
   __syncthreads();
 }
 
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_emit(const TessJob* __restrict__ jobs, int njobs, uint32_t ntiles,
+__global__ __launch_bounds__(kThreads) void k_emit(const TessJob* __restrict__ jobs, int njobs, uint32_t nitems,
                                                    const uint64_t* __restrict__ start, MeshView<R> m,
                                                    FrameHdr* __restrict__ hdr) {
   __shared__ FusedLds S;
   S.ntri[threadIdx.x] = c_ntri[threadIdx.x];
   __syncthreads();
-  ring_tiles<false>(S, jobs, njobs, ntiles, start, nullptr, 0,
+  ring_tiles<false>(S, jobs, njobs, nitems, start, nullptr, 0,
                     [&](uint32_t head, uint32_t n, uint32_t jb) { emit_ring<R>(S, head, n, jobs, jb, m, hdr); });
 }
 
@@ -1469,10 +1551,10 @@ static uint32_t grid8(uint64_t want, uint32_t cap) {
   return uint32_t((g + 7) / 8 * 8);
 }
 
-hipError_t launch_count(const TessJob* jobs, int njobs, uint32_t ntiles, uint32_t* cnt, uint32_t grid,
+hipError_t launch_count(const TessJob* jobs, int njobs, uint32_t nitems, uint32_t* cnt, uint32_t grid,
                         hipStream_t st) {
-  if (ntiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_count, dim3(grid8(ntiles, grid)), dim3(kThreads), 0, st, jobs, njobs, ntiles, cnt);
+  if (nitems == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count, dim3(grid8(nitems, grid)), dim3(kThreads), 0, st, jobs, njobs, nitems, cnt);
   return hipGetLastError();
 }
 
@@ -1494,15 +1576,15 @@ hipError_t launch_plan(const uint64_t* start, uint64_t ntiles, ObjMeta* meta, in
 }
 
 template <typename R>
-hipError_t launch_emit(const TessJob* jobs, int njobs, uint32_t ntiles, const uint64_t* start, const MeshView<R>& m,
+hipError_t launch_emit(const TessJob* jobs, int njobs, uint32_t nitems, const uint64_t* start, const MeshView<R>& m,
                        FrameHdr* hdr, uint32_t grid, hipStream_t st) {
-  if (ntiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_emit<R>, dim3(grid8(ntiles, grid)), dim3(kThreads), 0, st, jobs, njobs, ntiles, start, m, hdr);
+  if (nitems == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_emit<R>, dim3(grid8(nitems, grid)), dim3(kThreads), 0, st, jobs, njobs, nitems, start, m, hdr);
   return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, uint64_t count, double* out,
+hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, uint64_t count, R* out,
                                  hipStream_t st) {
   if (count == 0) return hipSuccess;
   hipLaunchKernelGGL(k_vertex_normals<R>, dim3(uint32_t((count + kThreads - 1) / kThreads)), dim3(kThreads), 0, st,
@@ -1576,24 +1658,24 @@ hipError_t launch_mesh_compare(const MeshView<double>& a, const ObjMeta* ma, con
 }
 
 template <typename R>
-hipError_t launch_tess_raster(const TessJob* jobs, int njobs, uint32_t ntiles, const uint64_t* start, uint32_t* cell,
+hipError_t launch_tess_raster(const TessJob* jobs, int njobs, uint32_t nitems, const uint64_t* start, uint32_t* cell,
                               uint64_t cap, const RasterJob<R>* rjobs, const uint32_t* jimg, FrameHdr* hdr, int mode,
                               uint64_t* key, uint32_t* id, Frag* frags, uint32_t fcap, uint32_t grid, hipStream_t st) {
-  if (ntiles == 0) return hipSuccess;
-  const dim3 g(grid8(ntiles, grid));
+  if (nitems == 0) return hipSuccess;
+  const dim3 g(grid8(nitems, grid));
   if (mode == kRasterAppend) {
-    hipLaunchKernelGGL((k_tess_raster<R, kRasterAppend>), g, dim3(kThreads), 0, st, jobs, njobs, ntiles, start, cell,
+    hipLaunchKernelGGL((k_tess_raster<R, kRasterAppend>), g, dim3(kThreads), 0, st, jobs, njobs, nitems, start, cell,
                        cap, rjobs, jimg, hdr, key, id, frags, fcap);
   } else if constexpr (sizeof(R) == 4) {
     if (mode != kRasterFp32) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_tess_raster<R, kRasterFp32>), g, dim3(kThreads), 0, st, jobs, njobs, ntiles, start, cell, cap,
+    hipLaunchKernelGGL((k_tess_raster<R, kRasterFp32>), g, dim3(kThreads), 0, st, jobs, njobs, nitems, start, cell, cap,
                        rjobs, jimg, hdr, key, id, frags, fcap);
   } else {
     if (mode == kRasterKey64)
-      hipLaunchKernelGGL((k_tess_raster<R, kRasterKey64>), g, dim3(kThreads), 0, st, jobs, njobs, ntiles, start, cell,
+      hipLaunchKernelGGL((k_tess_raster<R, kRasterKey64>), g, dim3(kThreads), 0, st, jobs, njobs, nitems, start, cell,
                          cap, rjobs, jimg, hdr, key, id, frags, fcap);
     else if (mode == kRasterId64)
-      hipLaunchKernelGGL((k_tess_raster<R, kRasterId64>), g, dim3(kThreads), 0, st, jobs, njobs, ntiles, start, cell,
+      hipLaunchKernelGGL((k_tess_raster<R, kRasterId64>), g, dim3(kThreads), 0, st, jobs, njobs, nitems, start, cell,
                          cap, rjobs, jimg, hdr, key, id, frags, fcap);
     else
       return hipErrorInvalidValue;
@@ -1632,7 +1714,7 @@ template hipError_t launch_emit<float>(const TessJob*, int, uint32_t, const uint
                                        FrameHdr*, uint32_t, hipStream_t);
 template hipError_t launch_vertex_normals<double>(const MeshView<double>&, const ObjMeta*, uint64_t, double*,
                                                   hipStream_t);
-template hipError_t launch_vertex_normals<float>(const MeshView<float>&, const ObjMeta*, uint64_t, double*,
+template hipError_t launch_vertex_normals<float>(const MeshView<float>&, const ObjMeta*, uint64_t, float*,
                                                  hipStream_t);
 template hipError_t launch_raster<double>(const MeshView<double>&, const RasterJob<double>*, int, FrameHdr*, int,
                                           uint64_t*, uint32_t*, Frag*, uint32_t, uint32_t, hipStream_t);

@@ -192,6 +192,16 @@ extern int contour3d_amd_mesh_count(const contour3d_amd_mesh_t * mesh, size_t ob
 extern int contour3d_amd_mesh_copy(const contour3d_amd_mesh_t * mesh, size_t obj, double * pos,
                                    double * wfn, double * vn, uint32_t * cell, uint32_t * edges);
 
+/* The smoothing stage on its own (normal.c:46-165): the unit vertex normals
+ * of every triangle of object `obj`, 9 values per triangle in the mesh's
+ * precision (doubles for an FP64 mesh, floats for an FP32 one; vertex v,
+ * axis d at [9t + 3v + d]), into `vn`, host or device.  Each is the sum of
+ * area * unit face normal over the triangles of the cells sharing the
+ * vertex's lattice edge that have a vertex on it, in serial order, from +0,
+ * normalised; rows of triangles outside the rendered interior are NaN.
+ * Timed as CONTOUR3D_AMD_STAGE_SMOOTH when profiling is on. */
+extern int contour3d_amd_mesh_normals(const contour3d_amd_mesh_t * mesh, size_t obj, void * vn);
+
 /* Render a built mesh (all its objects, in order) and then the lines, as
  * contour3d_amd_execute_rgb does. */
 extern int contour3d_amd_mesh_render(
@@ -242,10 +252,14 @@ enum {
                                            frame layout (image sizes, objects' fields and lattices,
                                            buffer capacities) repeats; thresholds, colours, cameras and
                                            lines may change between replays.  Default 0. */
-  CONTOUR3D_AMD_TUNE_FAIL_ALLOC = 4     /* failure injection for testing the error contract: n > 0 makes
+  CONTOUR3D_AMD_TUNE_FAIL_ALLOC = 4,    /* failure injection for testing the error contract: n > 0 makes
                                            the n-th device or pinned allocation from now fail as an
                                            out-of-memory would (once); the call that needed it returns 1
                                            with the error line, and the library stays usable.  0 disarms. */
+  CONTOUR3D_AMD_TUNE_TESS_ROWS = 5,     /* tessellation brick: rows of cells per tile, at most 2048 cells
+                                           (0: as many rows as fit) */
+  CONTOUR3D_AMD_TUNE_TESS_ZPLANES = 6   /* tessellation brick: z-planes one workgroup marches through per
+                                           work item (0: the default, 8) */
 };
 
 extern int contour3d_amd_tune(int param, int64_t value);
@@ -260,10 +274,12 @@ enum {
   CONTOUR3D_AMD_STAGE_COUNT = 0,   /* classify + per-tile triangle counts */
   CONTOUR3D_AMD_STAGE_SCAN = 1,    /* tile scan + totals */
   CONTOUR3D_AMD_STAGE_EMIT = 2,    /* triangle emission */
-  CONTOUR3D_AMD_STAGE_SMOOTH = 3,  /* vertex normals of every triangle (not used by frames) */
+  CONTOUR3D_AMD_STAGE_SMOOTH = 3,  /* vertex normals of every triangle (contour3d_amd_mesh_normals; not used
+                                      by frames) */
   CONTOUR3D_AMD_STAGE_RASTER = 4,  /* z-buffer passes incl. lines */
   CONTOUR3D_AMD_STAGE_RESOLVE = 5, /* vertex normals of the nearest triangles, colour, y-flip pack */
-  CONTOUR3D_AMD_NSTAGES = 6
+  CONTOUR3D_AMD_STAGE_FUSED = 6,   /* within RASTER of a frame: the fused tessellation + raster kernel alone */
+  CONTOUR3D_AMD_NSTAGES = 7
 };
 
 extern int contour3d_amd_profile_enable(int enable);

@@ -250,3 +250,38 @@ def test_c4_1024_property_run():
         api.finalize()
         del f
         torch.cuda.empty_cache()
+
+
+def test_mesh_normals_stage(orc):
+    """contour3d_amd_mesh_normals, the smoothing stage on its own: FP64
+    normals of interior triangles are bit-exact with the oracle's
+    (normal.c:46-165), other rows NaN; a device buffer receives the same
+    values as a host one; FP32 normals are the FP32 mesh's and within a
+    small angle of the oracle's."""
+    banner("smoothing stage: mesh_normals")
+    import torch
+
+    f = fields.turbulence((30, 26, 28), kmax=7, seed=17)
+    cont = [dict(field=f, threshold=t, color=(1, 2, 3)) for t in (-0.3, 0.4)]
+    m64 = api.Mesh(cont, precision=api.FP64)
+    m32 = api.Mesh(cont, precision=api.FP32)
+    try:
+        for q, t in enumerate((-0.3, 0.4)):
+            ref = orc.mesh(f, t)
+            ins = ref["interior"]
+            vn = m64.normals(q)
+            assert vn.dtype == np.float64 and vn.shape == ref["vn"].shape
+            assert same_bits(vn[ins], ref["vn"][ins])
+            assert np.isnan(vn[~ins]).all()
+            dev = torch.zeros((len(vn), 3, 3), dtype=torch.float64, device="cuda")
+            m64.normals(q, dev)
+            torch.cuda.synchronize()
+            assert same_bits(dev.cpu().numpy(), vn)
+            vn32 = m32.normals(q)
+            assert vn32.dtype == np.float32
+            assert same_bits(vn32.astype(np.float64), m32.copy(q)["vn"])
+            ang = vertex_angles(vn32[ins].astype(np.float64), ref["vn"][ins])
+            assert ang.mean() < 0.05 and ang.max() < 5.0
+    finally:
+        m64.free()
+        m32.free()
