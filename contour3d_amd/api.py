@@ -29,6 +29,7 @@ EXPORTS = (
     "contour3d_amd_set_precision",
     "contour3d_amd_identity",
     "contour3d_amd_execute_rgb",
+    "contour3d_amd_execute_rgb_noalloc",
     "contour3d_amd_execute_batch",
     "contour3d_amd_server_start",
     "contour3d_amd_server_submit",
@@ -75,6 +76,8 @@ def lib():
         L.contour3d_amd_execute_rgb.argtypes = [V, V, C.POINTER(C.c_size_t), V, V, C.POINTER(_abi.Color),
                                                 C.c_size_t, C.POINTER(_abi.ContourObj), C.c_size_t,
                                                 C.POINTER(_abi.LineObj), C.c_void_p, C.c_void_p, C.c_int]
+        L.contour3d_amd_execute_rgb_noalloc.restype = C.c_int
+        L.contour3d_amd_execute_rgb_noalloc.argtypes = L.contour3d_amd_execute_rgb.argtypes
         L.contour3d_amd_execute_batch.restype = C.c_int
         L.contour3d_amd_execute_batch.argtypes = [C.c_size_t, C.POINTER(_abi.Image), C.c_int]
         L.contour3d_amd_tune.argtypes = [C.c_int, C.c_int64]

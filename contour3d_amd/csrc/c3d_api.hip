@@ -25,6 +25,14 @@
 #include <unordered_map>
 #include <vector>
 
+// Device memory of contour3d_amd_execute_rgb_noalloc: one static array,
+// which every GPU the library runs on holds until the process ends.  Host
+// code never passes its name to a kernel: set-up (contour3d_amd_init)
+// takes its address on the current device with hipGetSymbolAddress, and
+// the frame's buffers are regions of that address.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace c3d {
 namespace {
 
@@ -62,16 +70,19 @@ bool injected_alloc_failure() {
   return false;
 }
 
-// grow-only device buffer
+// grow-only device buffer, or (fixed) a region of the static scratch array
+// that never grows
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool fixed = false;
   ~DevBuf() = default;
   // grows to at least `want` bytes, with a quarter of headroom unless
   // `exact` (a growing workload reallocates rarely; each reallocation frees
   // first, which waits for the device)
   hipError_t ensure(size_t want, bool exact = false) {
     if (want <= bytes) return hipSuccess;
+    if (fixed) return hipErrorOutOfMemory;
     if (p) {
       hipError_t e = hipFree(p);
       p = nullptr;
@@ -89,6 +100,7 @@ struct DevBuf {
     return hipSuccess;
   }
   void release() {
+    if (fixed) return;  // the scratch array lives as long as the process
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
@@ -99,12 +111,14 @@ struct DevBuf {
   }
 };
 
-// pinned host staging buffer
+// pinned host staging buffer; a fixed one is sized once, at set-up
 struct HostBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool fixed = false;
   hipError_t ensure(size_t want) {
     if (want <= bytes) return hipSuccess;
+    if (fixed) return hipErrorOutOfMemory;
     if (p) (void)hipHostFree(p);
     p = nullptr;
     bytes = 0;
@@ -211,6 +225,11 @@ struct TessLayout {
 struct Frame {
   hipStream_t st = nullptr;
   hipEvent_t done = nullptr;
+  // the allocation-free frame: every buffer a fixed region of the scratch
+  // array; device fields and identity lattices only (their grids go to
+  // `grids`)
+  bool noalloc = false;
+  DevBuf grids;
   std::vector<std::unique_ptr<DevBuf>> uploads;
   DevBuf cnt, tiles, tjobs, rjobs, hdr, jimg;
   DevBuf key, id, rgb, depth, fpix, fdep, frgb, frags, vntmp;
@@ -252,13 +271,16 @@ struct Frame {
     if (st) (void)hipStreamSynchronize(st);
     for (auto& u : uploads) u->release();
     uploads.clear();
-    DevBuf* bufs[] = {&cnt,  &tiles, &tjobs, &rjobs, &hdr,  &jimg, &key,  &id,
-                      &rgb,  &depth, &fpix,  &fdep,  &frgb, &frags, &vntmp};
+    DevBuf* bufs[] = {&cnt,  &tiles, &tjobs, &rjobs, &hdr,  &jimg, &key,  &id,   &rgb,
+                      &depth, &fpix, &fdep,  &frgb, &frags, &vntmp, &grids};
     for (auto* b : bufs) b->release();
     mesh.release();
-    frag_cap = 0;
+    if (!noalloc) frag_cap = 0;
     hrgb.release();
-    hstage.release();
+    if (hstage.p) (void)hipHostFree(hstage.p);
+    hstage.p = nullptr;
+    hstage.bytes = 0;
+    hstage.fixed = false;
     for (auto e : prof.pool) (void)hipEventDestroy(e);
     prof.pool.clear();
     prof.pending.clear();
@@ -307,12 +329,80 @@ struct Lib {
     return g < 8 ? 8 : g;
   }
   Frame main;
+  Frame scratch_frame;  // the allocation-free entry point's frame
   Server* server = nullptr;
 };
 
 Lib& lib() {
   static Lib L;
   return L;
+}
+
+// Limits of the allocation-free frame (the regions of the scratch array).
+constexpr uint64_t kScratchTiles = uint64_t(1) << 20;      // tile-count entries of all objects
+constexpr uint32_t kScratchObjs = 64;                      // contour objects
+constexpr uint64_t kScratchPixels = uint64_t(1) << 21;     // z-buffer entries (1920x1080 fits)
+constexpr uint32_t kScratchFrags = 1u << 20;               // fragment list entries
+constexpr uint64_t kScratchLineFrags = uint64_t(1) << 18;  // line fragments
+constexpr uint64_t kScratchTriangles = uint64_t(1) << 24;  // cell store entries
+constexpr uint64_t kScratchGridValues = uint64_t(1) << 16; // lattice coordinates of all objects
+
+// per device: the scratch array's address there (taken at set-up)
+constexpr int kMaxDevices = 64;
+void* g_scratch[kMaxDevices] = {};
+
+// Takes the scratch array's address on the current device and cuts it into
+// the buffers of the allocation-free frame F.
+int carve_scratch(Frame& F, int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    log_error("device %d: no scratch array slot", device);
+    return 1;
+  }
+  void* base = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&base, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !base) {
+    (void)hipGetLastError();
+    g_scratch[device] = nullptr;  // the allocation-free entry point reports it; nothing else needs the array
+    return 1;
+  }
+  g_scratch[device] = base;
+  size_t off = 0;
+  auto region = [&](DevBuf& b, size_t bytes) {
+    b.p = static_cast<unsigned char*>(base) + off;
+    b.bytes = bytes;
+    b.fixed = true;
+    off += (bytes + 255) / 256 * 256;
+  };
+  contour3d_amd_mesh& M = F.mesh;
+  region(F.cnt, (kScratchTiles + 1) * sizeof(uint32_t));
+  region(M.bstart, (kScratchTiles + 1) * sizeof(uint64_t));
+  region(F.tiles, ((kScratchTiles + kScanTile - 1) / kScanTile + 1) * sizeof(uint64_t));
+  region(F.tjobs, (kScratchObjs + 1) * sizeof(TessJob));
+  region(M.meta, (kScratchObjs + 1) * sizeof(ObjMeta));
+  region(F.hdr, sizeof(FrameHdr));
+  region(F.jimg, (2 * kScratchObjs + 2) * sizeof(uint32_t));
+  region(F.rjobs, sizeof(RasterJob<double>));
+  region(F.key, kScratchPixels * sizeof(uint64_t));
+  region(F.id, kScratchPixels * sizeof(uint32_t));
+  region(F.fpix, (kScratchLineFrags + 1) * sizeof(uint32_t));
+  region(F.fdep, (kScratchLineFrags + 1) * sizeof(double));
+  region(F.frgb, 3 * (kScratchLineFrags + 1));
+  region(F.frags, size_t(kScratchFrags) * sizeof(Frag));
+  region(M.cell, kScratchTriangles * sizeof(uint32_t));
+  region(F.grids, kScratchGridValues * sizeof(double));
+  region(F.rgb, 0);
+  region(F.depth, 0);
+  region(M.pos, 0);
+  region(M.wfn, 0);
+  region(M.edges, 0);
+  if (off > kScratchBytes) {
+    log_error("scratch layout of %zu bytes exceeds the %zu-byte array", off, kScratchBytes);
+    return 1;
+  }
+  M.cap = kScratchTriangles;
+  F.frag_cap = kScratchFrags;
+  F.noalloc = true;
+  return 0;
 }
 
 int ensure_init(Lib& L, int device) {
@@ -327,6 +417,13 @@ int ensure_init(Lib& L, int device) {
   C3D_HIP(L.main.create());
   L.nbr = build_nbr_table();
   C3D_HIP(upload_tables(L.nbr));
+  // set-up of the allocation-free entry point: its stream, pinned header
+  // staging and the scratch array's regions on this device
+  Frame& S = L.scratch_frame;
+  C3D_HIP(S.create());
+  C3D_HIP(S.hstage.ensure(sizeof(FrameHdr) + (kScratchObjs + 1) * sizeof(ObjMeta)));
+  S.hstage.fixed = true;
+  (void)carve_scratch(S, device);
   L.device = device;
   L.ready = true;
   return 0;
@@ -785,6 +882,11 @@ int ensure_capacity(contour3d_amd_mesh& M) {
 template <typename R>
 int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<const contour3d_contour_obj_t*>& objs) {
   const size_t n = objs.size();
+  if (F.noalloc && n > F.tjobs.bytes / sizeof(TessJob) - 1) {
+    log_error("the allocation-free call takes at most %zu contour objects", F.tjobs.bytes / sizeof(TessJob) - 1);
+    return 1;
+  }
+  size_t grid_used = 0;  // lattice coordinates in the allocation-free frame's grid region
   std::vector<FieldDev> fd(n);
   std::vector<int> group(n, -1);
   std::vector<uint32_t> boff(n + 1, 0);
@@ -806,7 +908,41 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
     f.tg = make_tile_geo(f.ncx, f.ncy, f.ncz, L.tess_rows, L.tess_zc);
     f.ntiles = f.ncz * f.tg.cols;  // at most the cell count
     f.nitems = f.tg.cols * f.tg.nzc;
-    if (is_device_ptr(ob.array)) {
+    if (F.noalloc) {
+      // the allocation-free frame reads device fields in place and the
+      // identity map of grids copied into its scratch region
+      if (!is_device_ptr(ob.array) || ob.converter != contour3d_amd_identity) {
+        log_error("contour object %zu: the allocation-free call takes device fields and the "
+                  "contour3d_amd_identity converter", o);
+        return 1;
+      }
+      f.f = ob.array;
+      const double* reuse = nullptr;
+      for (size_t q = 0; q < o && !reuse; q++)
+        if (std::memcmp(objs[q]->glsizes, ob.glsizes, sizeof(ob.glsizes)) == 0 &&
+            std::memcmp(objs[q]->grids, ob.grids, sizeof(ob.grids)) == 0)
+          reuse = fd[q].gx;
+      if (!reuse) {
+        const size_t need = ob.glsizes[0] + ob.glsizes[1] + ob.glsizes[2];
+        if (grid_used + need > F.grids.bytes / sizeof(double)) {
+          log_error("the allocation-free call holds at most %zu lattice coordinates (grids of all objects)",
+                    F.grids.bytes / sizeof(double));
+          return 1;
+        }
+        double* gd = F.grids.as<double>() + grid_used;
+        size_t go = 0;
+        for (int d = 0; d < 3; d++) {
+          C3D_HIP(hipMemcpyAsync(gd + go, ob.grids[d], ob.glsizes[d] * sizeof(double), hipMemcpyDefault, F.st));
+          go += ob.glsizes[d];
+        }
+        grid_used += need;
+        reuse = gd;
+      }
+      f.gx = reuse;
+      f.gy = f.gx + f.nx;
+      f.gz = f.gy + f.ny;
+      f.cx = f.cy = f.cz = nullptr;
+    } else if (is_device_ptr(ob.array)) {
       f.f = ob.array;
     } else {
       const size_t bytes = size_t(f.nx) * f.ny * f.nz * sizeof(double);
@@ -824,20 +960,22 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
       }
       f.f = static_cast<const double*>(dst);
     }
-    Geom* g = nullptr;
-    if (get_geometry(L, ob, &g)) return 1;
-    g->busy++;
-    F.geoms_used.push_back(g);
-    f.gx = g->grids.as<double>();
-    f.gy = f.gx + f.nx;
-    f.gz = f.gy + f.ny;
-    if (g->identity) {
-      f.cx = f.cy = f.cz = nullptr;
-    } else {
-      const size_t npts = size_t(f.nx) * f.ny * f.nz;
-      f.cx = g->table.as<double>();
-      f.cy = f.cx + npts;
-      f.cz = f.cy + npts;
+    if (!F.noalloc) {
+      Geom* g = nullptr;
+      if (get_geometry(L, ob, &g)) return 1;
+      g->busy++;
+      F.geoms_used.push_back(g);
+      f.gx = g->grids.as<double>();
+      f.gy = f.gx + f.nx;
+      f.gz = f.gy + f.ny;
+      if (g->identity) {
+        f.cx = f.cy = f.cz = nullptr;
+      } else {
+        const size_t npts = size_t(f.nx) * f.ny * f.nz;
+        f.cx = g->table.as<double>();
+        f.cy = f.cx + npts;
+        f.cz = f.cy + npts;
+      }
     }
     if (uint64_t(boff[o]) + f.ntiles >= (uint64_t(1) << 31)) {
       log_error("frame exceeds 2^31 tessellation tiles");
@@ -852,6 +990,11 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
     if (group[o] < 0) group[o] = int(o);
   }
   const uint64_t nbt = boff[n];
+  if (F.noalloc && nbt + 1 > F.cnt.bytes / sizeof(uint32_t)) {
+    log_error("the allocation-free call holds at most %zu tessellation tiles; the objects need %llu",
+              F.cnt.bytes / sizeof(uint32_t) - 1, (unsigned long long)nbt);
+    return 1;
+  }
   M.precision = sizeof(R) == 8 ? CONTOUR3D_AMD_FP64 : CONTOUR3D_AMD_FP32;
   M.base.assign(n, 0);
   M.count.assign(n, 0);
@@ -1035,6 +1178,11 @@ int raster_prepare(Frame& F, uint64_t npx) {
   const bool fp64 = sizeof(R) == 8;
   const Frags& LF = F.h_frags;
   const uint32_t nf = uint32_t(LF.pix.size());
+  if (F.noalloc && (nf + 1 > F.fpix.bytes / sizeof(uint32_t) || npx > F.key.bytes / sizeof(uint64_t))) {
+    log_error("the allocation-free call holds at most %zu line fragments and %zu pixels (%u and %llu asked)",
+              F.fpix.bytes / sizeof(uint32_t) - 1, F.key.bytes / sizeof(uint64_t), nf, (unsigned long long)npx);
+    return 1;
+  }
   C3D_HIP(F.key.ensure(npx * sizeof(uint64_t)));
   if (fp64) C3D_HIP(F.id.ensure(npx * sizeof(uint32_t)));
   C3D_HIP(F.fpix.ensure((nf + 1) * sizeof(uint32_t)));
@@ -1152,7 +1300,7 @@ int frame_finish(Frame& F) {
     F.prof.ntri += fin.ntri;
   }
   F.unread = false;
-  if ((fin.overflow & 2u) && fin.nfrag > F.frag_cap) {
+  if ((fin.overflow & 2u) && fin.nfrag > F.frag_cap && !F.noalloc) {
     const uint64_t grow = uint64_t(fin.nfrag) + fin.nfrag / 4;
     F.frag_cap = uint32_t(grow > 0xffffffffull ? 0xffffffffull : grow);
   }
@@ -1206,7 +1354,9 @@ int frame_setup(Frame& F, std::vector<Image>& imgs, uint64_t& npx) {
     log_error("%llu pixels in one frame exceed 2^31", (unsigned long long)npx);
     return 1;
   }
-  C3D_HIP(F.rgb.ensure(3 * npx + 1));
+  bool any_host_rgb = false;
+  for (const Image& im : imgs) any_host_rgb = any_host_rgb || !im.rgb_dev;
+  if (any_host_rgb) C3D_HIP(F.rgb.ensure(3 * npx + 1));
 
   bool any_dep = false;
   for (const Image& im : imgs) any_dep = any_dep || (im.depth && !im.dep_dev);
@@ -1464,6 +1614,34 @@ int run_frame(Lib& L, Frame& F, std::vector<Image>& imgs, int precision, bool wa
     }
     if (!(replayed && (F.last.overflow & 1u))) return 0;
     F.drop_graph();  // the cell store must grow: render again, with the read back
+  }
+  return 0;
+}
+
+// The allocation-free frame: planned against the scratch array's cell
+// store with no read back of its totals, never re-rendered or grown; a
+// frame beyond a scratch maximum fails with an error.
+int run_frame_noalloc(Lib& L, Frame& F, std::vector<Image>& imgs, int precision) {
+  if (precision != CONTOUR3D_AMD_FP64 && precision != CONTOUR3D_AMD_FP32) {
+    log_error("unknown precision %d", precision);
+    return 1;
+  }
+  bool replayed = false;
+  const int rc = precision == CONTOUR3D_AMD_FP64 ? run_frame_t<double>(L, F, F.mesh, imgs, false, replayed, true)
+                                                 : run_frame_t<float>(L, F, F.mesh, imgs, false, replayed, true);
+  if (rc) {
+    (void)hipStreamSynchronize(F.st);
+    return rc;
+  }
+  if (!replayed) {
+    log_error("allocation-free frame not planned against the scratch array");
+    return 1;
+  }
+  if (frame_finish(F)) return 1;
+  if (F.last.overflow & 1u) {
+    log_error("the allocation-free call holds at most %llu triangles (and 2^32 triangle and line ids); the frame "
+              "has %llu triangles", (unsigned long long)F.mesh.cap, (unsigned long long)F.last.ntri);
+    return 1;
   }
   return 0;
 }
@@ -1985,6 +2163,7 @@ int contour3d_amd_finalize(void) {
   }
   L.geoms.clear();
   L.main.release();
+  L.scratch_frame.release();
   L.ready = false;
   return 0;
 }
@@ -2054,6 +2233,41 @@ int contour3d_amd_execute_rgb(const contour3d_vector_t* camera_position, const c
                                                screen_local, bg_color, num_contours, contour_objs, num_lines,
                                                line_objs, rgb, depth, false)};
   return c3d::run_frame(L, L.main, imgs, precision);
+}
+
+int contour3d_amd_execute_rgb_noalloc(const contour3d_vector_t* camera_position,
+                                      const contour3d_vector_t* light_direction, const size_t screen_sizes[2],
+                                      const contour3d_vector_t* screen_center, const contour3d_vector_t screen_local[2],
+                                      const contour3d_color_t* bg_color, size_t num_contours,
+                                      const contour3d_contour_obj_t* contour_objs, size_t num_lines,
+                                      const contour3d_line_obj_t* line_objs, uint8_t* rgb, double* depth,
+                                      int precision) {
+  auto& L = lib();
+  std::lock_guard<std::mutex> g(L.mu);
+  if (!L.ready || L.device < 0 || L.device >= c3d::kMaxDevices) {
+    c3d::log_error("execute_rgb_noalloc: call contour3d_amd_init first (it sets up the scratch array)");
+    return 1;
+  }
+  if (!c3d::g_scratch[L.device] || !L.scratch_frame.noalloc) {
+    c3d::log_error("execute_rgb_noalloc: the scratch array could not be set up on device %d", L.device);
+    return 1;
+  }
+  if (!c3d::args_ok(camera_position, light_direction, screen_sizes, screen_center, screen_local, bg_color))
+    return 1;
+  if (!rgb || !c3d::is_device_ptr(rgb) || (depth && !c3d::is_device_ptr(depth))) {
+    c3d::log_error("execute_rgb_noalloc: rgb (and depth, if given) must be device pointers");
+    return 1;
+  }
+  const uint64_t npx = uint64_t(screen_sizes[0]) * uint64_t(screen_sizes[1]);
+  if (npx > c3d::kScratchPixels) {
+    c3d::log_error("execute_rgb_noalloc: %zu x %zu pixels exceed the maximum of %llu", screen_sizes[0],
+                   screen_sizes[1], (unsigned long long)c3d::kScratchPixels);
+    return 1;
+  }
+  std::vector<c3d::Image> imgs{c3d::make_image(camera_position, light_direction, screen_sizes, screen_center,
+                                               screen_local, bg_color, num_contours, contour_objs, num_lines,
+                                               line_objs, rgb, depth, false)};
+  return c3d::run_frame_noalloc(L, L.scratch_frame, imgs, precision);
 }
 
 int contour3d_amd_execute_batch(size_t num_images, const contour3d_amd_image_t* images, int precision) {

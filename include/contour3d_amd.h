@@ -71,6 +71,44 @@ extern int contour3d_amd_execute_rgb(
     int precision
 );
 
+/*
+ * contour3d_amd_execute_rgb without allocating: no counterpart in the
+ * reference.  For callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc --
+ * this variant calls no hipMalloc, hipFree or other allocating routine.  Its
+ * intermediate buffers (tile counts and starts, job tables, the cell store,
+ * z-buffer, fragment list, line fragments, lattice grids) are regions of one
+ * static device array of 128 MiB that each GPU the library runs on holds
+ * until the process ends; contour3d_amd_init (or the first call of any
+ * other entry point) sets it up, and this call returns 1 before that.
+ * The caller passes the inputs and outputs:
+ *   fields     device pointers (read in place), with the
+ *              contour3d_amd_identity converter
+ *   rgb        device, W*H*3 bytes in PPM row order
+ *   depth      device W*H doubles, or NULL
+ * Limits (1 with an error line beyond them): 2^21 pixels (1920x1080 fits),
+ * 2^24 triangles, 64 contour objects, 2^20 tessellation tiles (about 2^31
+ * cells), 2^18 line fragments, 65536 lattice coordinates.  Results are
+ * bit-identical to contour3d_amd_execute_rgb.  Every call on a GPU shares
+ * that GPU's array and keeps the last call's contents; calls are serialised
+ * by the library mutex, and calls that overlapped on one GPU would race.
+ */
+extern int contour3d_amd_execute_rgb_noalloc(
+    const contour3d_vector_t * camera_position,
+    const contour3d_vector_t * light_direction,
+    const size_t screen_sizes[2],
+    const contour3d_vector_t * screen_center,
+    const contour3d_vector_t screen_local[2],
+    const contour3d_color_t * bg_color,
+    size_t num_contours,
+    const contour3d_contour_obj_t * contour_objs,
+    size_t num_lines,
+    const contour3d_line_obj_t * line_objs,
+    uint8_t * rgb,
+    double * depth,
+    int precision
+);
+
 /* ---- batches ------------------------------------------------------------ */
 
 /* One image of a batch: the arguments of contour3d_amd_execute_rgb. */
