@@ -185,19 +185,20 @@ struct Prof {
 struct contour3d_amd_mesh {
   int precision = 0;
   int precision_built = -1;  // form the buffers were sized for
-  c3d::DevBuf pos, wfn, cell, edges;
+  c3d::DevBuf rec;     // a stored mesh: cap triangle records (c3d::MeshRec)
+  c3d::DevBuf cell;    // a frame: the cell index of each triangle (cap entries)
   c3d::DevBuf bstart;  // per-tile triangle starts of every object's field
   c3d::DevBuf meta;    // ObjMeta per object
   uint64_t cap = 0;
   uint64_t total = 0;
   uint64_t ntiles = 0;  // entries of bstart minus one
   std::vector<uint64_t> base, count;
+  std::vector<uint32_t> tiles;  // tiles of each object's field
+  std::vector<c3d::TileGeo> geo;  // and their layout
   std::vector<contour3d_color_t> color;
   void release() {
-    pos.release();
-    wfn.release();
+    rec.release();
     cell.release();
-    edges.release();
     bstart.release();
     meta.release();
     cap = 0;
@@ -392,9 +393,7 @@ int carve_scratch(Frame& F, int device) {
   region(F.grids, kScratchGridValues * sizeof(double));
   region(F.rgb, 0);
   region(F.depth, 0);
-  region(M.pos, 0);
-  region(M.wfn, 0);
-  region(M.edges, 0);
+  region(M.rec, 0);
   if (off > kScratchBytes) {
     log_error("scratch layout of %zu bytes exceeds the %zu-byte array", off, kScratchBytes);
     return 1;
@@ -673,10 +672,7 @@ int validate_obj(const contour3d_contour_obj_t& o, size_t n) {
 template <typename R>
 MeshView<R> view_of(const contour3d_amd_mesh& M) {
   MeshView<R> m;
-  m.pos = M.pos.as<R>();
-  m.wfn = M.wfn.as<R>();
-  m.cell = M.cell.as<uint32_t>();
-  m.edges = M.edges.as<uint32_t>();
+  m.rec = M.rec.as<MeshRec<R>>();
   m.cap = M.cap;
   return m;
 }
@@ -861,16 +857,10 @@ int ensure_capacity(contour3d_amd_mesh& M) {
   if (M.cap >= M.total && M.cap != 0 && M.precision_built == M.precision) return 0;
   M.precision_built = M.precision;
   const uint64_t cap = M.total + M.total / 4 + 1024;
-  M.pos.release();
-  M.wfn.release();
-  M.cell.release();
-  M.edges.release();
+  M.rec.release();
   M.cap = 0;
   // the capacity already holds a quarter of headroom
-  C3D_HIP(M.pos.ensure(9 * cap * sizeof(R), true));
-  C3D_HIP(M.wfn.ensure(3 * cap * sizeof(R), true));
-  C3D_HIP(M.cell.ensure(cap * sizeof(uint32_t), true));
-  C3D_HIP(M.edges.ensure(cap * sizeof(uint32_t), true));
+  C3D_HIP(M.rec.ensure(cap * sizeof(MeshRec<R>), true));
   M.cap = cap;
   return 0;
 }
@@ -998,6 +988,12 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
   M.precision = sizeof(R) == 8 ? CONTOUR3D_AMD_FP64 : CONTOUR3D_AMD_FP32;
   M.base.assign(n, 0);
   M.count.assign(n, 0);
+  M.tiles.resize(n);
+  M.geo.resize(n);
+  for (size_t o = 0; o < n; o++) {
+    M.tiles[o] = fd[o].ntiles;
+    M.geo[o] = fd[o].tg;
+  }
   M.color.resize(n);
   for (size_t o = 0; o < n; o++) M.color[o] = objs[o]->color;
   M.ntiles = nbt;
@@ -2086,9 +2082,12 @@ int smooth_object(Frame& F, const contour3d_amd_mesh& M, size_t obj, void* out) 
   if (n == 0) return 0;
   const ObjMeta* md = M.meta.as<ObjMeta>() + obj;
   if (prof_begin(F, CONTOUR3D_AMD_STAGE_SMOOTH)) return 1;
-  hipError_t e = M.precision == CONTOUR3D_AMD_FP64
-                     ? launch_vertex_normals<double>(view_of<double>(M), md, n, static_cast<double*>(out), F.st)
-                     : launch_vertex_normals<float>(view_of<float>(M), md, n, static_cast<float*>(out), F.st);
+  const uint32_t nt = M.tiles[obj];
+  const TileGeo& tg = M.geo[obj];
+  hipError_t e =
+      M.precision == CONTOUR3D_AMD_FP64
+          ? launch_vertex_normals<double>(view_of<double>(M), md, tg, n, nt, static_cast<double*>(out), F.st)
+          : launch_vertex_normals<float>(view_of<float>(M), md, tg, n, nt, static_cast<float*>(out), F.st);
   if (e != hipSuccess) {
     log_error("vertex normals: %s", hipGetErrorString(e));
     return 1;
@@ -2376,10 +2375,10 @@ int contour3d_amd_mesh_copy(const contour3d_amd_mesh_t* mesh, size_t obj, double
   }
   auto& L = lib();
   std::lock_guard<std::mutex> g(L.mu);
-  const uint64_t n = mesh->count[obj], b = mesh->base[obj], cap = mesh->cap;
+  const uint64_t n = mesh->count[obj], b = mesh->base[obj];
   if (n == 0) return 0;
   const size_t rs = mesh->precision == CONTOUR3D_AMD_FP64 ? 8 : 4;
-  std::vector<unsigned char> tmp(n * rs * 4);
+  const size_t RS = mesh->precision == CONTOUR3D_AMD_FP64 ? sizeof(c3d::MeshRec<double>) : sizeof(c3d::MeshRec<float>);
   auto real_at = [&](const unsigned char* q) {
     if (rs == 8) {
       double x;
@@ -2390,21 +2389,25 @@ int contour3d_amd_mesh_copy(const contour3d_amd_mesh_t* mesh, size_t obj, double
     std::memcpy(&y, q, 4);
     return double(y);
   };
-  auto soa = [&](const c3d::DevBuf& buf, int ncomp, double* dst) -> int {
-    for (int comp = 0; comp < ncomp; comp++) {
-      const unsigned char* src = static_cast<const unsigned char*>(buf.p) + (uint64_t(comp) * cap + b) * rs;
-      if (hipMemcpy(tmp.data(), src, n * rs, hipMemcpyDeviceToHost) != hipSuccess) {
-        c3d::log_error("mesh_copy: device copy failed");
-        return 1;
-      }
-      for (uint64_t t = 0; t < n; t++) dst[t * ncomp + comp] = real_at(tmp.data() + t * rs);
+  // the object's records, unpacked field by field
+  std::vector<unsigned char> recs;
+  if (pos || wfn || edges || cell) {
+    recs.resize(n * RS);
+    if (hipMemcpy(recs.data(), static_cast<const unsigned char*>(mesh->rec.p) + b * RS, n * RS,
+                  hipMemcpyDeviceToHost) != hipSuccess) {
+      c3d::log_error("mesh_copy: device copy failed");
+      return 1;
     }
-    return 0;
-  };
-  if (pos && soa(mesh->pos, 9, pos)) return 1;
-  if (wfn && soa(mesh->wfn, 3, wfn)) return 1;
-  if (edges && hipMemcpy(edges, mesh->edges.as<uint32_t>() + b, n * 4, hipMemcpyDeviceToHost) != hipSuccess)
-    return 1;
+  }
+  for (uint64_t t = 0; t < (recs.empty() ? 0 : n); t++) {
+    const unsigned char* r = recs.data() + t * RS;
+    if (pos)
+      for (int c = 0; c < 9; c++) pos[9 * t + c] = real_at(r + c * rs);
+    if (wfn)
+      for (int c = 0; c < 3; c++) wfn[3 * t + c] = real_at(r + (9 + c) * rs);
+    if (cell) std::memcpy(cell + t, r + 12 * rs, 4);
+    if (edges) std::memcpy(edges + t, r + 12 * rs + 4, 4);
+  }
   if (vn) {
     // the mesh keeps no per-triangle vertex normals; they are gathered now
     // with the same sums the resolve pass uses for nearest triangles
@@ -2421,7 +2424,6 @@ int contour3d_amd_mesh_copy(const contour3d_amd_mesh_t* mesh, size_t obj, double
     }
     for (uint64_t q = 0; q < n * 9; q++) vn[q] = real_at(raw.data() + q * rs);
   }
-  if (cell && hipMemcpy(cell, mesh->cell.as<uint32_t>() + b, n * 4, hipMemcpyDeviceToHost) != hipSuccess) return 1;
   return 0;
 }
 

@@ -57,15 +57,22 @@ struct TessJob {
   uint32_t xoff;
 };
 
-// Triangle storage, structure of arrays with capacity cap: component c of
-// triangle t lives at [c * cap + t].  R is double (FP64 form) or float
-// (FP32 form).
+// One stored triangle: 56 bytes in the FP32 form (R = float), 104 in the
+// FP64 form.  Records are stored contiguously in serial order, so a
+// tessellation drain writes whole runs of lines.
+template <typename R>
+struct alignas(8) MeshRec {
+  R p[9];          // vertex v, axis d at 3v + d
+  R w[3];          // area * unit face normal, the smoothing weight
+  uint32_t cell;   // linear cell index within its field
+  uint32_t edges;  // three 5-bit cell-local lattice edge ids | interior << 15
+};
+static_assert(sizeof(MeshRec<float>) == 56 && sizeof(MeshRec<double>) == 104, "record layout");
+
+// Triangle storage: cap records.
 template <typename R>
 struct MeshView {
-  R* pos;          // [9][cap]: vertex v, axis d at (3v + d)
-  R* wfn;          // [3][cap]: area * unit face normal, the smoothing weight
-  uint32_t* cell;  // linear cell index within its field
-  uint32_t* edges; // three 5-bit cell-local lattice edge ids | interior << 15
+  MeshRec<R>* rec;
   uint64_t cap;
 };
 
@@ -149,8 +156,8 @@ hipError_t launch_emit(const TessJob* jobs, int njobs, uint32_t nitems, const ui
 // mesh's precision per triangle (NaN rows for triangles outside the
 // rendered interior)
 template <typename R>
-hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, uint64_t count, R* out,
-                                 hipStream_t st);
+hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, const TileGeo& tg, uint64_t count,
+                                 uint32_t ntiles, R* out, hipStream_t st);
 // FP64 mesh a against mesh b (FP64, or FP32 when b_fp32; b points to its
 // MeshView) for one object: out[0..4] as k_mesh_compare documents
 hipError_t launch_mesh_compare(const MeshView<double>& a, const ObjMeta* ma, const void* b, int b_fp32,

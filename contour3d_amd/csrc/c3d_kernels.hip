@@ -510,11 +510,12 @@ __constant__ NbrTable c_nbr;
 
 
 
-__device__ __forceinline__ uint64_t lower_bound_cell(const uint32_t* __restrict__ cell, uint64_t lo, uint64_t hi,
+template <typename R>
+__device__ __forceinline__ uint64_t lower_bound_cell(const MeshRec<R>* __restrict__ rec, uint64_t lo, uint64_t hi,
                                                      uint32_t key) {
   while (lo < hi) {
     const uint64_t mid = (lo + hi) >> 1;
-    if (cell[mid] < key) lo = mid + 1;
+    if (rec[mid].cell < key) lo = mid + 1;
     else hi = mid;
   }
   return lo;
@@ -529,7 +530,6 @@ __device__ __forceinline__ uint64_t lower_bound_cell(const uint32_t* __restrict_
 template <typename R>
 __device__ V3<R> vertex_normal(const MeshView<R>& m, const ObjMeta& om, uint32_t c, uint32_t e) {
   const int64_t sy = om.ncx, sz = int64_t(om.ncx) * om.ncy;
-  const uint64_t cap = m.cap;
   V3<R> sum = {R(0), R(0), R(0)};
   const int nn = c_nbr.n[e];
   for (int q = 0; q < nn; q++) {
@@ -538,31 +538,141 @@ __device__ V3<R> vertex_normal(const MeshView<R>& m, const ObjMeta& om, uint32_t
     const uint32_t c2 = uint32_t(int64_t(c) + dk * sz + dj * sy + di);
     const uint32_t tl = tile_of_cell(om, c2);
     const uint64_t hi = om.bstart[tl + 1];
-    for (uint64_t u = lower_bound_cell(m.cell, om.bstart[tl], hi, c2); u < hi && m.cell[u] == c2; u++) {
-      const uint32_t eu = m.edges[u];
+    for (uint64_t u = lower_bound_cell(m.rec, om.bstart[tl], hi, c2); u < hi && m.rec[u].cell == c2; u++) {
+      const uint32_t eu = m.rec[u].edges;
       if ((eu & 31u) == want || ((eu >> 5) & 31u) == want || ((eu >> 10) & 31u) == want)
-        sum = vadd(sum, V3<R>{m.wfn[u], m.wfn[cap + u], m.wfn[2 * cap + u]});
+        sum = vadd(sum, V3<R>{m.rec[u].w[0], m.rec[u].w[1], m.rec[u].w[2]});
     }
   }
   return vunit(sum);
 }
 
+// Vertex normals of every triangle of one object, a workgroup per tile.
+// The cells a tile's vertex normals gather from lie in the tile's window:
+// the planes k-1..k+1, the rows j0-1..j1+1 and the columns i0-1..i0+w of the
+// tile's cells.  The workgroup first records, for every non-empty window
+// cell, where its triangles start (an offset within the tile holding it: a
+// tile holds at most 12 * kTileCells triangles, so 16 bits) and how many
+// there are, then each thread sums the smoothing weights of its triangles'
+// vertices exactly as vertex_normal does -- cells in the neighbour table's
+// increasing order, each cell's triangles in slot order, from +0 -- with
+// one LDS read per neighbour cell and independent reads of that cell's
+// records.  Rows of triangles outside the rendered interior are NaN.
+// Grid: one workgroup per tile of the object; dynamic LDS of
+// smooth_lds_bytes(window cells).
+constexpr uint16_t kWinEmpty = 0xffffu;
+
+__host__ __device__ inline uint32_t smooth_window(const TileGeo& g) {
+  return 3u * (g.rows + 2u) * (g.seg_w + 2u);
+}
+
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_vertex_normals(MeshView<R> m, const ObjMeta* __restrict__ meta,
-                                                             uint64_t count, R* __restrict__ out) {
-  const uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
-  if (t >= count) return;
+__global__ __launch_bounds__(kThreads) void k_smooth_tiles(MeshView<R> m, const ObjMeta* __restrict__ meta,
+                                                           R* __restrict__ out) {
+  extern __shared__ uint16_t s_dyn[];
+  __shared__ uint64_t s_b0[27], s_b1[27];  // triangle range of each window tile (dz, djb, dis); empty outside
   const ObjMeta om = *meta;
-  const uint64_t g = om.base + t;
-  const uint32_t ed = m.edges[g];
-  const bool in = (ed >> 15) & 1u;
+  const TileGeo& g = om.tg;
+  const uint32_t nwin_max = smooth_window(g);
+  uint16_t* s_start = s_dyn;
+  uint8_t* s_cnt = reinterpret_cast<uint8_t*>(s_dyn + nwin_max);
+  const uint32_t tile = blockIdx.x;
+  const uint64_t t0 = om.bstart[tile], t1 = om.bstart[tile + 1];
+  if (t0 == t1) return;
+  const uint32_t k = fdiv(tile, g.div_cols);
+  const uint32_t col = tile - k * g.cols;
+  const uint32_t jb = g.segs == 1 ? col : col / g.segs;
+  const uint32_t is = col - jb * g.segs;
+  const uint32_t j0 = jb * g.rows, i0 = is * g.seg_w;
+  const uint32_t rows = min(g.rows, om.ncy - j0);
+  const uint32_t w = g.segs == 1 ? om.ncx : min(g.seg_w, om.ncx - i0);
+  const uint32_t WY = rows + 2u, WX = w + 2u, nwin = 3u * WY * WX;
+  const uint32_t njb = (om.ncy + g.rows - 1) / g.rows;
+  for (uint32_t q = threadIdx.x; q < nwin; q += kThreads) s_start[q] = kWinEmpty;
+  if (threadIdx.x < 27) {
+    const int dz = int(threadIdx.x) / 9 - 1, dj = (int(threadIdx.x) / 3) % 3 - 1, di = int(threadIdx.x) % 3 - 1;
+    const int64_t kk = int64_t(k) + dz, bb = int64_t(jb) + dj, ss = int64_t(is) + di;
+    const bool in = kk >= 0 && kk < int64_t(om.ncz) && bb >= 0 && bb < int64_t(njb) && ss >= 0 &&
+                    ss < int64_t(g.segs);
+    const uint64_t tq = in ? uint64_t(kk) * g.cols + uint64_t(bb) * g.segs + uint64_t(ss) : 0;
+    s_b0[threadIdx.x] = in ? om.bstart[tq] : 0;
+    s_b1[threadIdx.x] = in ? om.bstart[tq + 1] : 0;
+  }
+  __syncthreads();
+  // record where each window cell's triangles start, and how many it has
+  const uint64_t oend = om.base + om.count;
+  for (int q = 0; q < 27; q++) {
+    const uint64_t b0 = s_b0[q], b1 = s_b1[q];
+    for (uint64_t u = b0 + threadIdx.x; u < b1; u += kThreads) {
+      const uint32_t c = m.rec[u].cell;
+      if (u > b0 && m.rec[u - 1].cell == c) continue;  // not the first triangle of its cell
+      const uint32_t jk = fdiv(c, om.div_ncx);
+      const uint32_t i = c - jk * om.ncx;
+      const uint32_t kc = fdiv(jk, om.div_ncy);
+      const uint32_t j = jk - kc * om.ncy;
+      const int64_t wz = int64_t(kc) - int64_t(k) + 1, wy = int64_t(j) - int64_t(j0) + 1,
+                    wx = int64_t(i) - int64_t(i0) + 1;
+      if (wz < 0 || wz > 2 || wy < 0 || wy >= int64_t(WY) || wx < 0 || wx >= int64_t(WX)) continue;
+      uint32_t n = 1;
+      while (n < 12 && u + n < b1 && m.rec[u + n].cell == c) n++;
+      const uint32_t idx = (uint32_t(wz) * WY + uint32_t(wy)) * WX + uint32_t(wx);
+      s_start[idx] = uint16_t(u - b0);
+      s_cnt[idx] = uint8_t(n);
+    }
+  }
+  __syncthreads();
   const R nan = R(__longlong_as_double(0x7ff8000000000000ll));
-  for (int vv = 0; vv < 3; vv++) {
-    V3<R> n = {nan, nan, nan};
-    if (in) n = vertex_normal<R>(m, om, m.cell[g], (ed >> (5 * vv)) & 31u);
-    out[9 * t + 3 * vv + 0] = n.x;
-    out[9 * t + 3 * vv + 1] = n.y;
-    out[9 * t + 3 * vv + 2] = n.z;
+  for (uint64_t u = t0 + threadIdx.x; u < t1; u += kThreads) {
+    const uint32_t c = m.rec[u].cell;
+    const uint32_t ed = m.rec[u].edges;
+    const uint64_t t = u - om.base;
+    if (!((ed >> 15) & 1u)) {
+#pragma unroll
+      for (int q = 0; q < 9; q++) out[9 * t + q] = nan;
+      continue;
+    }
+    const uint32_t jk = fdiv(c, om.div_ncx);
+    const uint32_t i = c - jk * om.ncx;
+    const uint32_t kc = fdiv(jk, om.div_ncy);
+    const uint32_t j = jk - kc * om.ncy;
+    for (int vv = 0; vv < 3; vv++) {
+      const uint32_t e = (ed >> (5 * vv)) & 31u;
+      V3<R> sum = {R(0), R(0), R(0)};
+      const int nn = c_nbr.n[e];
+      for (int q = 0; q < nn; q++) {
+        const int dk = c_nbr.v[e][q][0], dj = c_nbr.v[e][q][1], di = c_nbr.v[e][q][2];
+        const uint32_t want = uint32_t(c_nbr.v[e][q][3]);
+        // an interior cell's neighbours lie inside the lattice and the window
+        const uint32_t kk = uint32_t(int(kc) + dk), jj = uint32_t(int(j) + dj), ii = uint32_t(int(i) + di);
+        const uint32_t idx = ((kk - k + 1u) * WY + (jj - j0 + 1u)) * WX + (ii - i0 + 1u);
+        const uint16_t o = s_start[idx];
+        if (o == kWinEmpty) continue;
+        const uint32_t cnt = s_cnt[idx];
+        // the window tile holding the cell
+        const int tz = int(kk) - int(k), tj = int(g.rows == 1 ? jj : fdiv(jj, g.div_rows)) - int(jb),
+                  ts = int(g.segs == 1 ? 0u : fdiv(ii, g.div_seg_w)) - int(is);
+        const uint64_t u0 = s_b0[(tz + 1) * 9 + (tj + 1) * 3 + (ts + 1)] + o;
+        // the cell's edge words together, then the matching weights in slot order
+        uint32_t ew[12];
+#pragma unroll
+        for (int r = 0; r < 12; r++) ew[r] = uint32_t(r) < cnt && u0 + r < oend ? m.rec[u0 + r].edges : 0xffffffffu;
+        uint32_t hit = 0;
+#pragma unroll
+        for (int r = 0; r < 12; r++) {
+          const uint32_t eu = ew[r];
+          const bool on = (eu & 31u) == want || ((eu >> 5) & 31u) == want || ((eu >> 10) & 31u) == want;
+          hit |= uint32_t(uint32_t(r) < cnt && on) << r;
+        }
+        for (; hit; hit &= hit - 1u) {
+          const uint64_t u2 = u0 + uint32_t(__builtin_ctz(hit));
+          sum = vadd(sum, V3<R>{m.rec[u2].w[0], m.rec[u2].w[1], m.rec[u2].w[2]});
+        }
+      }
+      const V3<R> n = vunit(sum);
+      out[9 * t + 3 * vv + 0] = n.x;
+      out[9 * t + 3 * vv + 1] = n.y;
+      out[9 * t + 3 * vv + 2] = n.z;
+    }
   }
 }
 
@@ -583,18 +693,18 @@ __global__ __launch_bounds__(kThreads) void k_mesh_compare(MeshView<RA> a, const
   unsigned long long asum = 0, nv = 0, bad = 0, over1 = 0, over10 = 0;
   for (uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x; t < count; t += uint64_t(gridDim.x) * kThreads) {
     const uint64_t ga = oa.base + t, gb = ob.base + t;
-    const uint32_t ea = a.edges[ga], eb = b.edges[gb];
-    if (ea != eb || a.cell[ga] != b.cell[gb]) {
+    const uint32_t ea = a.rec[ga].edges, eb = b.rec[gb].edges;
+    if (ea != eb || a.rec[ga].cell != b.rec[gb].cell) {
       bad++;
       continue;
     }
 #pragma unroll
-    for (int c = 0; c < 9; c++) verr = fmax(verr, fabs(double(a.pos[c * a.cap + ga]) - double(b.pos[c * b.cap + gb])));
+    for (int c = 0; c < 9; c++) verr = fmax(verr, fabs(double(a.rec[ga].p[c]) - double(b.rec[gb].p[c])));
     if (!((ea >> 15) & 1u)) continue;
     for (int v = 0; v < 3; v++) {
       const uint32_t e = (ea >> (5 * v)) & 31u;
-      const V3<RA> na = vertex_normal<RA>(a, oa, a.cell[ga], e);
-      const V3<RB> nb = vertex_normal<RB>(b, ob, b.cell[gb], e);
+      const V3<RA> na = vertex_normal<RA>(a, oa, a.rec[ga].cell, e);
+      const V3<RB> nb = vertex_normal<RB>(b, ob, b.rec[gb].cell, e);
       const double d = double(na.x) * double(nb.x) + double(na.y) * double(nb.y) + double(na.z) * double(nb.z);
       if (!(d == d) || !isfinite(d)) continue;
       const double ang = acos(fmin(1.0, fmax(-1.0, d))) * (180.0 / 3.14159265358979323846);
@@ -642,11 +752,9 @@ __device__ __forceinline__ float unord32(uint32_t k) {
 template <typename R>
 __device__ __forceinline__ bool setup_tri(const MeshView<R>& m, uint64_t t, const ViewDev<R>& v, Projected<R>& P,
                                           R& area) {
-  const uint64_t cap = m.cap;
 #pragma unroll
   for (int n = 0; n < 3; n++) {
-    const V3<R> p = {m.pos[(3 * n + 0) * cap + t], m.pos[(3 * n + 1) * cap + t],
-                     m.pos[(3 * n + 2) * cap + t]};
+    const V3<R> p = {m.rec[t].p[3 * n + 0], m.rec[t].p[3 * n + 1], m.rec[t].p[3 * n + 2]};
     if (!project(v, p, P.q[n])) return false;
   }
   area = raster_box(v, P);
@@ -811,9 +919,9 @@ __device__ __forceinline__ void fetch_tri(const MeshView<R>& m, const RasterJob<
   T.ok = T.t < rjobs[T.jb].tri1;
   T.ed = 0;
   if (T.ok) {
-    T.ed = m.edges[T.t];
+    T.ed = m.rec[T.t].edges;
 #pragma unroll
-    for (int n = 0; n < 9; n++) T.p[n] = m.pos[uint64_t(n) * m.cap + T.t];
+    for (int n = 0; n < 9; n++) T.p[n] = m.rec[T.t].p[n];
   }
 }
 
@@ -1175,18 +1283,22 @@ k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t nitems, cons
 // ----------------------------------------------------------------------
 // tessellation: emit pass (the stored mesh)
 //
-// The tile loop of the queue kernels; every drain computes 256 triangles,
-// one per thread, and writes them at their serial positions: positions,
-// smoothing weight, lattice edge ids with the interior flag, cell.  The
-// 256 consecutive positions of a drain make every output stream's stores
-// contiguous.
+// The work-item loop of the queue kernels; every drain computes up to 256
+// triangles, one per thread, into records in LDS, and the workgroup then
+// stores those records at their serial positions as 8-byte words, thread q
+// taking words q, q + 256, ...: the drain's triangles are a few runs of
+// consecutive positions (one per object in the drained chunks), so every
+// store instruction writes contiguous bytes and whole lines, where one
+// store per record field (14 streams) had written scattered 4- or 8-byte
+// pieces.  The stores are non-temporal: the mesh streams past the caches,
+// and the field planes the workgroups march through stay in L2.
 template <typename R>
-__device__ __forceinline__ void emit_ring(FusedLds& S, uint32_t head, uint32_t n, const TessJob* __restrict__ jobs,
-                                          uint32_t jb, const MeshView<R>& m, FrameHdr* __restrict__ hdr) {
+__device__ __forceinline__ void emit_ring(FusedLds& S, MeshRec<R>* s_rec, uint32_t head, uint32_t n,
+                                          const TessJob* __restrict__ jobs, uint32_t jb, const MeshView<R>& m,
+                                          FrameHdr* __restrict__ hdr) {
   if (threadIdx.x < n) {
     const FieldDev f = jobs[jb].f;
     const uint32_t s = (head + threadIdx.x) & (kRing - 1u);
-    const uint64_t g = S.rg[s];
     const uint32_t cc = S.rc[s], x = S.re[s];
     const CellLoc L = locate(f, cc);
     const uint32_t q = (x >> 18) & 3u;
@@ -1197,27 +1309,28 @@ __device__ __forceinline__ void emit_ring(FusedLds& S, uint32_t head, uint32_t n
     slot_positions_mem<R>(f, L, thr, x & 0x3ffffu, P);
     const V3<R> w = smoothing_weight<R>(P);
     const uint32_t ent_edges = uint32_t(c_cslot[(x >> 24) * 12u + ((x >> 20) & 15u)] >> 18);
-    const uint32_t ed = ent_edges | uint32_t(interior_cell(f, L)) << 15;
-    const uint64_t cap = m.cap;
-    if (g >= cap) {
-      hdr->overflow = 1u;  // capacity reserved ahead of a replayed frame was too small
-    } else {
-      // the mesh streams past the caches (non-temporal stores), so that the
-      // field planes the workgroups march through stay in L2
-      R* p = m.pos + g;
-      R* wp = m.wfn + g;
+    MeshRec<R>& r = s_rec[threadIdx.x];
 #pragma unroll
-      for (int vv = 0; vv < 3; vv++) {
-        __builtin_nontemporal_store(P[vv].x, p + (3 * vv + 0) * cap);
-        __builtin_nontemporal_store(P[vv].y, p + (3 * vv + 1) * cap);
-        __builtin_nontemporal_store(P[vv].z, p + (3 * vv + 2) * cap);
-      }
-      __builtin_nontemporal_store(w.x, wp);
-      __builtin_nontemporal_store(w.y, wp + cap);
-      __builtin_nontemporal_store(w.z, wp + 2 * cap);
-      __builtin_nontemporal_store(ed, m.edges + g);
-      __builtin_nontemporal_store(cc, m.cell + g);
+    for (int vv = 0; vv < 3; vv++) {
+      r.p[3 * vv + 0] = P[vv].x;
+      r.p[3 * vv + 1] = P[vv].y;
+      r.p[3 * vv + 2] = P[vv].z;
     }
+    r.w[0] = w.x;
+    r.w[1] = w.y;
+    r.w[2] = w.z;
+    r.cell = cc;
+    r.edges = ent_edges | uint32_t(interior_cell(f, L)) << 15;
+    if (uint64_t(S.rg[s]) >= m.cap) hdr->overflow = 1u;  // capacity reserved ahead of a replayed frame was too small
+  }
+  __syncthreads();
+  constexpr uint32_t kWords = sizeof(MeshRec<R>) / 8u;
+  const uint64_t* src = reinterpret_cast<const uint64_t*>(s_rec);
+  for (uint32_t k = threadIdx.x; k < n * kWords; k += kThreads) {
+    const uint32_t r = k / kWords;
+    const uint64_t g = S.rg[(head + r) & (kRing - 1u)];
+    if (g < m.cap)
+      __builtin_nontemporal_store(src[k], reinterpret_cast<uint64_t*>(m.rec + g) + (k - r * kWords));
   }
   __syncthreads();
 }
@@ -1227,10 +1340,12 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TessJob* __restrict__ j
                                                    const uint64_t* __restrict__ start, MeshView<R> m,
                                                    FrameHdr* __restrict__ hdr) {
   __shared__ FusedLds S;
+  __shared__ MeshRec<R> s_rec[kThreads];
   S.ntri[threadIdx.x] = c_ntri[threadIdx.x];
   __syncthreads();
-  ring_tiles<false>(S, jobs, njobs, nitems, start, nullptr, 0,
-                    [&](uint32_t head, uint32_t n, uint32_t jb) { emit_ring<R>(S, head, n, jobs, jb, m, hdr); });
+  ring_tiles<false>(S, jobs, njobs, nitems, start, nullptr, 0, [&](uint32_t head, uint32_t n, uint32_t jb) {
+    emit_ring<R>(S, s_rec, head, n, jobs, jb, m, hdr);
+  });
 }
 
 // the latest triangle among the kept fragments at each pixel's final depth
@@ -1353,8 +1468,8 @@ __global__ __launch_bounds__(kThreads) void k_resolve(MeshView<R> m, const Raste
       setup_tri(m, t, v, P, area);
       R w0, w1, w2;
       pixel_weights(P, pixel_centre<R>(i, v.fw), pixel_centre<R>(j, v.fh), w0, w1, w2);
-      const uint32_t c = m.cell[t];
-      const uint32_t ed = m.edges[t];
+      const uint32_t c = m.rec[t].cell;
+      const uint32_t ed = m.rec[t].edges;
       V3<R> nv[3];
 #pragma unroll 1
       for (int n = 0; n < 3; n++) nv[n] = vertex_normal<R>(m, om, c, (ed >> (5 * n)) & 31u);
@@ -1584,11 +1699,12 @@ hipError_t launch_emit(const TessJob* jobs, int njobs, uint32_t nitems, const ui
 }
 
 template <typename R>
-hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, uint64_t count, R* out,
-                                 hipStream_t st) {
-  if (count == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_vertex_normals<R>, dim3(uint32_t((count + kThreads - 1) / kThreads)), dim3(kThreads), 0, st,
-                     m, meta_dev, count, out);
+hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, const TileGeo& tg, uint64_t count,
+                                 uint32_t ntiles, R* out, hipStream_t st) {
+  if (count == 0 || ntiles == 0) return hipSuccess;
+  // the window's start (16-bit) and count (8-bit) tables
+  const size_t lds = size_t(smooth_window(tg)) * 3u;
+  hipLaunchKernelGGL(k_smooth_tiles<R>, dim3(ntiles), dim3(kThreads), lds, st, m, meta_dev, out);
   return hipGetLastError();
 }
 
@@ -1712,10 +1828,10 @@ template hipError_t launch_emit<double>(const TessJob*, int, uint32_t, const uin
                                         FrameHdr*, uint32_t, hipStream_t);
 template hipError_t launch_emit<float>(const TessJob*, int, uint32_t, const uint64_t*, const MeshView<float>&,
                                        FrameHdr*, uint32_t, hipStream_t);
-template hipError_t launch_vertex_normals<double>(const MeshView<double>&, const ObjMeta*, uint64_t, double*,
-                                                  hipStream_t);
-template hipError_t launch_vertex_normals<float>(const MeshView<float>&, const ObjMeta*, uint64_t, float*,
-                                                 hipStream_t);
+template hipError_t launch_vertex_normals<double>(const MeshView<double>&, const ObjMeta*, const TileGeo&, uint64_t,
+                                                  uint32_t, double*, hipStream_t);
+template hipError_t launch_vertex_normals<float>(const MeshView<float>&, const ObjMeta*, const TileGeo&, uint64_t,
+                                                 uint32_t, float*, hipStream_t);
 template hipError_t launch_raster<double>(const MeshView<double>&, const RasterJob<double>*, int, FrameHdr*, int,
                                           uint64_t*, uint32_t*, Frag*, uint32_t, uint32_t, hipStream_t);
 template hipError_t launch_raster<float>(const MeshView<float>&, const RasterJob<float>*, int, FrameHdr*, int,

@@ -88,6 +88,7 @@ inline TileGeo make_tile_geo(uint32_t ncx, uint32_t ncy, uint32_t ncz, uint32_t 
 // l0 + tile_corner_off(p).
 struct TileSpan {
   uint32_t j0, i0, w, n;
+  uint32_t rows;  // rows of cells (n = rows * w)
   uint32_t c0;
   uint64_t l0;
 };
@@ -102,6 +103,7 @@ C3D_TD TileSpan tile_span(const TileGeo& g, uint32_t ncx, uint32_t ncy, uint32_t
   const uint32_t rows = g.rows < ncy - t.j0 ? g.rows : ncy - t.j0;
   t.w = g.segs == 1 ? ncx : (g.seg_w < ncx - t.i0 ? g.seg_w : ncx - t.i0);
   t.n = rows * t.w;
+  t.rows = rows;
   t.c0 = (k * ncy + t.j0) * ncx + t.i0;
   t.l0 = (uint64_t(k) * ny + t.j0) * nx + t.i0;
   return t;

@@ -36,6 +36,7 @@ static int check(uint32_t ncx, uint32_t ncy, uint32_t ncz, uint32_t max_rows, ui
     if (col >= g.cols || k0 >= k1 || k1 > ncz) return fail("item range", item, k0);
     const TileSpan t = tile_span(g, ncx, ncy, nx, ny, k0, col);
     if (t.n == 0 || t.n > uint32_t(kTileCells)) return fail("tile size", item, t.n);
+    if (t.rows * t.w != t.n) return fail("tile rows", t.rows, t.n);
     for (uint32_t k = k0; k < k1; k++) {
       const uint32_t tile = k * g.cols + col;
       const uint32_t ctile = t.c0 + (k - k0) * (ncx * ncy);
