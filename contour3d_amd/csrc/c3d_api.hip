@@ -189,6 +189,12 @@ struct contour3d_amd_mesh {
   c3d::DevBuf cell;    // a frame: the cell index of each triangle (cap entries)
   c3d::DevBuf bstart;  // per-tile triangle starts of every object's field
   c3d::DevBuf meta;    // ObjMeta per object
+  c3d::DevBuf code;    // a stored mesh: the corner code of every cell of every object (ObjMeta::code_off)
+  // and each cell's first triangle within its tile: a cache made by the
+  // first smoothing of a build (calls on a mesh hold the library's lock)
+  mutable c3d::DevBuf cstart;
+  mutable bool starts_ready = false;
+  uint64_t code_bytes = 0;
   uint64_t cap = 0;
   uint64_t total = 0;
   uint64_t ntiles = 0;  // entries of bstart minus one
@@ -201,6 +207,9 @@ struct contour3d_amd_mesh {
     cell.release();
     bstart.release();
     meta.release();
+    code.release();
+    cstart.release();
+    starts_ready = false;
     cap = 0;
   }
 };
@@ -674,6 +683,7 @@ MeshView<R> view_of(const contour3d_amd_mesh& M) {
   MeshView<R> m;
   m.rec = M.rec.as<MeshRec<R>>();
   m.cap = M.cap;
+  m.code = M.code.as<uint8_t>();
   return m;
 }
 
@@ -854,6 +864,7 @@ int ensure_cells(contour3d_amd_mesh& M) {
 
 template <typename R>
 int ensure_capacity(contour3d_amd_mesh& M) {
+  C3D_HIP(M.code.ensure(M.code_bytes, true));
   if (M.cap >= M.total && M.cap != 0 && M.precision_built == M.precision) return 0;
   M.precision_built = M.precision;
   const uint64_t cap = M.total + M.total / 4 + 1024;
@@ -1004,13 +1015,17 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
   tj.clear();
   F.obj_job.assign(n, 0);
   F.obj_q.assign(n, 0);
-  uint64_t jt = 0, jx = 0;
+  uint64_t jt = 0, jx = 0, codes = 0;
+  std::vector<uint64_t> code_off(n, 0);
   for (size_t o = 0; o < n; o++) {
     if (group[o] != int(o)) continue;
     TessJob job{};
     job.f = fd[o];
+    job.code_off = codes;
     for (size_t q = o; q < n; q++) {
       if (group[q] != int(o)) continue;
+      code_off[q] = codes;
+      codes += fd[o].ncells;
       job.ts.thr[job.ts.n] = objs[q]->threshold;
       job.ts.boff[job.ts.n] = boff[q];
       job.ts.obj[job.ts.n] = uint32_t(q);
@@ -1024,6 +1039,7 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
         jx += (fd[o].nitems + 7) / 8;
         tj.push_back(job);
         job.ts = ThrSet{};
+        job.code_off = codes;
       }
     }
     if (job.ts.n) {
@@ -1039,6 +1055,8 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
     return 1;
   }
   const uint32_t ntj = uint32_t(jt);
+  M.code_bytes = codes;  // a stored mesh keeps every cell's corner code
+  M.starts_ready = false;
 
   C3D_HIP(F.cnt.ensure((nbt + 1) * sizeof(uint32_t)));
   C3D_HIP(M.bstart.ensure((nbt + 1) * sizeof(uint64_t)));
@@ -1060,6 +1078,7 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
     hm[o].tile_off = boff[o];
     hm[o].job = F.obj_job[o];
     hm[o].q = F.obj_q[o];
+    hm[o].code_off = code_off[o];
     hm[o].rgb[0] = objs[o]->color.r;
     hm[o].rgb[1] = objs[o]->color.g;
     hm[o].rgb[2] = objs[o]->color.b;
@@ -2082,12 +2101,24 @@ int smooth_object(Frame& F, const contour3d_amd_mesh& M, size_t obj, void* out) 
   if (n == 0) return 0;
   const ObjMeta* md = M.meta.as<ObjMeta>() + obj;
   if (prof_begin(F, CONTOUR3D_AMD_STAGE_SMOOTH)) return 1;
-  const uint32_t nt = M.tiles[obj];
-  const TileGeo& tg = M.geo[obj];
-  hipError_t e =
-      M.precision == CONTOUR3D_AMD_FP64
-          ? launch_vertex_normals<double>(view_of<double>(M), md, tg, n, nt, static_cast<double*>(out), F.st)
-          : launch_vertex_normals<float>(view_of<float>(M), md, tg, n, nt, static_cast<float*>(out), F.st);
+  // the cell starts of every object, once per build
+  hipError_t e = hipSuccess;
+  if (!M.starts_ready) {
+    C3D_HIP(M.cstart.ensure(M.code_bytes * sizeof(uint16_t)));
+    for (size_t o = 0; o < M.count.size() && e == hipSuccess; o++)
+      if (M.count[o]) e = launch_cell_starts(M.code.as<uint8_t>(), M.meta.as<ObjMeta>() + o, M.tiles[o],
+                                             M.cstart.as<uint16_t>(), F.st);
+    if (e != hipSuccess) {
+      log_error("cell starts: %s", hipGetErrorString(e));
+      return 1;
+    }
+    M.starts_ready = true;
+  }
+  e = M.precision == CONTOUR3D_AMD_FP64
+          ? launch_vertex_normals<double>(view_of<double>(M), md, M.cstart.as<uint16_t>(), n,
+                                          static_cast<double*>(out), F.st)
+          : launch_vertex_normals<float>(view_of<float>(M), md, M.cstart.as<uint16_t>(), n,
+                                         static_cast<float*>(out), F.st);
   if (e != hipSuccess) {
     log_error("vertex normals: %s", hipGetErrorString(e));
     return 1;

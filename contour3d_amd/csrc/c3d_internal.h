@@ -55,6 +55,7 @@ struct TessJob {
   ThrSet ts;
   uint32_t item0;
   uint32_t xoff;
+  uint64_t code_off;  // a stored mesh: threshold q's cell codes at code_off + q * f.ncells
 };
 
 // One stored triangle: 56 bytes in the FP32 form (R = float), 104 in the
@@ -69,11 +70,13 @@ struct alignas(8) MeshRec {
 };
 static_assert(sizeof(MeshRec<float>) == 56 && sizeof(MeshRec<double>) == 104, "record layout");
 
-// Triangle storage: cap records.
+// Triangle storage: cap records, and the corner code (8 bits) of every
+// cell of every object's field, object after object (ObjMeta::code_off).
 template <typename R>
 struct MeshView {
   MeshRec<R>* rec;
   uint64_t cap;
+  uint8_t* code;
 };
 
 // Per contour object, on the device.
@@ -87,6 +90,7 @@ struct ObjMeta {
   uint32_t tile_off;       // offset of the object's first tile in the frame's tile arrays
   uint32_t job, q;         // its field pass and threshold slot there
   uint8_t rgb[4];
+  uint64_t code_off;       // a stored mesh: first of its cells' corner codes (and cell starts)
 };
 
 // One covered (pixel, triangle) of the FP64 z-buffer pass.
@@ -152,12 +156,16 @@ hipError_t launch_plan(const uint64_t* start, uint64_t ntiles, ObjMeta* meta, in
 template <typename R>
 hipError_t launch_emit(const TessJob* jobs, int njobs, uint32_t nitems, const uint64_t* start,
                        const MeshView<R>& m, FrameHdr* hdr, uint32_t grid, hipStream_t st);
+// every cell's first triangle within its tile (cstart, 16 bits), for one
+// object of a stored mesh, from the cell codes the emit pass stored
+hipError_t launch_cell_starts(const uint8_t* code, const ObjMeta* meta_dev, uint32_t ntiles, uint16_t* cstart,
+                              hipStream_t st);
 // unit vertex normals of every triangle of one object, 9 values of the
 // mesh's precision per triangle (NaN rows for triangles outside the
-// rendered interior)
+// rendered interior); needs the object's cell starts
 template <typename R>
-hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, const TileGeo& tg, uint64_t count,
-                                 uint32_t ntiles, R* out, hipStream_t st);
+hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, const uint16_t* cstart,
+                                 uint64_t count, R* out, hipStream_t st);
 // FP64 mesh a against mesh b (FP64, or FP32 when b_fp32; b points to its
 // MeshView) for one object: out[0..4] as k_mesh_compare documents
 hipError_t launch_mesh_compare(const MeshView<double>& a, const ObjMeta* ma, const void* b, int b_fp32,

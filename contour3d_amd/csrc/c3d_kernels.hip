@@ -547,132 +547,94 @@ __device__ V3<R> vertex_normal(const MeshView<R>& m, const ObjMeta& om, uint32_t
   return vunit(sum);
 }
 
-// Vertex normals of every triangle of one object, a workgroup per tile.
-// The cells a tile's vertex normals gather from lie in the tile's window:
-// the planes k-1..k+1, the rows j0-1..j1+1 and the columns i0-1..i0+w of the
-// tile's cells.  The workgroup first records, for every non-empty window
-// cell, where its triangles start (an offset within the tile holding it: a
-// tile holds at most 12 * kTileCells triangles, so 16 bits) and how many
-// there are, then each thread sums the smoothing weights of its triangles'
-// vertices exactly as vertex_normal does -- cells in the neighbour table's
-// increasing order, each cell's triangles in slot order, from +0 -- with
-// one LDS read per neighbour cell and independent reads of that cell's
-// records.  Rows of triangles outside the rendered interior are NaN.
-// Grid: one workgroup per tile of the object; dynamic LDS of
-// smooth_lds_bytes(window cells).
-constexpr uint16_t kWinEmpty = 0xffffu;
-
-__host__ __device__ inline uint32_t smooth_window(const TileGeo& g) {
-  return 3u * (g.rows + 2u) * (g.seg_w + 2u);
+// Cell starts of one object of a stored mesh, a workgroup per tile: for
+// every cell c of the tile, the number of the object's triangles in the
+// tile's cells before c (a tile holds at most 12 * kTileCells triangles, so
+// 16 bits), from the corner codes the emit pass stored.  The triangles of
+// cell c are then bstart[tile] + cstart[c] + slot, slot < c_ntri[code[c]],
+// in slot order.
+__global__ __launch_bounds__(kThreads) void k_cell_starts(const uint8_t* __restrict__ code,
+                                                          const ObjMeta* __restrict__ meta,
+                                                          uint16_t* __restrict__ cstart) {
+  __shared__ uint64_t s_wave[kThreads / 64];
+  __shared__ uint8_t s_ntri[256];
+  s_ntri[threadIdx.x] = c_ntri[threadIdx.x];
+  __syncthreads();
+  const ObjMeta om = *meta;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t k = fdiv(tile, om.tg.div_cols);
+  const TileSpan t = tile_span(om.tg, om.ncx, om.ncy, om.ncx + 1u, om.ncy + 1u, k, tile - k * om.tg.cols);
+  const uint8_t* cd = code + om.code_off;
+  uint16_t* cs = cstart + om.code_off;
+  uint32_t run = 0;
+  for (uint32_t ch = 0; ch < t.n; ch += kThreads) {
+    const uint32_t p = ch + threadIdx.x;
+    const uint32_t n = p < t.n ? s_ntri[cd[t.c0 + p]] : 0u;
+    uint64_t tot;
+    const uint32_t ex = uint32_t(block_excl_scan64(n, s_wave, tot));
+    if (p < t.n) cs[t.c0 + p] = uint16_t(run + ex);
+    run += uint32_t(tot);
+  }
 }
 
+// Vertex normals of every triangle of one object, a thread per (triangle,
+// vertex): the sum of vertex_normal -- cells sharing the vertex's lattice
+// edge in the neighbour table's increasing order, in each the triangles
+// with a vertex on the edge in slot order (c_emask of the cell's stored
+// corner code), from +0 -- gathered directly: a neighbour cell's triangles
+// start at bstart[its tile] + cstart[cell].  A thread's (at most four)
+// neighbour cells are fetched together.  Rows of triangles outside the
+// rendered interior are NaN.  (Measured against this: one 32-bit cell word
+// of code and start, the masks in LDS and a persistent grid were 5% slower;
+// a workgroup per tile gathering a window of cells into LDS 3-4x slower.)
 template <typename R>
-__global__ __launch_bounds__(kThreads) void k_smooth_tiles(MeshView<R> m, const ObjMeta* __restrict__ meta,
-                                                           R* __restrict__ out) {
-  extern __shared__ uint16_t s_dyn[];
-  __shared__ uint64_t s_b0[27], s_b1[27];  // triangle range of each window tile (dz, djb, dis); empty outside
+__global__ __launch_bounds__(kThreads) void k_smooth(MeshView<R> m, const ObjMeta* __restrict__ meta,
+                                                     const uint16_t* __restrict__ cstart, R* __restrict__ out) {
   const ObjMeta om = *meta;
-  const TileGeo& g = om.tg;
-  const uint32_t nwin_max = smooth_window(g);
-  uint16_t* s_start = s_dyn;
-  uint8_t* s_cnt = reinterpret_cast<uint8_t*>(s_dyn + nwin_max);
-  const uint32_t tile = blockIdx.x;
-  const uint64_t t0 = om.bstart[tile], t1 = om.bstart[tile + 1];
-  if (t0 == t1) return;
-  const uint32_t k = fdiv(tile, g.div_cols);
-  const uint32_t col = tile - k * g.cols;
-  const uint32_t jb = g.segs == 1 ? col : col / g.segs;
-  const uint32_t is = col - jb * g.segs;
-  const uint32_t j0 = jb * g.rows, i0 = is * g.seg_w;
-  const uint32_t rows = min(g.rows, om.ncy - j0);
-  const uint32_t w = g.segs == 1 ? om.ncx : min(g.seg_w, om.ncx - i0);
-  const uint32_t WY = rows + 2u, WX = w + 2u, nwin = 3u * WY * WX;
-  const uint32_t njb = (om.ncy + g.rows - 1) / g.rows;
-  for (uint32_t q = threadIdx.x; q < nwin; q += kThreads) s_start[q] = kWinEmpty;
-  if (threadIdx.x < 27) {
-    const int dz = int(threadIdx.x) / 9 - 1, dj = (int(threadIdx.x) / 3) % 3 - 1, di = int(threadIdx.x) % 3 - 1;
-    const int64_t kk = int64_t(k) + dz, bb = int64_t(jb) + dj, ss = int64_t(is) + di;
-    const bool in = kk >= 0 && kk < int64_t(om.ncz) && bb >= 0 && bb < int64_t(njb) && ss >= 0 &&
-                    ss < int64_t(g.segs);
-    const uint64_t tq = in ? uint64_t(kk) * g.cols + uint64_t(bb) * g.segs + uint64_t(ss) : 0;
-    s_b0[threadIdx.x] = in ? om.bstart[tq] : 0;
-    s_b1[threadIdx.x] = in ? om.bstart[tq + 1] : 0;
-  }
-  __syncthreads();
-  // record where each window cell's triangles start, and how many it has
-  const uint64_t oend = om.base + om.count;
-  for (int q = 0; q < 27; q++) {
-    const uint64_t b0 = s_b0[q], b1 = s_b1[q];
-    for (uint64_t u = b0 + threadIdx.x; u < b1; u += kThreads) {
-      const uint32_t c = m.rec[u].cell;
-      if (u > b0 && m.rec[u - 1].cell == c) continue;  // not the first triangle of its cell
-      const uint32_t jk = fdiv(c, om.div_ncx);
-      const uint32_t i = c - jk * om.ncx;
-      const uint32_t kc = fdiv(jk, om.div_ncy);
-      const uint32_t j = jk - kc * om.ncy;
-      const int64_t wz = int64_t(kc) - int64_t(k) + 1, wy = int64_t(j) - int64_t(j0) + 1,
-                    wx = int64_t(i) - int64_t(i0) + 1;
-      if (wz < 0 || wz > 2 || wy < 0 || wy >= int64_t(WY) || wx < 0 || wx >= int64_t(WX)) continue;
-      uint32_t n = 1;
-      while (n < 12 && u + n < b1 && m.rec[u + n].cell == c) n++;
-      const uint32_t idx = (uint32_t(wz) * WY + uint32_t(wy)) * WX + uint32_t(wx);
-      s_start[idx] = uint16_t(u - b0);
-      s_cnt[idx] = uint8_t(n);
-    }
-  }
-  __syncthreads();
+  const uint64_t n3 = 3 * om.count;
+  const uint8_t* cd = m.code + om.code_off;
+  const uint16_t* cs = cstart + om.code_off;
+  const int64_t sy = om.ncx, sz = int64_t(om.ncx) * om.ncy;
   const R nan = R(__longlong_as_double(0x7ff8000000000000ll));
-  for (uint64_t u = t0 + threadIdx.x; u < t1; u += kThreads) {
-    const uint32_t c = m.rec[u].cell;
-    const uint32_t ed = m.rec[u].edges;
-    const uint64_t t = u - om.base;
+  for (uint64_t it = uint64_t(blockIdx.x) * kThreads + threadIdx.x; it < n3; it += uint64_t(gridDim.x) * kThreads) {
+    const uint64_t t = it / 3u;
+    const uint32_t vv = uint32_t(it - 3u * t);
+    const uint64_t u = om.base + t;
+    const uint32_t c = m.rec[u].cell, ed = m.rec[u].edges;
+    R* o = out + 9u * t + 3u * vv;
     if (!((ed >> 15) & 1u)) {
-#pragma unroll
-      for (int q = 0; q < 9; q++) out[9 * t + q] = nan;
+      o[0] = nan;
+      o[1] = nan;
+      o[2] = nan;
       continue;
     }
-    const uint32_t jk = fdiv(c, om.div_ncx);
-    const uint32_t i = c - jk * om.ncx;
-    const uint32_t kc = fdiv(jk, om.div_ncy);
-    const uint32_t j = jk - kc * om.ncy;
-    for (int vv = 0; vv < 3; vv++) {
-      const uint32_t e = (ed >> (5 * vv)) & 31u;
-      V3<R> sum = {R(0), R(0), R(0)};
-      const int nn = c_nbr.n[e];
-      for (int q = 0; q < nn; q++) {
-        const int dk = c_nbr.v[e][q][0], dj = c_nbr.v[e][q][1], di = c_nbr.v[e][q][2];
-        const uint32_t want = uint32_t(c_nbr.v[e][q][3]);
-        // an interior cell's neighbours lie inside the lattice and the window
-        const uint32_t kk = uint32_t(int(kc) + dk), jj = uint32_t(int(j) + dj), ii = uint32_t(int(i) + di);
-        const uint32_t idx = ((kk - k + 1u) * WY + (jj - j0 + 1u)) * WX + (ii - i0 + 1u);
-        const uint16_t o = s_start[idx];
-        if (o == kWinEmpty) continue;
-        const uint32_t cnt = s_cnt[idx];
-        // the window tile holding the cell
-        const int tz = int(kk) - int(k), tj = int(g.rows == 1 ? jj : fdiv(jj, g.div_rows)) - int(jb),
-                  ts = int(g.segs == 1 ? 0u : fdiv(ii, g.div_seg_w)) - int(is);
-        const uint64_t u0 = s_b0[(tz + 1) * 9 + (tj + 1) * 3 + (ts + 1)] + o;
-        // the cell's edge words together, then the matching weights in slot order
-        uint32_t ew[12];
+    const uint32_t e = (ed >> (5u * vv)) & 31u;
+    const int nn = c_nbr.n[e];
+    uint32_t mask[4];
+    uint64_t s2[4];
 #pragma unroll
-        for (int r = 0; r < 12; r++) ew[r] = uint32_t(r) < cnt && u0 + r < oend ? m.rec[u0 + r].edges : 0xffffffffu;
-        uint32_t hit = 0;
-#pragma unroll
-        for (int r = 0; r < 12; r++) {
-          const uint32_t eu = ew[r];
-          const bool on = (eu & 31u) == want || ((eu >> 5) & 31u) == want || ((eu >> 10) & 31u) == want;
-          hit |= uint32_t(uint32_t(r) < cnt && on) << r;
-        }
-        for (; hit; hit &= hit - 1u) {
-          const uint64_t u2 = u0 + uint32_t(__builtin_ctz(hit));
-          sum = vadd(sum, V3<R>{m.rec[u2].w[0], m.rec[u2].w[1], m.rec[u2].w[2]});
-        }
+    for (int q = 0; q < 4; q++) {
+      mask[q] = 0;
+      s2[q] = 0;
+      if (q < nn) {
+        // an interior cell's neighbours lie inside the lattice
+        const uint32_t c2 = uint32_t(int64_t(c) + c_nbr.v[e][q][0] * sz + c_nbr.v[e][q][1] * sy + c_nbr.v[e][q][2]);
+        mask[q] = c_emask[uint32_t(cd[c2]) * 19u + uint32_t(c_nbr.v[e][q][3])];
+        s2[q] = om.bstart[tile_of_cell(om, c2)] + cs[c2];
       }
-      const V3<R> n = vunit(sum);
-      out[9 * t + 3 * vv + 0] = n.x;
-      out[9 * t + 3 * vv + 1] = n.y;
-      out[9 * t + 3 * vv + 2] = n.z;
     }
+    V3<R> sum = {R(0), R(0), R(0)};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      for (uint32_t h = mask[q]; h; h &= h - 1u) {
+        const uint64_t u2 = s2[q] + uint32_t(__builtin_ctz(h));
+        if (u2 < m.cap) sum = vadd(sum, V3<R>{m.rec[u2].w[0], m.rec[u2].w[1], m.rec[u2].w[2]});
+      }
+    }
+    const V3<R> nv = vunit(sum);
+    o[0] = nv.x;
+    o[1] = nv.y;
+    o[2] = nv.z;
   }
 }
 
@@ -1124,11 +1086,15 @@ __device__ __forceinline__ void drain_ring(FusedLds& S, Frag* s_frag, uint32_t h
 // serial order and calls drain(head, n, jb) for every full 256 items (and
 // for the rest when the field pass changes and at the end).  With
 // kStoreCell the cell index of each triangle is stored at its serial
-// position as it is queued.  Expects S.ntri loaded and a barrier since.
-template <bool kStoreCell, typename Drain>
+// position as it is queued; with kStoreCode every cell's corner code of
+// every threshold q is stored at codeout[code_off + q * ncells + cell]: by
+// each cell's thread as bytes (kStoreCode 1) or from LDS as 4-byte words
+// (kStoreCode 2; measured faster in the FP64 emit, slower in the FP32 one).
+// Expects S.ntri loaded and a barrier since.
+template <bool kStoreCell, int kStoreCode, typename Drain>
 __device__ __forceinline__ void ring_tiles(FusedLds& S, const TessJob* __restrict__ jobs, int njobs, uint32_t nitems,
                                            const uint64_t* __restrict__ start, uint32_t* __restrict__ cellout,
-                                           uint64_t cap, Drain&& drain) {
+                                           uint64_t cap, uint8_t* __restrict__ codeout, Drain&& drain) {
   // ring state, the same in every thread: oldest item, items queued, and
   // the field pass they belong to
   uint32_t head = 0, npend = 0, cur = 0;
@@ -1189,6 +1155,12 @@ __device__ __forceinline__ void ring_tiles(FusedLds& S, const TessJob* __restric
             code[q] = q < ts.n ? cd : 0u;
             packed |= uint64_t(S.ntri[code[q]]) << (16 * q);
           }
+          if (kStoreCode == 1) {
+            uint8_t* co = codeout + jobs[jb].code_off + (c0 + threadIdx.x);
+#pragma unroll
+            for (int q = 0; q < kMaxThr; q++)
+              if (q < ts.n) co[uint64_t(q) * f.ncells] = uint8_t(code[q]);
+          }
         }
         uint64_t tot;
         const uint64_t ex = block_excl_scan64(packed, S.wave, tot);
@@ -1201,6 +1173,29 @@ __device__ __forceinline__ void ring_tiles(FusedLds& S, const TessJob* __restric
           if (valid) S.code[q][threadIdx.x] = uint8_t(code[q]);
         }
         __syncthreads();
+        if (kStoreCode == 2) {
+          // the chunk's codes of threshold q are bytes [d0, d0 + nv) of the
+          // code array: whole 4-byte words inside that range are stored as
+          // words, the bytes of the (at most two) words it shares with the
+          // neighbouring chunks one by one
+          const uint32_t nv = t.n - ch * kThreads < kThreads ? t.n - ch * kThreads : kThreads;
+          const uint32_t nw = (kThreads + 3u) / 4u + 1u;  // words touched by 256 bytes at any alignment
+          for (uint32_t w = threadIdx.x; w < uint32_t(ts.n) * nw; w += kThreads) {
+            const uint32_t q = w / nw, kw = w - q * nw;
+            const uint64_t d0 = jobs[jb].code_off + uint64_t(q) * f.ncells + c0;
+            const uint64_t a = (d0 & ~uint64_t(3)) + 4u * kw;
+            const uint64_t lo = a > d0 ? a : d0, hi = a + 4u < d0 + nv ? a + 4u : d0 + nv;
+            if (lo >= hi) continue;
+            if (lo == a && hi == a + 4u) {
+              const uint32_t l = uint32_t(a - d0);
+              const uint32_t word = uint32_t(S.code[q][l]) | uint32_t(S.code[q][l + 1]) << 8 |
+                                    uint32_t(S.code[q][l + 2]) << 16 | uint32_t(S.code[q][l + 3]) << 24;
+              *reinterpret_cast<uint32_t*>(codeout + a) = word;
+            } else {
+              for (uint64_t b = lo; b < hi; b++) codeout[b] = S.code[q][uint32_t(b - d0)];
+            }
+          }
+        }
         const uint32_t total = qoff[kMaxThr];
         for (uint32_t pos = 0; pos < total;) {
           // queue as many of the chunk's triangles as the ring holds, in
@@ -1272,8 +1267,8 @@ k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t nitems, cons
   __syncthreads();
   // the first pass of a frame (Append in FP64, the packed-key z-test in
   // FP32) stores the cell indices
-  ring_tiles<MODE == kRasterAppend || MODE == kRasterFp32>(S, jobs, njobs, nitems, start, cellout, cap,
-                                    [&](uint32_t head, uint32_t n, uint32_t jb) {
+  ring_tiles<MODE == kRasterAppend || MODE == kRasterFp32, 0>(S, jobs, njobs, nitems, start, cellout, cap,
+                                                                   nullptr, [&](uint32_t head, uint32_t n, uint32_t jb) {
                                       drain_ring<R, MODE>(S, s_frag, head, n, jobs, jb, rjobs, jimg, hdr, key, id,
                                                           frags, fcap);
                                     });
@@ -1343,7 +1338,8 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TessJob* __restrict__ j
   __shared__ MeshRec<R> s_rec[kThreads];
   S.ntri[threadIdx.x] = c_ntri[threadIdx.x];
   __syncthreads();
-  ring_tiles<false>(S, jobs, njobs, nitems, start, nullptr, 0, [&](uint32_t head, uint32_t n, uint32_t jb) {
+  ring_tiles<false, sizeof(R) == 8 ? 2 : 1>(S, jobs, njobs, nitems, start, nullptr, 0, m.code,
+                                            [&](uint32_t head, uint32_t n, uint32_t jb) {
     emit_ring<R>(S, s_rec, head, n, jobs, jb, m, hdr);
   });
 }
@@ -1698,13 +1694,21 @@ hipError_t launch_emit(const TessJob* jobs, int njobs, uint32_t nitems, const ui
   return hipGetLastError();
 }
 
+hipError_t launch_cell_starts(const uint8_t* code, const ObjMeta* meta_dev, uint32_t ntiles, uint16_t* cstart,
+                              hipStream_t st) {
+  if (ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cell_starts, dim3(ntiles), dim3(kThreads), 0, st, code, meta_dev, cstart);
+  return hipGetLastError();
+}
+
 template <typename R>
-hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, const TileGeo& tg, uint64_t count,
-                                 uint32_t ntiles, R* out, hipStream_t st) {
-  if (count == 0 || ntiles == 0) return hipSuccess;
-  // the window's start (16-bit) and count (8-bit) tables
-  const size_t lds = size_t(smooth_window(tg)) * 3u;
-  hipLaunchKernelGGL(k_smooth_tiles<R>, dim3(ntiles), dim3(kThreads), lds, st, m, meta_dev, out);
+hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, const uint16_t* cstart,
+                                 uint64_t count, R* out, hipStream_t st) {
+  if (count == 0) return hipSuccess;
+  // one item per thread (the kernel strides past 2^24 workgroups)
+  const uint64_t blocks = (3 * count + kThreads - 1) / kThreads;
+  const uint32_t grid = uint32_t(blocks < (uint64_t(1) << 24) ? blocks : (uint64_t(1) << 24));
+  hipLaunchKernelGGL(k_smooth<R>, dim3(grid), dim3(kThreads), 0, st, m, meta_dev, cstart, out);
   return hipGetLastError();
 }
 
@@ -1828,10 +1832,10 @@ template hipError_t launch_emit<double>(const TessJob*, int, uint32_t, const uin
                                         FrameHdr*, uint32_t, hipStream_t);
 template hipError_t launch_emit<float>(const TessJob*, int, uint32_t, const uint64_t*, const MeshView<float>&,
                                        FrameHdr*, uint32_t, hipStream_t);
-template hipError_t launch_vertex_normals<double>(const MeshView<double>&, const ObjMeta*, const TileGeo&, uint64_t,
-                                                  uint32_t, double*, hipStream_t);
-template hipError_t launch_vertex_normals<float>(const MeshView<float>&, const ObjMeta*, const TileGeo&, uint64_t,
-                                                 uint32_t, float*, hipStream_t);
+template hipError_t launch_vertex_normals<double>(const MeshView<double>&, const ObjMeta*, const uint16_t*,
+                                                  uint64_t, double*, hipStream_t);
+template hipError_t launch_vertex_normals<float>(const MeshView<float>&, const ObjMeta*, const uint16_t*, uint64_t,
+                                                 float*, hipStream_t);
 template hipError_t launch_raster<double>(const MeshView<double>&, const RasterJob<double>*, int, FrameHdr*, int,
                                           uint64_t*, uint32_t*, Frag*, uint32_t, uint32_t, hipStream_t);
 template hipError_t launch_raster<float>(const MeshView<float>&, const RasterJob<float>*, int, FrameHdr*, int,
