@@ -584,9 +584,13 @@ __global__ __launch_bounds__(kThreads) void k_cell_starts(const uint8_t* __restr
 // corner code), from +0 -- gathered directly: a neighbour cell's triangles
 // start at bstart[its tile] + cstart[cell].  A thread's (at most four)
 // neighbour cells are fetched together.  Rows of triangles outside the
-// rendered interior are NaN.  (Measured against this: one 32-bit cell word
-// of code and start, the masks in LDS and a persistent grid were 5% slower;
-// a workgroup per tile gathering a window of cells into LDS 3-4x slower.)
+// rendered interior are NaN.
+//
+// (Measured against this: one 32-bit cell word of code and start, the masks
+// in LDS and a persistent grid were 5% slower; a workgroup per tile, tiles
+// z-fastest along each column and one contiguous run per XCD (to keep the
+// planes above and below in L2), 1.55x slower; a workgroup per tile
+// gathering a window of cells into LDS 3-4x slower.)
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_smooth(MeshView<R> m, const ObjMeta* __restrict__ meta,
                                                      const uint16_t* __restrict__ cstart, R* __restrict__ out) {
@@ -1287,6 +1291,11 @@ k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t nitems, cons
 // store per record field (14 streams) had written scattered 4- or 8-byte
 // pieces.  The stores are non-temporal: the mesh streams past the caches,
 // and the field planes the workgroups march through stay in L2.
+//
+// Every earlier vector memory operation is waited for once before the
+// drain's stores, so that nothing is in flight when they issue: the
+// compiler then issues the store sequence back to back, where otherwise it
+// waited between stores for loads still counted ahead of them.
 template <typename R>
 __device__ __forceinline__ void emit_ring(FusedLds& S, MeshRec<R>* s_rec, uint32_t head, uint32_t n,
                                           const TessJob* __restrict__ jobs, uint32_t jb, const MeshView<R>& m,
@@ -1319,13 +1328,16 @@ __device__ __forceinline__ void emit_ring(FusedLds& S, MeshRec<R>* s_rec, uint32
     if (uint64_t(S.rg[s]) >= m.cap) hdr->overflow = 1u;  // capacity reserved ahead of a replayed frame was too small
   }
   __syncthreads();
+  __builtin_amdgcn_s_waitcnt(0);
   constexpr uint32_t kWords = sizeof(MeshRec<R>) / 8u;
   const uint64_t* src = reinterpret_cast<const uint64_t*>(s_rec);
+  char* dst = reinterpret_cast<char*>(m.rec);
   for (uint32_t k = threadIdx.x; k < n * kWords; k += kThreads) {
     const uint32_t r = k / kWords;
-    const uint64_t g = S.rg[(head + r) & (kRing - 1u)];
+    const uint32_t g = S.rg[(head + r) & (kRing - 1u)];
     if (g < m.cap)
-      __builtin_nontemporal_store(src[k], reinterpret_cast<uint64_t*>(m.rec + g) + (k - r * kWords));
+      __builtin_nontemporal_store(src[k], reinterpret_cast<uint64_t*>(dst + uint64_t(g) * sizeof(MeshRec<R>) +
+                                                                     8u * (k - r * kWords)));
   }
   __syncthreads();
 }
