@@ -272,6 +272,17 @@ def shared_scene(api, fields, contours, reps):
         log(f"shared scene: plain {name}")
         plain(prec)
         res[f"plain_{name}_fps"] = round(len(views) / _timed(lambda: plain(prec), reps), 2)
+        # the same loop once more with the stage timers on: the device time
+        # of the frames' stages beside the wall time per frame, so that a
+        # slow loop shows whether the kernels or the gaps between them grew
+        api.profile_reset()
+        api.profile_enable(True)
+        wall = _timed(lambda: plain(prec), 1)
+        api.profile_enable(False)
+        ms = api.profile_read()[0]
+        res[f"plain_{name}_ms_per_frame"] = {"wall": round(wall / len(views) * 1e3, 3),
+                                             "stages": round(sum(ms[s_] for s_ in ("count", "scan", "raster", "resolve"))
+                                                             / len(views), 3)}
     log("shared scene: batched FP32 and FP64")
     api.execute_batch(imgs, precision=api.FP32)
     res["batched_fp32_fps"] = round(len(views) / _timed(lambda: api.execute_batch(imgs, precision=api.FP32), reps), 2)

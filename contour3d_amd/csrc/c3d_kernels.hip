@@ -1095,7 +1095,7 @@ __device__ __forceinline__ void drain_ring(FusedLds& S, Frag* s_frag, uint32_t h
 // each cell's thread as bytes (kStoreCode 1) or from LDS as 4-byte words
 // (kStoreCode 2; measured faster in the FP64 emit, slower in the FP32 one).
 // Expects S.ntri loaded and a barrier since.
-template <bool kStoreCell, int kStoreCode, typename Drain>
+template <bool kStoreCell, int kStoreCode, bool kReload, typename Drain>
 __device__ __forceinline__ void ring_tiles(FusedLds& S, const TessJob* __restrict__ jobs, int njobs, uint32_t nitems,
                                            const uint64_t* __restrict__ start, uint32_t* __restrict__ cellout,
                                            uint64_t cap, uint8_t* __restrict__ codeout, Drain&& drain) {
@@ -1114,56 +1114,66 @@ __device__ __forceinline__ void ring_tiles(FusedLds& S, const TessJob* __restric
       npend -= n;
     }
     cur = jb;
-    const FieldDev f = jobs[jb].f;
-    const ThrSet ts = jobs[jb].ts;
+    // The pass's field and thresholds are read through jbx, which with
+    // kReload is made opaque after every drain: the compiler then reloads
+    // them (scalar loads from the constant cache) instead of holding them in
+    // scalar registers across the drain, where the fused kernel's image
+    // constants need those registers (it had spilled scalars to vector lanes
+    // at twice the rate; the frames got 2-3.5% faster, the emit, whose
+    // drains hold no image constants, 3% slower).
+    uint32_t jbx = jb;
+    const auto Fd = [&]() -> const FieldDev& { return jobs[jbx].f; };
+    const auto Ts = [&]() -> const ThrSet& { return jobs[jbx].ts; };
     uint32_t col, k0, k1;
-    item_range(f, item, col, k0, k1);
-    const TileSpan t = tile_span(f, k0, col);
-    const uint32_t nch = (t.n + kThreads - 1) / kThreads;
-    const uint64_t nxy = uint64_t(f.nx) * f.ny;
-    const uint32_t ncxy = f.ncx * f.ncy;
+    item_range(Fd(), item, col, k0, k1);
+    // the tile's span and plane strides, recomputed from the pass (and so
+    // after a drain) rather than held
+    const auto Tsp = [&]() { return tile_span(Fd(), k0, col); };
+    const uint32_t nch = (Tsp().n + kThreads - 1) / kThreads;
+    const auto nxy = [&]() { return uint64_t(Fd().nx) * Fd().ny; };
+    const auto ncxy = [&]() { return Fd().ncx * Fd().ncy; };
     // codes of the first plane's corners (each thread reads back only what
     // it wrote: no barrier)
     for (uint32_t ch = 0; ch < nch; ch++) {
       const uint32_t p = ch * kThreads + threadIdx.x;
-      if (p < t.n) {
+      if (p < Tsp().n) {
         double v[4];
-        load_plane4(gptr(f.f) + t.l0 + tile_corner_off(f, t, p), f.nx, v);
-        S.low[ch][threadIdx.x] = uint16_t(plane_codes(ts, v));
+        load_plane4(gptr(Fd().f) + Tsp().l0 + tile_corner_off(Fd(), Tsp(), p), Fd().nx, v);
+        S.low[ch][threadIdx.x] = uint16_t(plane_codes(Ts(), v));
       }
     }
     for (uint32_t k = k0; k < k1; k++) {
-      const uint32_t tile = k * f.tg.cols + col;
-      const uint32_t ctile = t.c0 + (k - k0) * ncxy;
-      const C3D_GLOBAL double* up = gptr(f.f) + t.l0 + uint64_t(k - k0 + 1) * nxy;
+      const uint32_t tile = k * Fd().tg.cols + col;
+      const uint32_t ctile = Tsp().c0 + (k - k0) * ncxy();
+      const C3D_GLOBAL double* up = gptr(Fd().f) + Tsp().l0 + uint64_t(k - k0 + 1) * nxy();
       uint64_t base[kMaxThr];
 #pragma unroll
-      for (int q = 0; q < kMaxThr; q++) base[q] = q < ts.n ? start[ts.boff[q] + tile] : 0;
+      for (int q = 0; q < kMaxThr; q++) base[q] = q < Ts().n ? start[Ts().boff[q] + tile] : 0;
       for (uint32_t ch = 0; ch < nch; ch++) {
         const uint32_t c0 = ctile + ch * kThreads;
         const uint32_t p = ch * kThreads + threadIdx.x;
-        const bool valid = p < t.n;
+        const bool valid = p < Tsp().n;
         uint32_t code[kMaxThr];
         uint64_t packed = 0;
 #pragma unroll
         for (int q = 0; q < kMaxThr; q++) code[q] = 0;
         if (valid) {
           double v[4];
-          load_plane4(up + tile_corner_off(f, t, p), f.nx, v);
-          const uint32_t hi = plane_codes(ts, v);
+          load_plane4(up + tile_corner_off(Fd(), Tsp(), p), Fd().nx, v);
+          const uint32_t hi = plane_codes(Ts(), v);
           const uint32_t lo = S.low[ch][threadIdx.x];
           S.low[ch][threadIdx.x] = uint16_t(hi);
 #pragma unroll
           for (int q = 0; q < kMaxThr; q++) {
             const uint32_t cd = ((lo >> (4 * q)) & 15u) | ((hi >> (4 * q)) & 15u) << 4;
-            code[q] = q < ts.n ? cd : 0u;
+            code[q] = q < Ts().n ? cd : 0u;
             packed |= uint64_t(S.ntri[code[q]]) << (16 * q);
           }
           if (kStoreCode == 1) {
             uint8_t* co = codeout + jobs[jb].code_off + (c0 + threadIdx.x);
 #pragma unroll
             for (int q = 0; q < kMaxThr; q++)
-              if (q < ts.n) co[uint64_t(q) * f.ncells] = uint8_t(code[q]);
+              if (q < Ts().n) co[uint64_t(q) * Fd().ncells] = uint8_t(code[q]);
           }
         }
         uint64_t tot;
@@ -1182,11 +1192,11 @@ __device__ __forceinline__ void ring_tiles(FusedLds& S, const TessJob* __restric
           // code array: whole 4-byte words inside that range are stored as
           // words, the bytes of the (at most two) words it shares with the
           // neighbouring chunks one by one
-          const uint32_t nv = t.n - ch * kThreads < kThreads ? t.n - ch * kThreads : kThreads;
+          const uint32_t nv = Tsp().n - ch * kThreads < kThreads ? Tsp().n - ch * kThreads : kThreads;
           const uint32_t nw = (kThreads + 3u) / 4u + 1u;  // words touched by 256 bytes at any alignment
-          for (uint32_t w = threadIdx.x; w < uint32_t(ts.n) * nw; w += kThreads) {
+          for (uint32_t w = threadIdx.x; w < uint32_t(Ts().n) * nw; w += kThreads) {
             const uint32_t q = w / nw, kw = w - q * nw;
-            const uint64_t d0 = jobs[jb].code_off + uint64_t(q) * f.ncells + c0;
+            const uint64_t d0 = jobs[jb].code_off + uint64_t(q) * Fd().ncells + c0;
             const uint64_t a = (d0 & ~uint64_t(3)) + 4u * kw;
             const uint64_t lo = a > d0 ? a : d0, hi = a + 4u < d0 + nv ? a + 4u : d0 + nv;
             if (lo >= hi) continue;
@@ -1236,6 +1246,7 @@ __device__ __forceinline__ void ring_tiles(FusedLds& S, const TessJob* __restric
           __syncthreads();
           while (npend >= kThreads) {
             drain(head, kThreads, jb);
+            if constexpr (kReload) asm volatile("" : "+s"(jbx));
             head = (head + kThreads) & (kRing - 1u);
             npend -= kThreads;
           }
@@ -1271,7 +1282,7 @@ k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t nitems, cons
   __syncthreads();
   // the first pass of a frame (Append in FP64, the packed-key z-test in
   // FP32) stores the cell indices
-  ring_tiles<MODE == kRasterAppend || MODE == kRasterFp32, 0>(S, jobs, njobs, nitems, start, cellout, cap,
+  ring_tiles<MODE == kRasterAppend || MODE == kRasterFp32, 0, true>(S, jobs, njobs, nitems, start, cellout, cap,
                                                                    nullptr, [&](uint32_t head, uint32_t n, uint32_t jb) {
                                       drain_ring<R, MODE>(S, s_frag, head, n, jobs, jb, rjobs, jimg, hdr, key, id,
                                                           frags, fcap);
@@ -1350,7 +1361,7 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TessJob* __restrict__ j
   __shared__ MeshRec<R> s_rec[kThreads];
   S.ntri[threadIdx.x] = c_ntri[threadIdx.x];
   __syncthreads();
-  ring_tiles<false, sizeof(R) == 8 ? 2 : 1>(S, jobs, njobs, nitems, start, nullptr, 0, m.code,
+  ring_tiles<false, sizeof(R) == 8 ? 2 : 1, false>(S, jobs, njobs, nitems, start, nullptr, 0, m.code,
                                             [&](uint32_t head, uint32_t n, uint32_t jb) {
     emit_ring<R>(S, s_rec, head, n, jobs, jb, m, hdr);
   });
