@@ -185,7 +185,8 @@ struct Prof {
 struct contour3d_amd_mesh {
   int precision = 0;
   int precision_built = -1;  // form the buffers were sized for
-  c3d::DevBuf rec;     // a stored mesh: cap triangle records (c3d::MeshRec)
+  c3d::DevBuf pos;     // a stored mesh: cap position records (c3d::MeshPos)
+  c3d::DevBuf aux;     // and cap gather records (c3d::MeshAux)
   c3d::DevBuf cell;    // a frame: the cell index of each triangle (cap entries)
   c3d::DevBuf bstart;  // per-tile triangle starts of every object's field
   c3d::DevBuf meta;    // ObjMeta per object
@@ -203,7 +204,8 @@ struct contour3d_amd_mesh {
   std::vector<c3d::TileGeo> geo;  // and their layout
   std::vector<contour3d_color_t> color;
   void release() {
-    rec.release();
+    pos.release();
+    aux.release();
     cell.release();
     bstart.release();
     meta.release();
@@ -402,7 +404,8 @@ int carve_scratch(Frame& F, int device) {
   region(F.grids, kScratchGridValues * sizeof(double));
   region(F.rgb, 0);
   region(F.depth, 0);
-  region(M.rec, 0);
+  region(M.pos, 0);
+  region(M.aux, 0);
   if (off > kScratchBytes) {
     log_error("scratch layout of %zu bytes exceeds the %zu-byte array", off, kScratchBytes);
     return 1;
@@ -681,7 +684,8 @@ int validate_obj(const contour3d_contour_obj_t& o, size_t n) {
 template <typename R>
 MeshView<R> view_of(const contour3d_amd_mesh& M) {
   MeshView<R> m;
-  m.rec = M.rec.as<MeshRec<R>>();
+  m.pos = M.pos.as<MeshPos<R>>();
+  m.aux = M.aux.as<MeshAux<R>>();
   m.cap = M.cap;
   m.code = M.code.as<uint8_t>();
   return m;
@@ -868,10 +872,12 @@ int ensure_capacity(contour3d_amd_mesh& M) {
   if (M.cap >= M.total && M.cap != 0 && M.precision_built == M.precision) return 0;
   M.precision_built = M.precision;
   const uint64_t cap = M.total + M.total / 4 + 1024;
-  M.rec.release();
+  M.pos.release();
+  M.aux.release();
   M.cap = 0;
   // the capacity already holds a quarter of headroom
-  C3D_HIP(M.rec.ensure(cap * sizeof(MeshRec<R>), true));
+  C3D_HIP(M.pos.ensure(cap * sizeof(MeshPos<R>), true));
+  C3D_HIP(M.aux.ensure(cap * sizeof(MeshAux<R>), true));
   M.cap = cap;
   return 0;
 }
@@ -2408,8 +2414,12 @@ int contour3d_amd_mesh_copy(const contour3d_amd_mesh_t* mesh, size_t obj, double
   std::lock_guard<std::mutex> g(L.mu);
   const uint64_t n = mesh->count[obj], b = mesh->base[obj];
   if (n == 0) return 0;
-  const size_t rs = mesh->precision == CONTOUR3D_AMD_FP64 ? 8 : 4;
-  const size_t RS = mesh->precision == CONTOUR3D_AMD_FP64 ? sizeof(c3d::MeshRec<double>) : sizeof(c3d::MeshRec<float>);
+  const bool fp64 = mesh->precision == CONTOUR3D_AMD_FP64;
+  const size_t rs = fp64 ? 8 : 4;
+  // record sizes and the byte offset of the edge word in its record (the
+  // gather record in FP64, the position record in FP32)
+  const size_t PS = fp64 ? sizeof(c3d::MeshPos<double>) : sizeof(c3d::MeshPos<float>);
+  const size_t AS = fp64 ? sizeof(c3d::MeshAux<double>) : sizeof(c3d::MeshAux<float>);
   auto real_at = [&](const unsigned char* q) {
     if (rs == 8) {
       double x;
@@ -2421,23 +2431,25 @@ int contour3d_amd_mesh_copy(const contour3d_amd_mesh_t* mesh, size_t obj, double
     return double(y);
   };
   // the object's records, unpacked field by field
-  std::vector<unsigned char> recs;
-  if (pos || wfn || edges || cell) {
-    recs.resize(n * RS);
-    if (hipMemcpy(recs.data(), static_cast<const unsigned char*>(mesh->rec.p) + b * RS, n * RS,
-                  hipMemcpyDeviceToHost) != hipSuccess) {
-      c3d::log_error("mesh_copy: device copy failed");
-      return 1;
-    }
+  std::vector<unsigned char> prec, arec;
+  if (pos || edges) prec.resize(n * PS);
+  if (wfn || cell || edges) arec.resize(n * AS);
+  if ((!prec.empty() && hipMemcpy(prec.data(), static_cast<const unsigned char*>(mesh->pos.p) + b * PS, n * PS,
+                                  hipMemcpyDeviceToHost) != hipSuccess) ||
+      (!arec.empty() && hipMemcpy(arec.data(), static_cast<const unsigned char*>(mesh->aux.p) + b * AS, n * AS,
+                                  hipMemcpyDeviceToHost) != hipSuccess)) {
+    c3d::log_error("mesh_copy: device copy failed");
+    return 1;
   }
-  for (uint64_t t = 0; t < (recs.empty() ? 0 : n); t++) {
-    const unsigned char* r = recs.data() + t * RS;
+  for (uint64_t t = 0; t < n; t++) {
+    const unsigned char* rp = prec.empty() ? nullptr : prec.data() + t * PS;
+    const unsigned char* ra = arec.empty() ? nullptr : arec.data() + t * AS;
     if (pos)
-      for (int c = 0; c < 9; c++) pos[9 * t + c] = real_at(r + c * rs);
+      for (int c = 0; c < 9; c++) pos[9 * t + c] = real_at(rp + c * rs);
     if (wfn)
-      for (int c = 0; c < 3; c++) wfn[3 * t + c] = real_at(r + (9 + c) * rs);
-    if (cell) std::memcpy(cell + t, r + 12 * rs, 4);
-    if (edges) std::memcpy(edges + t, r + 12 * rs + 4, 4);
+      for (int c = 0; c < 3; c++) wfn[3 * t + c] = real_at(ra + c * rs);
+    if (cell) std::memcpy(cell + t, ra + 3 * rs, 4);
+    if (edges) std::memcpy(edges + t, fp64 ? ra + 3 * rs + 4 : rp + 9 * rs, 4);
   }
   if (vn) {
     // the mesh keeps no per-triangle vertex normals; they are gathered now

@@ -58,23 +58,51 @@ struct TessJob {
   uint64_t code_off;  // a stored mesh: threshold q's cell codes at code_off + q * f.ncells
 };
 
-// One stored triangle: 56 bytes in the FP32 form (R = float), 104 in the
-// FP64 form.  Records are stored contiguously in serial order, so a
-// tessellation drain writes whole runs of lines.
+// One stored triangle, as two records in two arrays, both in serial order:
+// its positions, and what the smoothing gathers from neighbouring
+// triangles (the weight and the cell), so that those gathers read 32-byte
+// (FP64) or 16-byte (FP32) records.  The edge word sits where it keeps both
+// records a multiple of 8 bytes (a drain stores them as 8-byte words):
+//   FP64: MeshPos 72 B {p[9]},        MeshAux 32 B {w[3], cell, edges}
+//   FP32: MeshPos 40 B {p[9], edges}, MeshAux 16 B {w[3], cell}
+// p: vertex v, axis d at 3v + d; w: area * unit face normal, the smoothing
+// weight; cell: linear cell index within its field; edges: three 5-bit
+// cell-local lattice edge ids | interior << 15.
 template <typename R>
-struct alignas(8) MeshRec {
-  R p[9];          // vertex v, axis d at 3v + d
-  R w[3];          // area * unit face normal, the smoothing weight
-  uint32_t cell;   // linear cell index within its field
-  uint32_t edges;  // three 5-bit cell-local lattice edge ids | interior << 15
+struct MeshPos;
+template <>
+struct alignas(8) MeshPos<double> {
+  double p[9];
 };
-static_assert(sizeof(MeshRec<float>) == 56 && sizeof(MeshRec<double>) == 104, "record layout");
+template <>
+struct alignas(8) MeshPos<float> {
+  float p[9];
+  uint32_t edges;
+};
+template <typename R>
+struct MeshAux;
+template <>
+struct alignas(8) MeshAux<double> {
+  double w[3];
+  uint32_t cell;
+  uint32_t edges;
+};
+template <>
+struct alignas(8) MeshAux<float> {
+  float w[3];
+  uint32_t cell;
+};
+static_assert(sizeof(MeshPos<double>) == 72 && sizeof(MeshAux<double>) == 32 && sizeof(MeshPos<float>) == 40 &&
+                  sizeof(MeshAux<float>) == 16,
+              "record layout");
 
-// Triangle storage: cap records, and the corner code (8 bits) of every
-// cell of every object's field, object after object (ObjMeta::code_off).
+// Triangle storage: cap records of each kind, and the corner code (8 bits)
+// of every cell of every object's field, object after object
+// (ObjMeta::code_off).
 template <typename R>
 struct MeshView {
-  MeshRec<R>* rec;
+  MeshPos<R>* pos;
+  MeshAux<R>* aux;
   uint64_t cap;
   uint8_t* code;
 };

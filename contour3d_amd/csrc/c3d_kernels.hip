@@ -510,12 +510,20 @@ __constant__ NbrTable c_nbr;
 
 
 
+// the edge word of stored triangle u (in the gather record in FP64, in the
+// position record in FP32)
 template <typename R>
-__device__ __forceinline__ uint64_t lower_bound_cell(const MeshRec<R>* __restrict__ rec, uint64_t lo, uint64_t hi,
+__device__ __forceinline__ uint32_t edges_of(const MeshView<R>& m, uint64_t u) {
+  if constexpr (sizeof(R) == 8) return m.aux[u].edges;
+  else return m.pos[u].edges;
+}
+
+template <typename R>
+__device__ __forceinline__ uint64_t lower_bound_cell(const MeshAux<R>* __restrict__ aux, uint64_t lo, uint64_t hi,
                                                      uint32_t key) {
   while (lo < hi) {
     const uint64_t mid = (lo + hi) >> 1;
-    if (rec[mid].cell < key) lo = mid + 1;
+    if (aux[mid].cell < key) lo = mid + 1;
     else hi = mid;
   }
   return lo;
@@ -538,10 +546,10 @@ __device__ V3<R> vertex_normal(const MeshView<R>& m, const ObjMeta& om, uint32_t
     const uint32_t c2 = uint32_t(int64_t(c) + dk * sz + dj * sy + di);
     const uint32_t tl = tile_of_cell(om, c2);
     const uint64_t hi = om.bstart[tl + 1];
-    for (uint64_t u = lower_bound_cell(m.rec, om.bstart[tl], hi, c2); u < hi && m.rec[u].cell == c2; u++) {
-      const uint32_t eu = m.rec[u].edges;
+    for (uint64_t u = lower_bound_cell(m.aux, om.bstart[tl], hi, c2); u < hi && m.aux[u].cell == c2; u++) {
+      const uint32_t eu = edges_of(m, u);
       if ((eu & 31u) == want || ((eu >> 5) & 31u) == want || ((eu >> 10) & 31u) == want)
-        sum = vadd(sum, V3<R>{m.rec[u].w[0], m.rec[u].w[1], m.rec[u].w[2]});
+        sum = vadd(sum, V3<R>{m.aux[u].w[0], m.aux[u].w[1], m.aux[u].w[2]});
     }
   }
   return vunit(sum);
@@ -589,7 +597,8 @@ __global__ __launch_bounds__(kThreads) void k_cell_starts(const uint8_t* __restr
 // (Measured against this: one 32-bit cell word of code and start, the masks
 // in LDS and a persistent grid were 5% slower; a workgroup per tile, tiles
 // z-fastest along each column and one contiguous run per XCD (to keep the
-// planes above and below in L2), 1.55x slower; a workgroup per tile
+// planes above and below in L2), 1.55-1.66x slower: the serial order streams
+// the records and the normals in address order; a workgroup per tile
 // gathering a window of cells into LDS 3-4x slower.)
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_smooth(MeshView<R> m, const ObjMeta* __restrict__ meta,
@@ -604,7 +613,7 @@ __global__ __launch_bounds__(kThreads) void k_smooth(MeshView<R> m, const ObjMet
     const uint64_t t = it / 3u;
     const uint32_t vv = uint32_t(it - 3u * t);
     const uint64_t u = om.base + t;
-    const uint32_t c = m.rec[u].cell, ed = m.rec[u].edges;
+    const uint32_t c = m.aux[u].cell, ed = edges_of(m, u);
     R* o = out + 9u * t + 3u * vv;
     if (!((ed >> 15) & 1u)) {
       o[0] = nan;
@@ -632,7 +641,7 @@ __global__ __launch_bounds__(kThreads) void k_smooth(MeshView<R> m, const ObjMet
     for (int q = 0; q < 4; q++) {
       for (uint32_t h = mask[q]; h; h &= h - 1u) {
         const uint64_t u2 = s2[q] + uint32_t(__builtin_ctz(h));
-        if (u2 < m.cap) sum = vadd(sum, V3<R>{m.rec[u2].w[0], m.rec[u2].w[1], m.rec[u2].w[2]});
+        if (u2 < m.cap) sum = vadd(sum, V3<R>{m.aux[u2].w[0], m.aux[u2].w[1], m.aux[u2].w[2]});
       }
     }
     const V3<R> nv = vunit(sum);
@@ -659,18 +668,18 @@ __global__ __launch_bounds__(kThreads) void k_mesh_compare(MeshView<RA> a, const
   unsigned long long asum = 0, nv = 0, bad = 0, over1 = 0, over10 = 0;
   for (uint64_t t = uint64_t(blockIdx.x) * kThreads + threadIdx.x; t < count; t += uint64_t(gridDim.x) * kThreads) {
     const uint64_t ga = oa.base + t, gb = ob.base + t;
-    const uint32_t ea = a.rec[ga].edges, eb = b.rec[gb].edges;
-    if (ea != eb || a.rec[ga].cell != b.rec[gb].cell) {
+    const uint32_t ea = edges_of(a, ga), eb = edges_of(b, gb);
+    if (ea != eb || a.aux[ga].cell != b.aux[gb].cell) {
       bad++;
       continue;
     }
 #pragma unroll
-    for (int c = 0; c < 9; c++) verr = fmax(verr, fabs(double(a.rec[ga].p[c]) - double(b.rec[gb].p[c])));
+    for (int c = 0; c < 9; c++) verr = fmax(verr, fabs(double(a.pos[ga].p[c]) - double(b.pos[gb].p[c])));
     if (!((ea >> 15) & 1u)) continue;
     for (int v = 0; v < 3; v++) {
       const uint32_t e = (ea >> (5 * v)) & 31u;
-      const V3<RA> na = vertex_normal<RA>(a, oa, a.rec[ga].cell, e);
-      const V3<RB> nb = vertex_normal<RB>(b, ob, b.rec[gb].cell, e);
+      const V3<RA> na = vertex_normal<RA>(a, oa, a.aux[ga].cell, e);
+      const V3<RB> nb = vertex_normal<RB>(b, ob, b.aux[gb].cell, e);
       const double d = double(na.x) * double(nb.x) + double(na.y) * double(nb.y) + double(na.z) * double(nb.z);
       if (!(d == d) || !isfinite(d)) continue;
       const double ang = acos(fmin(1.0, fmax(-1.0, d))) * (180.0 / 3.14159265358979323846);
@@ -720,7 +729,7 @@ __device__ __forceinline__ bool setup_tri(const MeshView<R>& m, uint64_t t, cons
                                           R& area) {
 #pragma unroll
   for (int n = 0; n < 3; n++) {
-    const V3<R> p = {m.rec[t].p[3 * n + 0], m.rec[t].p[3 * n + 1], m.rec[t].p[3 * n + 2]};
+    const V3<R> p = {m.pos[t].p[3 * n + 0], m.pos[t].p[3 * n + 1], m.pos[t].p[3 * n + 2]};
     if (!project(v, p, P.q[n])) return false;
   }
   area = raster_box(v, P);
@@ -885,9 +894,9 @@ __device__ __forceinline__ void fetch_tri(const MeshView<R>& m, const RasterJob<
   T.ok = T.t < rjobs[T.jb].tri1;
   T.ed = 0;
   if (T.ok) {
-    T.ed = m.rec[T.t].edges;
+    T.ed = edges_of(m, T.t);
 #pragma unroll
-    for (int n = 0; n < 9; n++) T.p[n] = m.rec[T.t].p[n];
+    for (int n = 0; n < 9; n++) T.p[n] = m.pos[T.t].p[n];
   }
 }
 
@@ -1294,23 +1303,41 @@ k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t nitems, cons
 // tessellation: emit pass (the stored mesh)
 //
 // The work-item loop of the queue kernels; every drain computes up to 256
-// triangles, one per thread, into records in LDS, and the workgroup then
-// stores those records at their serial positions as 8-byte words, thread q
-// taking words q, q + 256, ...: the drain's triangles are a few runs of
-// consecutive positions (one per object in the drained chunks), so every
-// store instruction writes contiguous bytes and whole lines, where one
-// store per record field (14 streams) had written scattered 4- or 8-byte
-// pieces.  The stores are non-temporal: the mesh streams past the caches,
-// and the field planes the workgroups march through stay in L2.
+// triangles, one per thread, into records in LDS (a position record and a
+// gather record each), and the workgroup then stores those records at
+// their serial positions as 8-byte words, thread q taking words q, q + 256,
+// ... of each array: the drain's triangles are a few runs of consecutive
+// positions (one per object in the drained chunks), so every store
+// instruction writes contiguous bytes and whole lines, where one store per
+// record field (14 streams) had written scattered 4- or 8-byte pieces.  The
+// stores are non-temporal: the mesh streams past the caches, and the field
+// planes the workgroups march through stay in L2.
 //
 // Every earlier vector memory operation is waited for once before the
 // drain's stores, so that nothing is in flight when they issue: the
 // compiler then issues the store sequence back to back, where otherwise it
 // waited between stores for loads still counted ahead of them.
+
+// stores n records of type Rec from LDS at the drain's serial positions
+template <typename Rec>
+__device__ __forceinline__ void store_drain(const FusedLds& S, const Rec* s_src, uint32_t head, uint32_t n,
+                                            Rec* out, uint64_t cap) {
+  constexpr uint32_t kWords = sizeof(Rec) / 8u;
+  const uint64_t* src = reinterpret_cast<const uint64_t*>(s_src);
+  char* dst = reinterpret_cast<char*>(out);
+  for (uint32_t k = threadIdx.x; k < n * kWords; k += kThreads) {
+    const uint32_t r = k / kWords;
+    const uint32_t g = S.rg[(head + r) & (kRing - 1u)];
+    if (g < cap)
+      __builtin_nontemporal_store(src[k], reinterpret_cast<uint64_t*>(dst + uint64_t(g) * sizeof(Rec) +
+                                                                     8u * (k - r * kWords)));
+  }
+}
+
 template <typename R>
-__device__ __forceinline__ void emit_ring(FusedLds& S, MeshRec<R>* s_rec, uint32_t head, uint32_t n,
-                                          const TessJob* __restrict__ jobs, uint32_t jb, const MeshView<R>& m,
-                                          FrameHdr* __restrict__ hdr) {
+__device__ __forceinline__ void emit_ring(FusedLds& S, MeshPos<R>* s_pos, MeshAux<R>* s_aux, uint32_t head,
+                                          uint32_t n, const TessJob* __restrict__ jobs, uint32_t jb,
+                                          const MeshView<R>& m, FrameHdr* __restrict__ hdr) {
   if (threadIdx.x < n) {
     const FieldDev f = jobs[jb].f;
     const uint32_t s = (head + threadIdx.x) & (kRing - 1u);
@@ -1324,32 +1351,27 @@ __device__ __forceinline__ void emit_ring(FusedLds& S, MeshRec<R>* s_rec, uint32
     slot_positions_mem<R>(f, L, thr, x & 0x3ffffu, P);
     const V3<R> w = smoothing_weight<R>(P);
     const uint32_t ent_edges = uint32_t(c_cslot[(x >> 24) * 12u + ((x >> 20) & 15u)] >> 18);
-    MeshRec<R>& r = s_rec[threadIdx.x];
+    const uint32_t edges = ent_edges | uint32_t(interior_cell(f, L)) << 15;
+    MeshPos<R>& rp = s_pos[threadIdx.x];
+    MeshAux<R>& ra = s_aux[threadIdx.x];
 #pragma unroll
     for (int vv = 0; vv < 3; vv++) {
-      r.p[3 * vv + 0] = P[vv].x;
-      r.p[3 * vv + 1] = P[vv].y;
-      r.p[3 * vv + 2] = P[vv].z;
+      rp.p[3 * vv + 0] = P[vv].x;
+      rp.p[3 * vv + 1] = P[vv].y;
+      rp.p[3 * vv + 2] = P[vv].z;
     }
-    r.w[0] = w.x;
-    r.w[1] = w.y;
-    r.w[2] = w.z;
-    r.cell = cc;
-    r.edges = ent_edges | uint32_t(interior_cell(f, L)) << 15;
+    ra.w[0] = w.x;
+    ra.w[1] = w.y;
+    ra.w[2] = w.z;
+    ra.cell = cc;
+    if constexpr (sizeof(R) == 8) ra.edges = edges;
+    else rp.edges = edges;
     if (uint64_t(S.rg[s]) >= m.cap) hdr->overflow = 1u;  // capacity reserved ahead of a replayed frame was too small
   }
   __syncthreads();
   __builtin_amdgcn_s_waitcnt(0);
-  constexpr uint32_t kWords = sizeof(MeshRec<R>) / 8u;
-  const uint64_t* src = reinterpret_cast<const uint64_t*>(s_rec);
-  char* dst = reinterpret_cast<char*>(m.rec);
-  for (uint32_t k = threadIdx.x; k < n * kWords; k += kThreads) {
-    const uint32_t r = k / kWords;
-    const uint32_t g = S.rg[(head + r) & (kRing - 1u)];
-    if (g < m.cap)
-      __builtin_nontemporal_store(src[k], reinterpret_cast<uint64_t*>(dst + uint64_t(g) * sizeof(MeshRec<R>) +
-                                                                     8u * (k - r * kWords)));
-  }
+  store_drain(S, s_pos, head, n, m.pos, m.cap);
+  store_drain(S, s_aux, head, n, m.aux, m.cap);
   __syncthreads();
 }
 
@@ -1358,12 +1380,13 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TessJob* __restrict__ j
                                                    const uint64_t* __restrict__ start, MeshView<R> m,
                                                    FrameHdr* __restrict__ hdr) {
   __shared__ FusedLds S;
-  __shared__ MeshRec<R> s_rec[kThreads];
+  __shared__ MeshPos<R> s_pos[kThreads];
+  __shared__ MeshAux<R> s_aux[kThreads];
   S.ntri[threadIdx.x] = c_ntri[threadIdx.x];
   __syncthreads();
   ring_tiles<false, sizeof(R) == 8 ? 2 : 1, false>(S, jobs, njobs, nitems, start, nullptr, 0, m.code,
-                                            [&](uint32_t head, uint32_t n, uint32_t jb) {
-    emit_ring<R>(S, s_rec, head, n, jobs, jb, m, hdr);
+                                                   [&](uint32_t head, uint32_t n, uint32_t jb) {
+    emit_ring<R>(S, s_pos, s_aux, head, n, jobs, jb, m, hdr);
   });
 }
 
@@ -1487,8 +1510,8 @@ __global__ __launch_bounds__(kThreads) void k_resolve(MeshView<R> m, const Raste
       setup_tri(m, t, v, P, area);
       R w0, w1, w2;
       pixel_weights(P, pixel_centre<R>(i, v.fw), pixel_centre<R>(j, v.fh), w0, w1, w2);
-      const uint32_t c = m.rec[t].cell;
-      const uint32_t ed = m.rec[t].edges;
+      const uint32_t c = m.aux[t].cell;
+      const uint32_t ed = edges_of(m, t);
       V3<R> nv[3];
 #pragma unroll 1
       for (int n = 0; n < 3; n++) nv[n] = vertex_normal<R>(m, om, c, (ed >> (5 * n)) & 31u);
