@@ -271,7 +271,11 @@ def shared_scene(api, fields, contours, reps):
     for name, prec in (("fp64", api.FP64), ("fp32", api.FP32)):
         log(f"shared scene: plain {name}")
         plain(prec)
-        res[f"plain_{name}_fps"] = round(len(views) / _timed(lambda: plain(prec), reps), 2)
+        rep_s = []
+        for _ in range(reps):
+            rep_s.append(_timed(lambda: plain(prec), 1))
+        res[f"plain_{name}_fps"] = round(len(views) * reps / sum(rep_s), 2)
+        res[f"plain_{name}_rep_ms_per_frame"] = [round(t / len(views) * 1e3, 3) for t in rep_s]
         # the same loop once more with the stage timers on: the device time
         # of the frames' stages beside the wall time per frame, so that a
         # slow loop shows whether the kernels or the gaps between them grew

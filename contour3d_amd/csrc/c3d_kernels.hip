@@ -636,10 +636,25 @@ __global__ __launch_bounds__(kThreads) void k_smooth(MeshView<R> m, const ObjMet
         s2[q] = om.bstart[tile_of_cell(om, c2)] + cs[c2];
       }
     }
+    // each neighbour's first four hits are loaded together, then added in
+    // slot order (more hits, rare, follow one at a time)
     V3<R> sum = {R(0), R(0), R(0)};
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      for (uint32_t h = mask[q]; h; h &= h - 1u) {
+      uint32_t h = mask[q];
+      V3<R> wv[4];
+      bool hv[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const uint64_t u2 = s2[q] + uint32_t(__builtin_ctz(h | 0x80000000u));
+        hv[r] = h != 0 && u2 < m.cap;
+        wv[r] = hv[r] ? V3<R>{m.aux[u2].w[0], m.aux[u2].w[1], m.aux[u2].w[2]} : V3<R>{R(0), R(0), R(0)};
+        h &= h - 1u;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (hv[r]) sum = vadd(sum, wv[r]);
+      for (; h; h &= h - 1u) {
         const uint64_t u2 = s2[q] + uint32_t(__builtin_ctz(h));
         if (u2 < m.cap) sum = vadd(sum, V3<R>{m.aux[u2].w[0], m.aux[u2].w[1], m.aux[u2].w[2]});
       }
