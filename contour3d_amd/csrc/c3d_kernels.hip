@@ -257,9 +257,91 @@ __device__ __forceinline__ bool xcd_item(const TessJob* __restrict__ jobs, int n
 // ----------------------------------------------------------------------
 // tessellation: count pass
 //
-// Per work item, each thread keeps its cells' lower-plane codes (4 bits per
-// threshold and chunk) in registers and loads only the four upper corners of
-// each cell per plane: the field is read about once.
+// Per work item, each thread takes column segments of R rows of the tile
+// (R = 4, 2 or 1, the largest that divides the tile's rows): a segment's
+// R cells share their rows of lattice points, so a plane costs 2 (R + 1)
+// point loads and threshold compares for R cells instead of 4 R.  The
+// cells' lower-plane codes (4 bits per threshold) stay in registers while
+// the item marches up its column, so the field is read about once.  Only
+// the tile totals are written, so the cells may be taken in any order.
+
+// the bits of one lattice point: thr_q < v at bit 4q
+__device__ __forceinline__ uint32_t point_bits(const ThrSet& ts, double v) {
+  uint32_t b = 0;
+#pragma unroll
+  for (int q = 0; q < kMaxThr; q++)
+    if (q < ts.n) b |= uint32_t(ts.thr[q] < v) << (4 * q);
+  return b;
+}
+
+// the plane codes (plane_codes order) of the R cells of a segment whose
+// top-left point is at p: points of rows 0..R, columns 0 and 1
+template <int R>
+__device__ __forceinline__ void segment_codes(const ThrSet& ts, const C3D_GLOBAL double* p, uint32_t nx,
+                                              uint32_t out[R]) {
+  double v[R + 1][2];
+#pragma unroll
+  for (int jj = 0; jj <= R; jj++) {
+    v[jj][0] = p[uint64_t(jj) * nx];
+    v[jj][1] = p[uint64_t(jj) * nx + 1];
+  }
+  uint32_t b0 = point_bits(ts, v[0][0]) | point_bits(ts, v[0][1]) << 1;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const uint32_t b1 = point_bits(ts, v[r + 1][0]) | point_bits(ts, v[r + 1][1]) << 1;
+    out[r] = b0 | b1 << 2;
+    b0 = b1;
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void count_item(const FieldDev& f, const ThrSet& ts, const TileSpan& t, uint32_t col,
+                                           uint32_t k0, uint32_t k1, const uint8_t* s_ntri, uint64_t* s_wave,
+                                           uint32_t* __restrict__ cnt) {
+  constexpr int M = kTileChunks / R;  // segments per thread at most (R * M = kTileCells / kThreads cells)
+  const uint32_t nseg = (t.rows / R) * t.w;
+  const uint64_t nxy = uint64_t(f.nx) * f.ny;
+  uint32_t off[M], low[M][R];
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const uint32_t sg = m * kThreads + threadIdx.x;
+    off[m] = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) low[m][r] = 0;
+    if (sg < nseg) {
+      // segment sg: rows R * (sg / w) .. + R - 1, column sg % w of the tile
+      const uint32_t rs = R == 1 && f.tg.segs != 1 ? 0u : fdiv(sg, f.div_ncx);
+      off[m] = rs * R * f.nx + (sg - rs * t.w);
+      segment_codes<R>(ts, gptr(f.f) + t.l0 + off[m], f.nx, low[m]);
+    }
+  }
+  for (uint32_t k = k0; k < k1; k++) {
+    const C3D_GLOBAL double* up = gptr(f.f) + t.l0 + uint64_t(k - k0 + 1) * nxy;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+      const uint32_t sg = m * kThreads + threadIdx.x;
+      if (sg < nseg) {
+        uint32_t hi[R];
+        segment_codes<R>(ts, up + off[m], f.nx, hi);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+#pragma unroll
+          for (int q = 0; q < kMaxThr; q++) {
+            const uint32_t cd = ((low[m][r] >> (4 * q)) & 15u) | ((hi[r] >> (4 * q)) & 15u) << 4;
+            acc += uint64_t(q < ts.n ? s_ntri[cd] : 0u) << (16 * q);
+          }
+          low[m][r] = hi[r];
+        }
+      }
+    }
+    uint64_t tot;
+    block_excl_scan64(acc, s_wave, tot);
+    const uint32_t tile = k * f.tg.cols + col;
+    if (threadIdx.x < uint32_t(ts.n)) cnt[ts.boff[threadIdx.x] + tile] = uint32_t(tot >> (16 * threadIdx.x)) & 0xffffu;
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void k_count(const TessJob* __restrict__ jobs, int njobs, uint32_t nitems,
                                                     uint32_t* __restrict__ cnt) {
   __shared__ uint64_t s_wave[kThreads / 64];
@@ -275,42 +357,9 @@ __global__ __launch_bounds__(kThreads) void k_count(const TessJob* __restrict__ 
     uint32_t col, k0, k1;
     item_range(f, item, col, k0, k1);
     const TileSpan t = tile_span(f, k0, col);
-    const uint64_t nxy = uint64_t(f.nx) * f.ny;
-    uint32_t off[kTileChunks], low[kTileChunks];
-#pragma unroll
-    for (int ch = 0; ch < kTileChunks; ch++) {
-      const uint32_t p = ch * kThreads + threadIdx.x;
-      off[ch] = p < t.n ? tile_corner_off(f, t, p) : 0u;
-      low[ch] = 0;
-      if (p < t.n) {
-        double v[4];
-        load_plane4(gptr(f.f) + t.l0 + off[ch], f.nx, v);
-        low[ch] = plane_codes(ts, v);
-      }
-    }
-    for (uint32_t k = k0; k < k1; k++) {
-      const C3D_GLOBAL double* up = gptr(f.f) + t.l0 + uint64_t(k - k0 + 1) * nxy;
-      uint64_t acc = 0;
-#pragma unroll
-      for (int ch = 0; ch < kTileChunks; ch++) {
-        const uint32_t p = ch * kThreads + threadIdx.x;
-        if (p < t.n) {
-          double v[4];
-          load_plane4(up + off[ch], f.nx, v);
-          const uint32_t hi = plane_codes(ts, v);
-#pragma unroll
-          for (int q = 0; q < kMaxThr; q++) {
-            const uint32_t cd = ((low[ch] >> (4 * q)) & 15u) | ((hi >> (4 * q)) & 15u) << 4;
-            acc += uint64_t(q < ts.n ? s_ntri[cd] : 0u) << (16 * q);
-          }
-          low[ch] = hi;
-        }
-      }
-      uint64_t tot;
-      block_excl_scan64(acc, s_wave, tot);
-      const uint32_t tile = k * f.tg.cols + col;
-      if (threadIdx.x < uint32_t(ts.n)) cnt[ts.boff[threadIdx.x] + tile] = uint32_t(tot >> (16 * threadIdx.x)) & 0xffffu;
-    }
+    if (t.rows % 4u == 0) count_item<4>(f, ts, t, col, k0, k1, s_ntri, s_wave, cnt);
+    else if (t.rows % 2u == 0) count_item<2>(f, ts, t, col, k0, k1, s_ntri, s_wave, cnt);
+    else count_item<1>(f, ts, t, col, k0, k1, s_ntri, s_wave, cnt);
   }
 }
 
