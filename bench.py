@@ -276,6 +276,10 @@ def shared_scene(api, fields, contours, reps):
             rep_s.append(_timed(lambda: plain(prec), 1))
         res[f"plain_{name}_fps"] = round(len(views) * reps / sum(rep_s), 2)
         res[f"plain_{name}_rep_ms_per_frame"] = [round(t / len(views) * 1e3, 3) for t in rep_s]
+        # the median round: an occasional one-off delay in a single round
+        # (seen in the first round after the 128^3 C1 frames, outside the
+        # frames' kernels) moves the mean, not the median
+        res[f"plain_{name}_fps_median_round"] = round(len(views) / float(np.median(rep_s)), 2)
         # the same loop once more with the stage timers on: the device time
         # of the frames' stages beside the wall time per frame, so that a
         # slow loop shows whether the kernels or the gaps between them grew
