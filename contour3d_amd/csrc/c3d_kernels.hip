@@ -1194,9 +1194,9 @@ __device__ __forceinline__ void drain_ring(FusedLds& S, Frag* s_frag, uint32_t h
 // each plane's upper codes are first computed for the whole tile by column
 // segments (classify_plane, into s_hi: the cells of a segment share their
 // rows of lattice points) and the chunks read them from LDS; the fused
-// kernel measured 3.6% (FP64) and 1.3% (FP32) faster so, the emit, whose
-// FP64 build then lost a workgroup per CU to the extra LDS, loads each
-// cell's corners itself.  With
+// kernel measured 3.6% (FP64) and 1.3% (FP32) faster so, the FP32 emit
+// 1.5%; the FP64 emit, which would lose a workgroup per CU to the extra
+// LDS, loads each cell's corners itself.  With
 // kStoreCell the cell index of each triangle is stored at its serial
 // position as it is queued; with kStoreCode every cell's corner code of
 // every threshold q is stored at codeout[code_off + q * ncells + cell]: by
@@ -1501,9 +1501,11 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TessJob* __restrict__ j
   __shared__ FusedLds S;
   __shared__ MeshPos<R> s_pos[kThreads];
   __shared__ MeshAux<R> s_aux[kThreads];
+  constexpr bool kPl = sizeof(R) == 4;  // the FP64 build would lose a workgroup per CU to the LDS
+  __shared__ uint16_t s_hi[kPl ? kTileCells : 1];
   S.ntri[threadIdx.x] = c_ntri[threadIdx.x];
   __syncthreads();
-  ring_tiles<false, sizeof(R) == 8 ? 2 : 1, false, false>(S, nullptr, jobs, njobs, nitems, start, nullptr, 0, m.code,
+  ring_tiles<false, sizeof(R) == 8 ? 2 : 1, false, kPl>(S, s_hi, jobs, njobs, nitems, start, nullptr, 0, m.code,
                                                           [&](uint32_t head, uint32_t n, uint32_t jb) {
     emit_ring<R>(S, s_pos, s_aux, head, n, jobs, jb, m, hdr);
   });

@@ -297,7 +297,7 @@ enum {
   CONTOUR3D_AMD_TUNE_TESS_ROWS = 5,     /* tessellation brick: rows of cells per tile, at most 2048 cells
                                            (0: as many rows as fit) */
   CONTOUR3D_AMD_TUNE_TESS_ZPLANES = 6   /* tessellation brick: z-planes one workgroup marches through per
-                                           work item (0: the default, 8) */
+                                           work item (0: the default, 4) */
 };
 
 extern int contour3d_amd_tune(int param, int64_t value);
