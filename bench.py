@@ -415,7 +415,7 @@ def uneven_batch_frame(api, fields, nfields, side, reps, seed=0):
             "gcells_per_s_graph": round(cells / graph / 1e9, 2),
             "autotune": {"best": tuned, "best_ms": round(best_ms, 3), "worst_ms": round(max(ms_all), 3),
                          "default_ms": round(next(m for k, m in sweep if k == {"tess_grid": 0, "tess_rows": 0,
-                                                                                "tess_zplanes": 8}), 3),
+                                                                                "tess_zplanes": 4}), 3),
                          "sweep": [dict(k, ms=round(m, 3)) for k, m in sweep]}}
 
 

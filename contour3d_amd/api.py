@@ -470,7 +470,7 @@ def tune(param, value):
 AUTOTUNE_KNOBS = {
     TUNE_TESS_GRID: (0, 16384, 1 << 30),   # persistent workgroups (0: 128 per CU), or one per work item
     TUNE_TESS_ROWS: (0, 2, 1),             # rows of cells per tile (0: as many as fit in 2048 cells)
-    TUNE_TESS_ZPLANES: (4, 8, 16, 32),     # z-planes per work item
+    TUNE_TESS_ZPLANES: (2, 4, 8, 16),      # z-planes per work item (4 by default)
 }
 KNOB_NAMES = {TUNE_TESS_GRID: "tess_grid", TUNE_TESS_ROWS: "tess_rows", TUNE_TESS_ZPLANES: "tess_zplanes"}
 

@@ -23,7 +23,7 @@
 namespace c3d {
 
 constexpr int kTileCells = 2048;     // cells of a tessellation tile at most
-constexpr int kDefaultZPlanes = 4;   // z-planes a work item marches by default (4, 8, 16 measured at 512^3: 4 best)
+constexpr int kDefaultZPlanes = 4;   // z-planes a work item marches by default (2-6, 8, 16 measured at 512^3: 4 best)
 
 // Division by a runtime constant with a multiply-high (Granlund-Montgomery,
 // "round-up" variant): q = (umulhi(n, m) + n) >> s for every 32-bit n.
