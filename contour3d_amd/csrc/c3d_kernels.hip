@@ -1469,7 +1469,7 @@ __device__ __forceinline__ void emit_ring(FusedLds& S, MeshPos<R>* s_pos, MeshAu
     V3<R> P[3];
     slot_positions_mem<R>(f, L, thr, x & 0x3ffffu, P);
     const V3<R> w = smoothing_weight<R>(P);
-    const uint32_t ent_edges = uint32_t(c_cslot[(x >> 24) * 12u + ((x >> 20) & 15u)] >> 18);
+    const uint32_t ent_edges = uint32_t(c_cslot[(x >> 24) * 12u + ((x >> 20) & 15u)] >> 18) & 0x7fffu;
     const uint32_t edges = ent_edges | uint32_t(interior_cell(f, L)) << 15;
     MeshPos<R>& rp = s_pos[threadIdx.x];
     MeshAux<R>& ra = s_aux[threadIdx.x];
