@@ -102,6 +102,30 @@ def profile_summary():
         return {}
 
 
+def copy_bandwidth(reps=5):
+    """HBM bytes per second of a plain device-to-device copy of 1 GiB (read
+    once, written once), the best of `reps` copies timed with HIP events:
+    the practical ceiling beside the nominal 8 TB/s."""
+    import torch
+
+    x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    best = 0.0
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        y.copy_(x)
+        e1.record()
+        e1.synchronize()
+        best = max(best, 2 * x.numel() * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+    del x, y
+    torch.cuda.empty_cache()
+    return best
+
+
 def mesh_tessellation(api, contours, precision, reps):
     """Stage times of contour3d_amd_mesh_build (the stored mesh) on the
     frame's objects, one warm-up build then `reps` timed ones, and of the
@@ -715,6 +739,7 @@ def main():
     achieved = dom_b / (fused_ms * 1e-3) / 1e9
     prof = profile_summary() if n == 512 else {}
     kprof = prof.get("kernels", {}).get("k_tess_raster<double,3>", {})
+    copy_gbs = copy_bandwidth()
     log("stored-mesh tessellation (mesh build) and smoothing, FP64 and FP32")
     tess = {}
     for name, prec in (("fp64", api.FP64), ("fp32", api.FP32)):
@@ -726,6 +751,7 @@ def main():
                       "emit_ms": round(st["emit"], 4), "triangles": int(tri),
                       "gcells_per_s": round(cells / (t_ms * 1e-3) / 1e9, 2), "algorithmic_bytes": round(b),
                       "hbm_frac": round(b / (t_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                      "frac_of_copy": round(b / (t_ms * 1e-3) / 1e9 / copy_gbs, 4),
                       "smooth_ms": round(st["smooth"], 4), "smooth_algorithmic_bytes": round(bs),
                       "smooth_hbm_frac": round(bs / (st["smooth"] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
                       if st["smooth"] > 0 else None}
@@ -750,6 +776,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_tess_raster<double> (fused tessellation + raster, FP64 frame)",
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": kprof.get("hbm_bytes"),
+                     "copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4),
                      "algorithmic_bytes": round(dom_b), "kernel_ms": fused_ms,
                      "algorithmic": "B_tess = 8 B x lattice points + 96 B x triangles",
                      "limiter": kprof.get("limiter"), "valu": kprof.get("valu"),
