@@ -354,7 +354,7 @@ Lib& lib() {
 constexpr uint64_t kScratchTiles = uint64_t(1) << 20;      // tile-count entries of all objects
 constexpr uint32_t kScratchObjs = 64;                      // contour objects
 constexpr uint64_t kScratchPixels = uint64_t(1) << 21;     // z-buffer entries (1920x1080 fits)
-constexpr uint32_t kScratchFrags = 1u << 20;               // fragment list entries
+constexpr uint32_t kScratchFrags = 1u << 19;               // fragment list entries
 constexpr uint64_t kScratchLineFrags = uint64_t(1) << 18;  // line fragments
 constexpr uint64_t kScratchTriangles = uint64_t(1) << 24;  // cell store entries
 constexpr uint64_t kScratchGridValues = uint64_t(1) << 16; // lattice coordinates of all objects
@@ -395,11 +395,11 @@ int carve_scratch(Frame& F, int device) {
   region(F.jimg, (2 * kScratchObjs + 2) * sizeof(uint32_t));
   region(F.rjobs, sizeof(RasterJob<double>));
   region(F.key, kScratchPixels * sizeof(uint64_t));
-  region(F.id, kScratchPixels * sizeof(uint32_t));
+  region(F.id, kScratchPixels * sizeof(uint64_t));
   region(F.fpix, (kScratchLineFrags + 1) * sizeof(uint32_t));
   region(F.fdep, (kScratchLineFrags + 1) * sizeof(double));
   region(F.frgb, 3 * (kScratchLineFrags + 1));
-  region(F.frags, size_t(kScratchFrags) * sizeof(Frag));
+  region(F.frags, size_t(kScratchFrags) * sizeof(Frag64));
   region(M.cell, kScratchTriangles * sizeof(uint32_t));
   region(F.grids, kScratchGridValues * sizeof(double));
   region(F.rgb, 0);
@@ -1205,7 +1205,8 @@ int raster_prepare(Frame& F, uint64_t npx) {
     return 1;
   }
   C3D_HIP(F.key.ensure(npx * sizeof(uint64_t)));
-  if (fp64) C3D_HIP(F.id.ensure(npx * sizeof(uint32_t)));
+  // FP64: 64-bit ids for fused frames (32-bit for stored meshes)
+  if (fp64) C3D_HIP(F.id.ensure(npx * sizeof(uint64_t)));
   C3D_HIP(F.fpix.ensure((nf + 1) * sizeof(uint32_t)));
   C3D_HIP(F.fdep.ensure((nf + 1) * sizeof(double)));
   C3D_HIP(F.frgb.ensure(3 * size_t(nf + 1)));
@@ -1219,7 +1220,7 @@ int raster_prepare(Frame& F, uint64_t npx) {
                          hipMemcpyHostToDevice, F.st));
   (void)fp64;
   if (F.frag_cap == 0) F.frag_cap = kDefaultFragCap;
-  C3D_HIP(F.frags.ensure(size_t(F.frag_cap) * sizeof(Frag)));
+  C3D_HIP(F.frags.ensure(size_t(F.frag_cap) * (fp64 ? sizeof(Frag64) : sizeof(Frag))));
   return 0;
 }
 
@@ -1240,7 +1241,7 @@ int raster_launch(Lib& L, Frame& F, const contour3d_amd_mesh& M, const RasterJob
   uint64_t* key = F.key.as<uint64_t>();
   const MeshView<R> mv = view_of<R>(M);
   if (prof_begin(F, CONTOUR3D_AMD_STAGE_RASTER)) return 1;
-  C3D_HIP(launch_zinit(key, id, npx, fp64 ? 0 : 1, hdr, F.st));
+  C3D_HIP(launch_zinit(key, id, false, npx, fp64 ? 0 : 1, false, hdr, F.st));
   if constexpr (sizeof(R) == 8) {
     // depth pass keeping the fragments that may end up nearest, then the id
     // pass over that list; when the list overflows, the geometry id pass
@@ -1248,15 +1249,15 @@ int raster_launch(Lib& L, Frame& F, const contour3d_amd_mesh& M, const RasterJob
     // next frame
     C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, kRasterDepth64, key, id, F.frags.as<Frag>(), F.frag_cap, rgrid,
                              F.st));
-    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 0, key, id, F.st));
-    C3D_HIP(launch_frag_ids(F.frags.as<Frag>(), F.frag_cap, hdr, key, id,
+    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 0, key, id, false, F.st));
+    C3D_HIP(launch_frag_ids(F.frags.p, false, F.frag_cap, hdr, key, id,
                             std::min<uint32_t>((F.frag_cap + 255) / 256, 4096), F.st));
     C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, kRasterId64, key, id, nullptr, 0, uint32_t(persistent_blocks()),
                              F.st));
-    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 1, 0, key, id, F.st));
+    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 1, 0, key, id, false, F.st));
   } else {
     C3D_HIP(launch_raster<R>(mv, rjobs, njobs, hdr, kRasterFp32, key, id, nullptr, 0, rgrid, F.st));
-    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 1, key, id, F.st));
+    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 1, key, id, false, F.st));
   }
   if (prof_end(F)) return 1;
   if (prof_begin(F, CONTOUR3D_AMD_STAGE_RESOLVE)) return 1;
@@ -1433,42 +1434,44 @@ int fused_launch(Lib& L, Frame& F, const contour3d_amd_mesh& M, const RasterJob<
                  uint32_t lines) {
   const bool fp64 = sizeof(R) == 8;
   FrameHdr* hdr = F.hdr.as<FrameHdr>();
-  uint32_t* id = fp64 ? F.id.as<uint32_t>() : nullptr;
+  void* id = fp64 ? F.id.p : nullptr;
   uint64_t* key = F.key.as<uint64_t>();
   const TessLayout& tl = F.tl;
   const TessJob* tj = F.tjobs.as<TessJob>();
   const uint32_t* jimg = F.jimg.as<uint32_t>();
-  const uint64_t* start = M.bstart.as<uint64_t>();
-  uint32_t* cell = M.cell.as<uint32_t>();
+  // FP64 frames name triangles by object, cell and slot: no tile starts
+  // and no cell store
+  const uint64_t* start = fp64 ? nullptr : M.bstart.as<uint64_t>();
+  uint32_t* cell = fp64 ? nullptr : M.cell.as<uint32_t>();
   const uint32_t grid = L.tess_grid(tl.ntj);
   const uint32_t fgrid = std::min<uint32_t>((F.frag_cap + 255) / 256, 4096);
-  Frag* frags = F.frags.as<Frag>();
+  void* frags = F.frags.p;
   if (prof_begin(F, CONTOUR3D_AMD_STAGE_RASTER)) return 1;
-  C3D_HIP(launch_zinit(key, id, npx, fp64 ? 0 : 1, hdr, F.st));
+  C3D_HIP(launch_zinit(key, id, fp64, npx, fp64 ? 0 : 1, fp64, hdr, F.st));
   if constexpr (sizeof(R) == 8) {
     // every covered pixel into the fragment list, then its keys; after an
     // overflow of the list (which grows for the next frame) the z-test
     // passes over the geometry run instead (each kernel decides on the
     // device)
     if (prof_inner_begin(F, CONTOUR3D_AMD_STAGE_FUSED)) return 1;
-    C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterAppend, key, id,
-                                  frags, F.frag_cap, grid, F.st));
+    C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, 0, rjobs, jimg, hdr, kRasterAppend, key, id, frags,
+                                  F.frag_cap, grid, F.st));
     if (prof_inner_end(F)) return 1;
-    C3D_HIP(launch_frag_keys(frags, F.frag_cap, hdr, key, fgrid, F.st));
-    C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterKey64, key, id,
+    C3D_HIP(launch_frag_keys(frags, true, F.frag_cap, hdr, key, fgrid, F.st));
+    C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, 0, rjobs, jimg, hdr, kRasterKey64, key, id,
                                   nullptr, 0, grid, F.st));
-    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 0, key, id, F.st));
-    C3D_HIP(launch_frag_ids(frags, F.frag_cap, hdr, key, id, fgrid, F.st));
-    C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterId64, key, id,
+    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 0, key, id, true, F.st));
+    C3D_HIP(launch_frag_ids(frags, true, F.frag_cap, hdr, key, id, fgrid, F.st));
+    C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, 0, rjobs, jimg, hdr, kRasterId64, key, id,
                                   nullptr, 0, grid, F.st));
-    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 1, 0, key, id, F.st));
+    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 1, 0, key, id, true, F.st));
   } else {
     // one pass: the packed key carries the id
     if (prof_inner_begin(F, CONTOUR3D_AMD_STAGE_FUSED)) return 1;
     C3D_HIP(launch_tess_raster<R>(tj, tl.njobs, tl.ntj, start, cell, M.cap, rjobs, jimg, hdr, kRasterFp32, key, id,
                                   nullptr, 0, grid, F.st));
     if (prof_inner_end(F)) return 1;
-    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 1, key, id, F.st));
+    C3D_HIP(launch_line_frags(F.fpix.as<uint32_t>(), F.fdep.as<double>(), lines, hdr, 0, 1, key, id, false, F.st));
   }
   if (prof_end(F)) return 1;
   if (prof_begin(F, CONTOUR3D_AMD_STAGE_RESOLVE)) return 1;
@@ -1495,12 +1498,18 @@ int graph_frame(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& img
   if (tess_prepare<R>(L, F, M, objs)) return 1;
   if (upload_pass_images(F, imgs)) return 1;
   if (raster_prepare<R>(F, npx)) return 1;
-  const bool ready = M.cap > 0 && M.cell.p && M.cap < kMaxFrameTriangles && F.h_nline <= line_capacity(F);
+  // FP64 frames have no cell store to outgrow
+  const bool ready =
+      (fp64 || (M.cap > 0 && M.cell.p && M.cap < kMaxFrameTriangles)) && F.h_nline <= line_capacity(F);
   if (!ready) {
     F.drop_graph();
-    FrameHdr hh{};
-    if (tess_head<R>(L, F, M, rjd, njobs, ~uint64_t(0))) return 1;
-    if (tess_readback<R>(F, M, hh, F.h_nline, true)) return 1;
+    if (!fp64) {
+      FrameHdr hh{};
+      if (tess_head<R>(L, F, M, rjd, njobs, ~uint64_t(0))) return 1;
+      if (tess_readback<R>(F, M, hh, F.h_nline, true)) return 1;
+    } else {
+      F.unread = true;
+    }
     if (fused_launch<R>(L, F, M, rjd, njobs, npx, F.h_nline)) return 1;
     return raster_copies(F, imgs, fp64);
   }
@@ -1519,7 +1528,7 @@ int graph_frame(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& img
     const bool prof_on = F.prof.on;
     F.prof.on = false;
     C3D_HIP(hipStreamBeginCapture(F.st, hipStreamCaptureModeThreadLocal));
-    int rc = tess_head<R>(L, F, M, rjd, njobs, M.cap);
+    int rc = fp64 ? 0 : tess_head<R>(L, F, M, rjd, njobs, M.cap);
     if (!rc) rc = fused_launch<R>(L, F, M, rjd, njobs, npx, lines);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(F.st, &g);
@@ -1581,12 +1590,26 @@ int run_frame_t(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& img
                          hipMemcpyHostToDevice, F.st));
   RasterJob<R>* rjd = F.rjobs.as<RasterJob<R>>();
   const int njobs = int(imgs.size());
+  // FP64 ids hold the object index below kLineObj - 1 (tri_id64)
+  if (sizeof(R) == 8 && objs.size() + 1 >= kLineObj) {
+    log_error("%zu contour objects exceed a frame's %llu", objs.size(), (unsigned long long)(kLineObj - 2));
+    return 1;
+  }
   if (graph) return graph_frame<R>(L, F, M, imgs, objs, rjd, npx, replayed);
   FrameHdr hh{};
   if (tess_prepare<R>(L, F, M, objs)) return 1;
   if (upload_pass_images(F, imgs)) return 1;
   // the line fragments' count goes to the header before the plan reads it
   if (raster_prepare<R>(F, npx)) return 1;
+  if constexpr (sizeof(R) == 8) {
+    // no count, scan or plan: triangles are named by object, cell and slot
+    // (tri_id64), and the frame's triangle total comes with its final
+    // header
+    if (fused_launch<R>(L, F, M, rjd, njobs, npx, F.h_nline)) return 1;
+    replayed = true;
+    F.unread = true;
+    return raster_copies(F, imgs, true);
+  }
   if (speculate && M.cap > 0 && M.cell.p && M.cap < kMaxFrameTriangles) {
     // planned against the cell store that earlier frames reserved, with no
     // read back of the totals (no synchronisation mid-frame); the plan

@@ -121,11 +121,30 @@ struct ObjMeta {
   uint64_t code_off;       // a stored mesh: first of its cells' corner codes (and cell starts)
 };
 
-// One covered (pixel, triangle) of the FP64 z-buffer pass.
+// One covered (pixel, triangle) of the stored-mesh FP64 z-buffer pass.
 struct Frag {
   uint32_t px;
   uint32_t id;
   uint64_t key;
+};
+
+// A triangle of an FP64 frame is named by tri_id64: (object + 1) << 40 |
+// cell << 4 | slot.  These ids ascend in the reference's drawing order
+// (contour, then cell index k, j, i, then slot; contour/main.c:49-112), so
+// "largest id" breaks depth ties as its sequential >= test does, and they
+// need no count pass: the resolve reads the object, cell and slot straight
+// from the id.  Line fragments follow every object (kLineObj).
+constexpr uint64_t kLineObj = 0xffffffull;
+__host__ __device__ inline uint64_t tri_id64(uint32_t obj, uint32_t cell, uint32_t slot) {
+  return (uint64_t(obj) + 1u) << 40 | uint64_t(cell) << 4 | slot;
+}
+
+// One covered (pixel, triangle) of a fused FP64 frame.
+struct Frag64 {
+  uint64_t key;
+  uint64_t id;  // tri_id64
+  uint32_t px;
+  uint32_t pad;
 };
 
 // Cells sharing each of the 19 cell-local lattice edges, as (dk, dj, di,
@@ -155,10 +174,11 @@ struct RasterJob {
 };
 
 // Frame-wide values on the device.  k_plan writes ntri, nwork and the
-// overflow bit 1; k_zinit clears nfrag and bit 2; the host writes nline
-// before the plan runs.
+// overflow bit 1 (meshes, FP32 frames); k_zinit clears nfrag and bit 2, and
+// in an FP64 frame, which has no plan, also ntri and bit 1, and its Append
+// pass then adds up ntri; the host writes nline before the plan runs.
 struct FrameHdr {
-  uint64_t ntri;        // triangles in the mesh
+  uint64_t ntri;        // triangles in the mesh (or the frame)
   uint64_t nwork;       // raster work items (sum of the images' triangle ranges)
   uint32_t overflow;    // bit 1: mesh capacity too small; bit 2: FP64 fragment list overflowed
   uint32_t nfrag;       // FP64 fragments kept by the depth pass
@@ -201,19 +221,26 @@ hipError_t launch_mesh_compare(const MeshView<double>& a, const ObjMeta* ma, con
 // Frames without a stored mesh: tessellation fused with the raster.  Every
 // triangle is drawn from registers into each image that shows its object
 // (jimg: per-field-pass offsets [njobs + 1] into a list of the images that
-// show any of the pass's objects) and only its cell index is stored
-// (cell[cap]).  `mode` is a raster mode.
+// show any of the pass's objects).  FP64 frames name triangles by
+// tri_id64 (id: uint64_t per pixel, frags: Frag64) and need no tile starts
+// (start, cell: null); their Append pass adds its triangles to hdr->ntri.
+// FP32 frames number them serially from the tile starts (id: uint32_t,
+// frags: Frag) and store each one's cell index (cell[cap]).  `mode` is a
+// raster mode.
 template <typename R>
 hipError_t launch_tess_raster(const TessJob* jobs, int njobs, uint32_t nitems, const uint64_t* start, uint32_t* cell,
                               uint64_t cap, const RasterJob<R>* rjobs, const uint32_t* jimg, FrameHdr* hdr, int mode,
-                              uint64_t* key, uint32_t* id, Frag* frags, uint32_t fcap, uint32_t grid, hipStream_t st);
+                              uint64_t* key, void* id, void* frags, uint32_t fcap, uint32_t grid, hipStream_t st);
 // resolve of a fused frame: the nearest triangles and their neighbours'
-// smoothing weights are recomputed from the fields
+// smoothing weights are recomputed from the fields (id as launch_tess_raster)
 template <typename R>
 hipError_t launch_resolve_field(const TessJob* jobs, const uint32_t* cell, const RasterJob<R>* rjobs, int njobs,
                                 uint64_t npx_total, const ObjMeta* meta, int nobj, const uint8_t* frag_rgb,
-                                const FrameHdr* hdr, const uint64_t* key, const uint32_t* id, hipStream_t st);
-hipError_t launch_zinit(uint64_t* key, uint32_t* id, uint64_t npx, int precision, FrameHdr* hdr,
+                                const FrameHdr* hdr, const uint64_t* key, const void* id, hipStream_t st);
+// clears the z-buffer (and the ids: uint64_t per pixel when wide, else
+// uint32_t), the header's fragment count and list-overflow bit, and with
+// fresh (FP64 frames) also its triangle count and capacity bit
+hipError_t launch_zinit(uint64_t* key, void* id, bool wide, uint64_t npx, int precision, bool fresh, FrameHdr* hdr,
                         hipStream_t st);
 // raster modes:
 //   kRasterFp32     FP32 z-test over packed (depth << 32 | id) keys
@@ -232,16 +259,18 @@ template <typename R>
 hipError_t launch_raster(const MeshView<R>& m, const RasterJob<R>* rjobs, int njobs, FrameHdr* hdr, int mode,
                          uint64_t* key, uint32_t* id, Frag* frags, uint32_t cap, uint32_t grid, hipStream_t st);
 // z-buffer keys of every listed fragment (reads the count from hdr; does
-// nothing when the list overflowed)
-hipError_t launch_frag_keys(const Frag* frags, uint32_t cap, const FrameHdr* hdr, uint64_t* key, uint32_t grid,
-                            hipStream_t st);
+// nothing when the list overflowed); wide: Frag64, else Frag
+hipError_t launch_frag_keys(const void* frags, bool wide, uint32_t cap, const FrameHdr* hdr, uint64_t* key,
+                            uint32_t grid, hipStream_t st);
 // FP64 id pass over the fragment list (reads the count from hdr; does
-// nothing when the list overflowed)
-hipError_t launch_frag_ids(const Frag* frags, uint32_t cap, const FrameHdr* hdr, const uint64_t* key,
-                           uint32_t* id, uint32_t grid, hipStream_t st);
-// line fragments [0, hdr->nline) of the frame; `cap` sizes the grid
+// nothing when the list overflowed); wide: Frag64 and uint64_t ids
+hipError_t launch_frag_ids(const void* frags, bool wide, uint32_t cap, const FrameHdr* hdr, const uint64_t* key,
+                           void* id, uint32_t grid, hipStream_t st);
+// line fragments [0, hdr->nline) of the frame; `cap` sizes the grid; wide:
+// uint64_t ids after every object (kLineObj), else uint32_t ids after the
+// frame's hdr->ntri triangles
 hipError_t launch_line_frags(const uint32_t* pix, const double* depth, uint32_t cap, const FrameHdr* hdr,
-                             int pass, int precision, uint64_t* key, uint32_t* id, hipStream_t st);
+                             int pass, int precision, uint64_t* key, void* id, bool wide, hipStream_t st);
 template <typename R>
 hipError_t launch_resolve(const MeshView<R>& m, const RasterJob<R>* rjobs, int njobs, uint64_t npx_total,
                           const ObjMeta* meta, int nobj, const uint8_t* frag_rgb, const FrameHdr* hdr,

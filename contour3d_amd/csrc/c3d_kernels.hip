@@ -9,6 +9,7 @@
 #include "c3d_internal.h"
 
 #include <cfloat>
+#include <type_traits>
 
 namespace c3d {
 
@@ -870,8 +871,9 @@ constexpr int kFragLds = 1024;
 // slots), one LDS atomic per wave; fragments that do not fit go straight to
 // the frame's list (gfrags, gcap entries, counted in hdr->nfrag).  Called
 // by any subset of a wave's lanes.
-__device__ __forceinline__ void append_frag(const Frag& fr, Frag* s_frag, uint32_t* s_n, uint32_t cap_lds,
-                                            Frag* __restrict__ gfrags, uint32_t gcap, FrameHdr* __restrict__ hdr) {
+template <typename FT>
+__device__ __forceinline__ void append_frag(const FT& fr, FT* s_frag, uint32_t* s_n, uint32_t cap_lds,
+                                            FT* __restrict__ gfrags, uint32_t gcap, FrameHdr* __restrict__ hdr) {
   const uint64_t act = __ballot(1);
   const int lane = int(__lane_id());
   const int leader = __ffsll((long long)act) - 1;
@@ -906,17 +908,27 @@ __device__ __forceinline__ uint32_t setup_triangle(const ViewDev<R>& v, const V3
   return uint32_t(P.i1 - P.i0 + 1) * uint32_t(P.j1 - P.j0 + 1);
 }
 
-// One candidate pixel (i, j) of triangle P: the reference's barycentric
-// test and harmonic depth (render.c:91-124), then the z-buffer update of
-// `MODE`.  Depth64 and Append put fragments in the workgroup list s_frag
-// (s_n entries, `cap_lds` slots); Append sends those that do not fit
+// The id type of a raster pass (uint32_t serial ids, or uint64_t
+// tri_id64 in the fused FP64 frames) and its fragment record.
+template <typename TId>
+using FragOf = typename std::conditional<sizeof(TId) == 8, Frag64, Frag>::type;
+
+__device__ __forceinline__ void atomic_max_id(uint32_t* p, uint32_t v) { atomicMax(p, v); }
+__device__ __forceinline__ void atomic_max_id(uint64_t* p, uint64_t v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
+}
+
+// One candidate pixel (i, j) of triangle P (id tid1): the reference's
+// barycentric test and harmonic depth (render.c:91-124), then the z-buffer
+// update of `MODE`.  Depth64 and Append put fragments in the workgroup list
+// s_frag (s_n entries, `cap_lds` slots); Append sends those that do not fit
 // straight to the frame's list (gfrags, gcap entries, counted in
 // hdr->nfrag).
-template <typename R, int MODE>
+template <typename R, int MODE, typename TId>
 __device__ __forceinline__ void pixel_update(const ViewDev<R>& v, uint64_t row0, const Projected<R>& P, int32_t i,
-                                             int32_t j, uint32_t tid1, uint64_t* __restrict__ key,
-                                             uint32_t* __restrict__ id, Frag* s_frag, uint32_t* s_n,
-                                             uint32_t cap_lds, Frag* __restrict__ gfrags, uint32_t gcap,
+                                             int32_t j, TId tid1, uint64_t* __restrict__ key, TId* __restrict__ id,
+                                             FragOf<TId>* s_frag, uint32_t* s_n, uint32_t cap_lds,
+                                             FragOf<TId>* __restrict__ gfrags, uint32_t gcap,
                                              FrameHdr* __restrict__ hdr) {
   R w0, w1, w2;
   if (!pixel_weights(P, pixel_centre<R>(i, v.fw), pixel_centre<R>(j, v.fh), w0, w1, w2)) return;
@@ -928,7 +940,7 @@ __device__ __forceinline__ void pixel_update(const ViewDev<R>& v, uint64_t row0,
     const uint64_t k = uint64_t(ord32(float(dep))) << 32 | uint64_t(tid1);
     if (k > key[px]) atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
   } else if constexpr (MODE == kRasterAppend) {
-    Frag fr;
+    FragOf<TId> fr;
     fr.px = uint32_t(px);
     fr.id = tid1;
     fr.key = sizeof(R) == 8 ? ord64(double(dep)) : (uint64_t(ord32(float(dep))) << 32 | uint64_t(tid1));
@@ -944,29 +956,29 @@ __device__ __forceinline__ void pixel_update(const ViewDev<R>& v, uint64_t row0,
     // at least the value read before its own atomic
     const uint32_t o = atomicAdd(s_n, 1u);
     if (o < cap_lds) {
-      Frag fr;
+      FragOf<TId> fr;
       fr.px = uint32_t(px);
       fr.id = tid1;
       fr.key = k;
       s_frag[o] = fr;
     }
   } else {
-    if (ord64(double(dep)) == key[px]) atomicMax(id + px, tid1);
+    if (ord64(double(dep)) == key[px]) atomic_max_id(id + px, tid1);
   }
 }
 
 // one triangle drawn by its own lane: every candidate of its box in turn
-template <typename R, int MODE>
-__device__ __forceinline__ void draw_triangle(const ViewDev<R>& v, uint64_t row0, const V3<R> wpos[3], uint32_t tid1,
-                                              uint64_t* __restrict__ key, uint32_t* __restrict__ id,
-                                              Frag* s_frag, uint32_t* s_n, uint32_t cap_lds,
-                                              Frag* __restrict__ gfrags = nullptr, uint32_t gcap = 0,
+template <typename R, int MODE, typename TId>
+__device__ __forceinline__ void draw_triangle(const ViewDev<R>& v, uint64_t row0, const V3<R> wpos[3], TId tid1,
+                                              uint64_t* __restrict__ key, TId* __restrict__ id,
+                                              FragOf<TId>* s_frag, uint32_t* s_n, uint32_t cap_lds,
+                                              FragOf<TId>* __restrict__ gfrags = nullptr, uint32_t gcap = 0,
                                               FrameHdr* __restrict__ hdr = nullptr) {
   Projected<R> P;
   if (!setup_triangle<R>(v, wpos, P)) return;
   for (int32_t j = P.j0; j <= P.j1; j++)
     for (int32_t i = P.i0; i <= P.i1; i++)
-      pixel_update<R, MODE>(v, row0, P, i, j, tid1, key, id, s_frag, s_n, cap_lds, gfrags, gcap, hdr);
+      pixel_update<R, MODE, TId>(v, row0, P, i, j, tid1, key, id, s_frag, s_n, cap_lds, gfrags, gcap, hdr);
 }
 
 // registers of one raster work item, loaded a round ahead
@@ -1063,8 +1075,8 @@ __global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, const Raster
 // that is flushed to the frame's list once half full; the z-test modes run
 // only after that list overflowed.  A frame flagged by the plan as outgrowing `cap`
 // draws nothing (it is rendered again).
-constexpr uint32_t kFragLdsFused = 768;  // LDS fragment slots of a workgroup
-constexpr uint32_t kFragFlushAt = 384;   // flushed to the frame's list from this fill on
+constexpr uint32_t kFragLdsFused = 512;  // LDS fragment slots of a workgroup (about 20 per drain at 512^3)
+constexpr uint32_t kFragFlushAt = 256;   // flushed to the frame's list from this fill on
 constexpr uint32_t kRing = 512;          // a power of two, at least 2 * kThreads
 
 struct FusedLds {
@@ -1115,9 +1127,9 @@ __device__ __forceinline__ void slot_positions_mem(const FieldDev& f, const Cell
 // frame's list once they fill kFragFlushAt slots (every time with `all`).
 // One reservation on the frame's counter per flush keeps that shared atomic
 // rare.
-template <int MODE>
-__device__ __forceinline__ void flush_frags(FusedLds& S, Frag* s_frag, FrameHdr* __restrict__ hdr,
-                                            Frag* __restrict__ frags, uint32_t fcap, bool all) {
+template <int MODE, typename FT>
+__device__ __forceinline__ void flush_frags(FusedLds& S, FT* s_frag, FrameHdr* __restrict__ hdr,
+                                            FT* __restrict__ frags, uint32_t fcap, bool all) {
   __syncthreads();
   if constexpr (MODE == kRasterAppend) {
     if (threadIdx.x == 0) {
@@ -1145,21 +1157,23 @@ __device__ __forceinline__ void flush_frags(FusedLds& S, Frag* s_frag, FrameHdr*
 }
 
 // Draws the n (<= kThreads) oldest queued triangles of field pass jb, one
-// per thread.
-template <typename R, int MODE>
-__device__ __forceinline__ void drain_ring(FusedLds& S, Frag* s_frag, uint32_t head, uint32_t n,
+// per thread: with 64-bit ids (FP64) as tri_id64, else as their serial
+// position + 1.
+template <typename R, int MODE, typename TId>
+__device__ __forceinline__ void drain_ring(FusedLds& S, FragOf<TId>* s_frag, uint32_t head, uint32_t n,
                                            const TessJob* __restrict__ jobs, uint32_t jb,
                                            const RasterJob<R>* __restrict__ rjobs, const uint32_t* __restrict__ jimg,
                                            FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
-                                           uint32_t* __restrict__ id, Frag* __restrict__ frags, uint32_t fcap) {
+                                           TId* __restrict__ id, FragOf<TId>* __restrict__ frags, uint32_t fcap) {
   V3<R> P[3];
   bool drawn = false;
-  uint32_t obj = 0, g = 0;
+  uint32_t obj = 0;
+  TId tid = 0;
   if (threadIdx.x < n) {
     const FieldDev f = jobs[jb].f;
     const uint32_t s = (head + threadIdx.x) & (kRing - 1u);
-    g = S.rg[s];
     const uint32_t cc = S.rc[s], x = S.re[s];
+    if constexpr (sizeof(TId) == 4) tid = S.rg[s] + 1u;
     const CellLoc L = locate(f, cc);
     if (interior_cell(f, L)) {
       const uint32_t q = (x >> 18) & 3u;
@@ -1171,6 +1185,7 @@ __device__ __forceinline__ void drain_ring(FusedLds& S, Frag* s_frag, uint32_t h
         obj = q == z ? jobs[jb].ts.obj[z] : obj;
       }
       slot_positions_mem<R>(f, L, thr, x & 0x3ffffu, P);
+      if constexpr (sizeof(TId) == 8) tid = tri_id64(obj, cc, (x >> 20) & 15u);
       drawn = true;
     }
   }
@@ -1180,7 +1195,8 @@ __device__ __forceinline__ void drain_ring(FusedLds& S, Frag* s_frag, uint32_t h
     const bool here = drawn && obj >= rjobs[k].obj0 && obj < rjobs[k].obj1;
     const ViewDev<R> v = rjobs[k].v;
     if (here)
-      draw_triangle<R, MODE>(v, rjobs[k].px0, P, g + 1u, key, id, s_frag, &S.nfrag, kFragLdsFused, frags, fcap, hdr);
+      draw_triangle<R, MODE, TId>(v, rjobs[k].px0, P, tid, key, id, s_frag, &S.nfrag, kFragLdsFused, frags, fcap,
+                                  hdr);
   }
   flush_frags<MODE>(S, s_frag, hdr, frags, fcap, false);
 }
@@ -1202,8 +1218,11 @@ __device__ __forceinline__ void drain_ring(FusedLds& S, Frag* s_frag, uint32_t h
 // every threshold q is stored at codeout[code_off + q * ncells + cell]: by
 // each cell's thread as bytes (kStoreCode 1) or from LDS as 4-byte words
 // (kStoreCode 2; measured faster in the FP64 emit, slower in the FP32 one).
-// Expects S.ntri loaded and a barrier since.
-template <bool kStoreCell, int kStoreCode, bool kReload, bool kPlanes, typename Drain>
+// With kSerial each queued triangle carries its serial position (from the
+// tile starts `start`); without it (fused FP64 frames, whose ids come from
+// the cell and slot) start is not read.  Expects S.ntri loaded and a
+// barrier since.
+template <bool kStoreCell, int kStoreCode, bool kReload, bool kPlanes, bool kSerial, typename Drain>
 __device__ __forceinline__ void ring_tiles(FusedLds& S, uint16_t* s_hi, const TessJob* __restrict__ jobs, int njobs,
                                            uint32_t nitems, const uint64_t* __restrict__ start,
                                            uint32_t* __restrict__ cellout, uint64_t cap, uint8_t* __restrict__ codeout,
@@ -1268,7 +1287,7 @@ __device__ __forceinline__ void ring_tiles(FusedLds& S, uint16_t* s_hi, const Te
       }
       uint64_t base[kMaxThr];
 #pragma unroll
-      for (int q = 0; q < kMaxThr; q++) base[q] = q < Ts().n ? start[Ts().boff[q] + tile] : 0;
+      for (int q = 0; q < kMaxThr; q++) base[q] = kSerial && q < Ts().n ? start[Ts().boff[q] + tile] : 0;
       for (uint32_t ch = 0; ch < nch; ch++) {
         const uint32_t c0 = ctile + ch * kThreads;
         const uint32_t p = ch * kThreads + threadIdx.x;
@@ -1354,15 +1373,17 @@ __device__ __forceinline__ void ring_tiles(FusedLds& S, uint16_t* s_hi, const Te
 #pragma unroll
             for (uint32_t step = kThreads / 2; step; step >>= 1) l += S.pref[q][l + step] <= r ? step : 0u;
             const uint32_t sl = r - S.pref[q][l];
-            uint64_t bq = base[0];
-#pragma unroll
-            for (int z = 1; z < kMaxThr; z++) bq = q == z ? base[z] : bq;
-            const uint64_t g = bq + r;
             const uint32_t cc = c0 + l;
-            if (kStoreCell && g < cap) cellout[g] = cc;
             const uint32_t cd = S.code[q][l];
             const uint32_t s = (head + npend + e) & (kRing - 1u);
-            S.rg[s] = uint32_t(g);
+            if constexpr (kSerial) {
+              uint64_t bq = base[0];
+#pragma unroll
+              for (int z = 1; z < kMaxThr; z++) bq = q == z ? base[z] : bq;
+              const uint64_t g = bq + r;
+              if (kStoreCell && g < cap) cellout[g] = cc;
+              S.rg[s] = uint32_t(g);
+            }
             S.rc[s] = cc;
             S.re[s] = (uint32_t(c_cslot[cd * 12u + sl]) & 0x3ffffu) | uint32_t(q) << 18 | sl << 20 | cd << 24;
           }
@@ -1396,26 +1417,34 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 || MODE == kRasterAppend ? 5 :
 k_tess_raster(const TessJob* __restrict__ jobs, int njobs, uint32_t nitems, const uint64_t* __restrict__ start,
               uint32_t* __restrict__ cellout, uint64_t cap, const RasterJob<R>* __restrict__ rjobs,
               const uint32_t* __restrict__ jimg, FrameHdr* __restrict__ hdr, uint64_t* __restrict__ key,
-              uint32_t* __restrict__ id, Frag* __restrict__ frags, uint32_t fcap) {
+              void* __restrict__ idv, void* __restrict__ fragv, uint32_t fcap) {
   static_assert(MODE != kRasterDepth64, "the fused kernel keeps every fragment (kRasterAppend)");
+  // FP64 frames name triangles by tri_id64, FP32 ones by serial position
+  using TId = typename std::conditional<sizeof(R) == 8, uint64_t, uint32_t>::type;
+  constexpr bool kSerial = sizeof(R) == 4;
+  TId* id = static_cast<TId*>(idv);
+  FragOf<TId>* frags = static_cast<FragOf<TId>*>(fragv);
   constexpr bool kPlanes = true;
   __shared__ FusedLds S;
   __shared__ uint16_t s_hi[kPlanes ? kTileCells : 1];
-  __shared__ Frag s_frag[MODE == kRasterAppend ? kFragLdsFused : 1];
+  __shared__ FragOf<TId> s_frag[MODE == kRasterAppend ? kFragLdsFused : 1];
   if (hdr->overflow & 1u) return;
   // the FP64 z-test passes run only after the fragment list overflowed
   if ((MODE == kRasterKey64 || MODE == kRasterId64) && !(hdr->overflow & 2u)) return;
   S.ntri[threadIdx.x] = c_ntri[threadIdx.x];
   if (threadIdx.x == 0) S.nfrag = 0;
   __syncthreads();
-  // the first pass of a frame (Append in FP64, the packed-key z-test in
-  // FP32) stores the cell indices
-  ring_tiles<MODE == kRasterAppend || MODE == kRasterFp32, 0, true, kPlanes>(
+  uint64_t ndrawn = 0;  // triangles drained (the same in every thread)
+  // the first pass of an FP32 frame stores the cell indices
+  ring_tiles<kSerial && (MODE == kRasterAppend || MODE == kRasterFp32), 0, true, kPlanes, kSerial>(
       S, s_hi, jobs, njobs, nitems, start, cellout, cap, nullptr, [&](uint32_t head, uint32_t n, uint32_t jb) {
-                                      drain_ring<R, MODE>(S, s_frag, head, n, jobs, jb, rjobs, jimg, hdr, key, id,
-                                                          frags, fcap);
-                                    });
+        ndrawn += n;
+        drain_ring<R, MODE, TId>(S, s_frag, head, n, jobs, jb, rjobs, jimg, hdr, key, id, frags, fcap);
+      });
   flush_frags<MODE>(S, s_frag, hdr, frags, fcap, true);
+  // an FP64 frame's triangle count, which no count pass gives it
+  if (!kSerial && MODE == kRasterAppend && threadIdx.x == 0 && ndrawn)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&hdr->ntri), (unsigned long long)ndrawn);
 }
 
 // ----------------------------------------------------------------------
@@ -1505,43 +1534,48 @@ __global__ __launch_bounds__(kThreads) void k_emit(const TessJob* __restrict__ j
   __shared__ uint16_t s_hi[kPl ? kTileCells : 1];
   S.ntri[threadIdx.x] = c_ntri[threadIdx.x];
   __syncthreads();
-  ring_tiles<false, sizeof(R) == 8 ? 2 : 1, false, kPl>(S, s_hi, jobs, njobs, nitems, start, nullptr, 0, m.code,
+  ring_tiles<false, sizeof(R) == 8 ? 2 : 1, false, kPl, true>(S, s_hi, jobs, njobs, nitems, start, nullptr, 0, m.code,
                                                           [&](uint32_t head, uint32_t n, uint32_t jb) {
     emit_ring<R>(S, s_pos, s_aux, head, n, jobs, jb, m, hdr);
   });
 }
 
 // the latest triangle among the kept fragments at each pixel's final depth
-__global__ void k_frag_ids(const Frag* __restrict__ frags, uint32_t cap, const FrameHdr* __restrict__ hdr,
-                           const uint64_t* __restrict__ key, uint32_t* __restrict__ id) {
+template <typename TId>
+__global__ void k_frag_ids(const FragOf<TId>* __restrict__ frags, uint32_t cap, const FrameHdr* __restrict__ hdr,
+                           const uint64_t* __restrict__ key, TId* __restrict__ id) {
   if (hdr->overflow & 2u) return;
   const uint32_t n = min(hdr->nfrag, cap);
   for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < n; f += gridDim.x * blockDim.x) {
-    const Frag fr = frags[f];
-    if (fr.key == key[fr.px]) atomicMax(id + fr.px, fr.id);
+    const FragOf<TId> fr = frags[f];
+    if (fr.key == key[fr.px]) atomic_max_id(id + fr.px, fr.id);
   }
 }
 
 // the z-buffer keys of every listed fragment
-__global__ void k_frag_keys(const Frag* __restrict__ frags, uint32_t cap, const FrameHdr* __restrict__ hdr,
+template <typename FT>
+__global__ void k_frag_keys(const FT* __restrict__ frags, uint32_t cap, const FrameHdr* __restrict__ hdr,
                             uint64_t* __restrict__ key) {
   if (hdr->overflow & 2u) return;
   const uint32_t n = min(hdr->nfrag, cap);
   for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < n; f += gridDim.x * blockDim.x) {
-    const Frag fr = frags[f];
+    const uint64_t fkey = frags[f].key;
+    const uint32_t fpx = frags[f].px;
     // keys only grow, so a value read here (however stale) that is at least
     // this key means the atomic could not change the pixel: skip it
-    if (fr.key > key[fr.px])
-      atomicMax(reinterpret_cast<unsigned long long*>(key + fr.px), (unsigned long long)fr.key);
+    if (fkey > key[fpx]) atomicMax(reinterpret_cast<unsigned long long*>(key + fpx), (unsigned long long)fkey);
   }
 }
 
-// also clears the frame header's raster state (fragment count, list overflow)
-__global__ void k_zinit(uint64_t* __restrict__ key, uint32_t* __restrict__ id, uint64_t npx, uint64_t k0,
+// also clears the frame header's raster state (fragment count, list
+// overflow) and, fresh, its triangle count and capacity bit
+template <typename TId>
+__global__ void k_zinit(uint64_t* __restrict__ key, TId* __restrict__ id, uint64_t npx, uint64_t k0, bool fresh,
                         FrameHdr* __restrict__ hdr) {
   if (hdr && blockIdx.x == 0 && threadIdx.x == 0) {
     hdr->nfrag = 0;
-    hdr->overflow &= 1u;
+    hdr->overflow &= fresh ? 0u : 1u;
+    if (fresh) hdr->ntri = 0;
   }
   for (uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < npx; p += uint64_t(gridDim.x) * blockDim.x) {
     key[p] = k0;
@@ -1550,14 +1584,16 @@ __global__ void k_zinit(uint64_t* __restrict__ key, uint32_t* __restrict__ id, u
 }
 
 // Line fragments (pixels and depths computed on the host, in drawing
-// order) take part in the same z-test; their ids follow every triangle's.
+// order) take part in the same z-test; their ids follow every triangle's
+// (64-bit: after every object, kLineObj; 32-bit: after the frame's ntri).
+template <typename TId>
 __global__ void k_line_frags(const uint32_t* __restrict__ pix, const double* __restrict__ depth,
                              const FrameHdr* __restrict__ hdr, int pass, int fp32, uint64_t* __restrict__ key,
-                             uint32_t* __restrict__ id) {
+                             TId* __restrict__ id) {
   const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= hdr->nline) return;
   const uint32_t px = pix[f];
-  const uint32_t lid = uint32_t(hdr->ntri + 1 + f);
+  const TId lid = sizeof(TId) == 8 ? TId(kLineObj << 40 | f) : TId(hdr->ntri + 1 + f);
   if (fp32) {
     const uint64_t k = uint64_t(ord32(float(depth[f]))) << 32 | uint64_t(lid);
     atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
@@ -1567,7 +1603,7 @@ __global__ void k_line_frags(const uint32_t* __restrict__ pix, const double* __r
   if (pass == 0) {
     atomicMax(reinterpret_cast<unsigned long long*>(key + px), (unsigned long long)k);
   } else if (k == __hip_atomic_load(key + px, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-    atomicMax(id + px, lid);
+    atomic_max_id(id + px, lid);
   }
 }
 
@@ -1708,7 +1744,7 @@ void k_resolve_field(const TessJob* __restrict__ jobs, const uint32_t* __restric
                      const RasterJob<R>* __restrict__ rjobs, int njobs, uint64_t npx_total,
                      const ObjMeta* __restrict__ meta, int nobj, const uint8_t* __restrict__ frag_rgb,
                      const FrameHdr* __restrict__ hdr, const uint64_t* __restrict__ key,
-                     const uint32_t* __restrict__ idbuf) {
+                     const void* __restrict__ idv) {
   // a frame whose triangles outgrew the cell store is rendered again: skip it
   if (hdr->overflow & 1u) return;
   const uint64_t ntri = hdr->ntri;
@@ -1731,43 +1767,62 @@ void k_resolve_field(const TessJob* __restrict__ jobs, const uint32_t* __restric
     const int32_t j = int32_t(lp / uint64_t(v.W));
     const int32_t i = int32_t(lp - uint64_t(j) * uint64_t(v.W));
     const uint64_t k = key[p];
-    uint32_t id;
-    double dep;
-    if constexpr (sizeof(R) == 8) {
-      id = idbuf[p];
-      dep = unord64(k);
-    } else {
-      id = uint32_t(k);
-      dep = id ? double(unord32(uint32_t(k >> 32))) : -DBL_MAX;
-    }
     uint8_t r = J.bg[0], gg = J.bg[1], b = J.bg[2];
-    if (id != 0 && id <= ntri && J.obj0 < J.obj1) {
-      const uint64_t t = id - 1;
-      // the winner is one of the image's own objects
-      int lo = int(J.obj0), hi = int(J.obj1) - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (meta[mid].base <= t) lo = mid;
-        else hi = mid - 1;
+    double dep;
+    // the winner: a triangle (its object, cell and slot) or a line fragment
+    bool tri = false, line = false;
+    uint32_t obj = 0, c = 0, sl = 0;
+    uint64_t fi = 0;
+    if constexpr (sizeof(R) == 8) {
+      // tri_id64: (object + 1) << 40 | cell << 4 | slot
+      const uint64_t id = static_cast<const uint64_t*>(idv)[p];
+      dep = unord64(k);
+      const uint64_t ob = id >> 40;
+      line = ob == kLineObj;
+      fi = id & ((uint64_t(1) << 40) - 1u);
+      tri = id != 0 && !line && ob - 1u >= J.obj0 && ob - 1u < J.obj1;
+      obj = uint32_t(ob - 1u);
+      c = uint32_t(id >> 4);
+      sl = uint32_t(id & 15u);
+    } else {
+      const uint32_t id = uint32_t(k);
+      dep = id ? double(unord32(uint32_t(k >> 32))) : -DBL_MAX;
+      line = id > ntri;
+      fi = id - ntri - 1;
+      if (id != 0 && id <= ntri && J.obj0 < J.obj1) {
+        const uint64_t t = id - 1;
+        // the winner is one of the image's own objects
+        int lo = int(J.obj0), hi = int(J.obj1) - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (meta[mid].base <= t) lo = mid;
+          else hi = mid - 1;
+        }
+        obj = uint32_t(lo);
+        const ObjMeta& om = meta[lo];
+        c = cell[t];
+        // Within an object the cells ascend with the serial position and
+        // a cell has at most 12 triangles, so the slot of t is the number
+        // of the 11 positions before it, inside the object, that hold the
+        // same cell.  The 11 loads are independent and share one or two
+        // lines.
+#pragma unroll
+        for (uint32_t d = 1; d < 12; d++) sl += t >= om.base + d && cell[t - d] == c ? 1u : 0u;
+        // a position outside the object can only come from a damaged cell
+        // store: such a pixel keeps the background
+        tri = t >= om.base && t - om.base < om.count;
       }
-      const ObjMeta om = meta[lo];
+    }
+    if (tri) {
+      const ObjMeta om = meta[obj];
       const FieldDev f = jobs[om.job].f;
       double thr = jobs[om.job].ts.thr[0];
       for (uint32_t z = 1; z < kMaxThr; z++) thr = om.q == z ? jobs[om.job].ts.thr[z] : thr;
-      const uint32_t c = cell[t];
-      // Within an object the cells ascend with the serial position and a
-      // cell has at most 12 triangles, so the slot of t is the number of
-      // the 11 positions before it, inside the object, that hold the same
-      // cell.  The 11 loads are independent and share one or two lines.
-      uint32_t sl = 0;
-#pragma unroll
-      for (uint32_t d = 1; d < 12; d++) sl += t >= om.base + d && cell[t - d] == c ? 1u : 0u;
-      // a position outside the object, a cell index outside the object's
-      // field or its interior, or a slot outside its cell, can only come
-      // from a damaged cell store: such a pixel keeps the background rather
-      // than read out of bounds
+      // a cell index outside the object's field or its interior, or a slot
+      // outside its cell, can only come from a damaged id or cell store:
+      // such a pixel keeps the background rather than read out of bounds
       do {
-        if (t < om.base || t - om.base >= om.count || c >= f.ncells) break;
+        if (c >= f.ncells) break;
         const CellLoc L = locate(f, c);
         if (!interior_cell(f, L)) break;
         double cv[8];
@@ -1794,8 +1849,7 @@ void k_resolve_field(const TessJob* __restrict__ jobs, const uint32_t* __restric
         gg = uint8_t(fct * R(om.rgb[1]));
         b = uint8_t(fct * R(om.rgb[2]));
       } while (false);
-    } else if (id > ntri) {
-      const uint64_t fi = id - ntri - 1;
+    } else if (line) {
       r = frag_rgb[3 * fi + 0];
       gg = frag_rgb[3 * fi + 1];
       b = frag_rgb[3 * fi + 2];
@@ -1879,9 +1933,14 @@ hipError_t launch_vertex_normals(const MeshView<R>& m, const ObjMeta* meta_dev, 
   return hipGetLastError();
 }
 
-hipError_t launch_zinit(uint64_t* key, uint32_t* id, uint64_t npx, int precision, FrameHdr* hdr, hipStream_t st) {
+hipError_t launch_zinit(uint64_t* key, void* id, bool wide, uint64_t npx, int precision, bool fresh, FrameHdr* hdr,
+                        hipStream_t st) {
   const uint64_t k0 = precision == 0 ? 0x0010000000000000ull /* ord64(-DBL_MAX) */ : 0ull;
-  hipLaunchKernelGGL(k_zinit, dim3(grid8((npx + 255) / 256, 4096)), dim3(256), 0, st, key, id, npx, k0, hdr);
+  const dim3 g(grid8((npx + 255) / 256, 4096));
+  if (wide)
+    hipLaunchKernelGGL(k_zinit<uint64_t>, g, dim3(256), 0, st, key, static_cast<uint64_t*>(id), npx, k0, fresh, hdr);
+  else
+    hipLaunchKernelGGL(k_zinit<uint32_t>, g, dim3(256), 0, st, key, static_cast<uint32_t*>(id), npx, k0, fresh, hdr);
   return hipGetLastError();
 }
 
@@ -1902,23 +1961,38 @@ hipError_t launch_raster(const MeshView<R>& m, const RasterJob<R>* rjobs, int nj
   return hipGetLastError();
 }
 
-hipError_t launch_frag_keys(const Frag* frags, uint32_t cap, const FrameHdr* hdr, uint64_t* key, uint32_t grid,
-                            hipStream_t st) {
-  hipLaunchKernelGGL(k_frag_keys, dim3(grid), dim3(256), 0, st, frags, cap, hdr, key);
+hipError_t launch_frag_keys(const void* frags, bool wide, uint32_t cap, const FrameHdr* hdr, uint64_t* key,
+                            uint32_t grid, hipStream_t st) {
+  if (wide)
+    hipLaunchKernelGGL(k_frag_keys<Frag64>, dim3(grid), dim3(256), 0, st, static_cast<const Frag64*>(frags), cap, hdr,
+                       key);
+  else
+    hipLaunchKernelGGL(k_frag_keys<Frag>, dim3(grid), dim3(256), 0, st, static_cast<const Frag*>(frags), cap, hdr,
+                       key);
   return hipGetLastError();
 }
 
-hipError_t launch_frag_ids(const Frag* frags, uint32_t cap, const FrameHdr* hdr, const uint64_t* key, uint32_t* id,
-                           uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_frag_ids, dim3(grid), dim3(256), 0, st, frags, cap, hdr, key, id);
+hipError_t launch_frag_ids(const void* frags, bool wide, uint32_t cap, const FrameHdr* hdr, const uint64_t* key,
+                           void* id, uint32_t grid, hipStream_t st) {
+  if (wide)
+    hipLaunchKernelGGL(k_frag_ids<uint64_t>, dim3(grid), dim3(256), 0, st, static_cast<const Frag64*>(frags), cap,
+                       hdr, key, static_cast<uint64_t*>(id));
+  else
+    hipLaunchKernelGGL(k_frag_ids<uint32_t>, dim3(grid), dim3(256), 0, st, static_cast<const Frag*>(frags), cap, hdr,
+                       key, static_cast<uint32_t*>(id));
   return hipGetLastError();
 }
 
 hipError_t launch_line_frags(const uint32_t* pix, const double* depth, uint32_t cap, const FrameHdr* hdr, int pass,
-                             int precision, uint64_t* key, uint32_t* id, hipStream_t st) {
+                             int precision, uint64_t* key, void* id, bool wide, hipStream_t st) {
   if (cap == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_line_frags, dim3((cap + 255) / 256), dim3(256), 0, st, pix, depth, hdr, pass, precision, key,
-                     id);
+  const dim3 g((cap + 255) / 256);
+  if (wide)
+    hipLaunchKernelGGL(k_line_frags<uint64_t>, g, dim3(256), 0, st, pix, depth, hdr, pass, precision, key,
+                       static_cast<uint64_t*>(id));
+  else
+    hipLaunchKernelGGL(k_line_frags<uint32_t>, g, dim3(256), 0, st, pix, depth, hdr, pass, precision, key,
+                       static_cast<uint32_t*>(id));
   return hipGetLastError();
 }
 
@@ -1947,7 +2021,7 @@ hipError_t launch_mesh_compare(const MeshView<double>& a, const ObjMeta* ma, con
 template <typename R>
 hipError_t launch_tess_raster(const TessJob* jobs, int njobs, uint32_t nitems, const uint64_t* start, uint32_t* cell,
                               uint64_t cap, const RasterJob<R>* rjobs, const uint32_t* jimg, FrameHdr* hdr, int mode,
-                              uint64_t* key, uint32_t* id, Frag* frags, uint32_t fcap, uint32_t grid, hipStream_t st) {
+                              uint64_t* key, void* id, void* frags, uint32_t fcap, uint32_t grid, hipStream_t st) {
   if (nitems == 0) return hipSuccess;
   const dim3 g(grid8(nitems, grid));
   if (mode == kRasterAppend) {
@@ -1973,7 +2047,7 @@ hipError_t launch_tess_raster(const TessJob* jobs, int njobs, uint32_t nitems, c
 template <typename R>
 hipError_t launch_resolve_field(const TessJob* jobs, const uint32_t* cell, const RasterJob<R>* rjobs, int njobs,
                                 uint64_t npx_total, const ObjMeta* meta, int nobj, const uint8_t* frag_rgb,
-                                const FrameHdr* hdr, const uint64_t* key, const uint32_t* id, hipStream_t st) {
+                                const FrameHdr* hdr, const uint64_t* key, const void* id, hipStream_t st) {
   hipLaunchKernelGGL(k_resolve_field<R>, dim3(grid8((npx_total + kThreads - 1) / kThreads, 1u << 30)), dim3(kThreads),
                      0, st, jobs, cell, rjobs, njobs, npx_total, meta, nobj, frag_rgb, hdr, key, id);
   return hipGetLastError();
@@ -1981,16 +2055,16 @@ hipError_t launch_resolve_field(const TessJob* jobs, const uint32_t* cell, const
 
 template hipError_t launch_tess_raster<double>(const TessJob*, int, uint32_t, const uint64_t*, uint32_t*, uint64_t,
                                                const RasterJob<double>*, const uint32_t*, FrameHdr*, int, uint64_t*,
-                                               uint32_t*, Frag*, uint32_t, uint32_t, hipStream_t);
+                                               void*, void*, uint32_t, uint32_t, hipStream_t);
 template hipError_t launch_tess_raster<float>(const TessJob*, int, uint32_t, const uint64_t*, uint32_t*, uint64_t,
                                               const RasterJob<float>*, const uint32_t*, FrameHdr*, int, uint64_t*,
-                                              uint32_t*, Frag*, uint32_t, uint32_t, hipStream_t);
+                                              void*, void*, uint32_t, uint32_t, hipStream_t);
 template hipError_t launch_resolve_field<double>(const TessJob*, const uint32_t*, const RasterJob<double>*, int,
                                                  uint64_t, const ObjMeta*, int, const uint8_t*, const FrameHdr*,
-                                                 const uint64_t*, const uint32_t*, hipStream_t);
+                                                 const uint64_t*, const void*, hipStream_t);
 template hipError_t launch_resolve_field<float>(const TessJob*, const uint32_t*, const RasterJob<float>*, int,
                                                 uint64_t, const ObjMeta*, int, const uint8_t*, const FrameHdr*,
-                                                const uint64_t*, const uint32_t*, hipStream_t);
+                                                const uint64_t*, const void*, hipStream_t);
 template hipError_t launch_plan<double>(const uint64_t*, uint64_t, ObjMeta*, int, RasterJob<double>*, int, FrameHdr*,
                                         uint64_t, hipStream_t);
 template hipError_t launch_plan<float>(const uint64_t*, uint64_t, ObjMeta*, int, RasterJob<float>*, int, FrameHdr*,
