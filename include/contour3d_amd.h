@@ -87,8 +87,10 @@ extern int contour3d_amd_execute_rgb(
  *   rgb        device, W*H*3 bytes in PPM row order
  *   depth      device W*H doubles, or NULL
  * Limits (1 with an error line beyond them): 2^21 pixels (1920x1080 fits),
- * 2^24 triangles, 64 contour objects, 2^20 tessellation tiles (about 2^31
- * cells), 2^18 line fragments, 65536 lattice coordinates.  Results are
+ * 2^24 triangles in FP32 (FP64 frames store nothing per triangle), 64
+ * contour objects, 2^20 tessellation tiles (about 2^31 cells), 2^18 line
+ * fragments, 65536 lattice coordinates; an FP64 frame with more than 2^19
+ * covered pixel centres takes the slower z-test passes.  Results are
  * bit-identical to contour3d_amd_execute_rgb.  Every call on a GPU shares
  * that GPU's array and keeps the last call's contents; calls are serialised
  * by the library mutex, and calls that overlapped on one GPU would race.
@@ -134,8 +136,9 @@ typedef struct {
  * to what contour3d_amd_execute_rgb returns for it alone.  Images whose
  * contour lists are equal (same arrays, lattices, converters, thresholds,
  * colours, in order) share one tessellation: several cameras on one scene
- * cost one tessellation.  Limits: fewer than 2^31 pixels and 2^32 - 2^24
- * triangles per batch.
+ * cost one tessellation.  Limits: fewer than 2^31 pixels per batch, and in
+ * FP32 2^32 - 2^24 triangles per batch (FP64 frames name triangles by
+ * object, cell and slot and have no such limit).
  */
 extern int contour3d_amd_execute_batch(size_t num_images, const contour3d_amd_image_t * images, int precision);
 
@@ -282,7 +285,7 @@ enum {
                                            tessellation + raster, emit); each walks several tiles
                                            (0: the defaults, 128 per CU, and one per tile for emit) */
   CONTOUR3D_AMD_TUNE_RASTER_GRID = 1,   /* max workgroups of the raster passes (0 = one per 256 triangles) */
-  CONTOUR3D_AMD_TUNE_FRAG_CAPACITY = 2, /* fragment list entries, 16 B each (0: the default, 2^24; grows by
+  CONTOUR3D_AMD_TUNE_FRAG_CAPACITY = 2, /* FP64 fragment list entries, 24 B each (0: the default, 2^24; grows by
                                            itself after an overflow; an overflowing frame takes the slower
                                            z-test passes over the geometry) */
   CONTOUR3D_AMD_TUNE_GRAPHS = 3,        /* 1: contour3d_execute, _execute_rgb and _execute_batch capture
