@@ -219,8 +219,9 @@ struct contour3d_amd_mesh {
 namespace c3d {
 namespace {
 
-// initial fragment-list entries of a frame (16 B each): every covered pixel
-// of a 512^3 field's frame at 1920x1080 fits
+// initial fragment-list entries of an FP64 frame (24 B each; 16 B for a
+// stored mesh): every covered pixel of a 512^3 field's frame at 1920x1080
+// fits
 constexpr uint32_t kDefaultFragCap = 1u << 24;
 
 // Host-side layout of a frame's tessellation.
