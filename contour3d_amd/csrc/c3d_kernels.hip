@@ -861,12 +861,11 @@ __device__ __forceinline__ int find_rjob(const RasterJob<R>* __restrict__ jobs, 
 //                   list holds every fragment whose key equals the final
 //                   value.  Overflow (LDS or global) sets bit 2 of
 //                   hdr->overflow.
-//   kRasterId64     FP64 id pass over the geometry: the latest serial id at
-//                   each pixel's final depth; runs only when the fragment
-//                   list overflowed
+//   kRasterId64     FP64 id pass over the geometry: the latest id (serial,
+//                   or tri_id64 in the fused frames) at each pixel's final
+//                   depth; runs only when the fragment list overflowed
 constexpr int kFragLds = 1024;
 
-// Draws one triangle (world positions wpos, serial id tid1 - 1) into an
 // Appends fr to the workgroup's LDS fragment list (s_n entries, cap_lds
 // slots), one LDS atomic per wave; fragments that do not fit go straight to
 // the frame's list (gfrags, gcap entries, counted in hdr->nfrag).  Called
@@ -1061,13 +1060,15 @@ __global__ __launch_bounds__(kThreads) void k_raster(MeshView<R> m, const Raster
 // Each tile is classified and scanned in chunks of 256 cells as in the emit
 // pass, but a chunk holds only about a third as many triangles as cells, so
 // its triangles are not drawn in place.  They are queued instead, as
-// compact items (serial position, cell, vertex corners, threshold), in a
-// per-workgroup LDS ring that carries them over chunk and tile boundaries
-// (within one field pass), and the ring is drained 256 items at a time:
+// compact items (serial position in FP32, cell, vertex corners,
+// threshold), in a per-workgroup LDS ring that carries them over chunk and
+// tile boundaries (within one field pass), and the ring is drained 256
+// items at a time:
 // every lane then computes, projects and draws one triangle.  A drained
 // triangle rereads the corner values its vertices need (they were loaded
-// moments earlier by the same workgroup, so they come from cache).  Only
-// the cell index of each triangle is stored (at its serial position); the
+// moments earlier by the same workgroup, so they come from cache).  An FP32
+// frame stores only the cell index of each triangle (at its serial
+// position), an FP64 frame nothing (its ids name the cell and slot); the
 // resolve recomputes what it needs from the field.  Triangles are drawn
 // into every image of the field pass that shows their object (jimg:
 // offsets per field pass into a list of the images showing any of its
@@ -1733,8 +1734,9 @@ __device__ V3<R> vertex_normal_field(const FieldDev& f, double thr, uint32_t c, 
 }
 
 // k_resolve for a frame without a stored mesh: the nearest triangle is
-// found from its serial id (object among the image's objects by base, cell
-// from the stored cell index, slot from the cell store's neighbouring
+// found from its id (FP64: the object, cell and slot fields of tri_id64;
+// FP32, a serial id: the object among the image's objects by base, the cell
+// from the stored cell index, the slot from the cell store's neighbouring
 // entries) and recomputed from its field.  The kernel is latency-bound (a
 // chain of dependent gathers per pixel), so occupancy pays: the FP32 form
 // runs at 4 waves per SIMD (128 VGPRs, a small spill), the FP64 form at 3.
