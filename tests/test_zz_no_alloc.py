@@ -81,3 +81,34 @@ def test_noalloc_refuses_what_it_cannot_hold(capfd):
     assert noalloc(v, cont, [], api.FP64, rgb, None) == 0
     torch.cuda.synchronize()
     assert np.array_equal(rgb.cpu().numpy(), ref)
+
+
+def test_noalloc_fp64_beyond_its_fragment_list(capfd):
+    """An FP64 frame with more covered pixel centres than the array's 2^19
+    fragment slots takes the z-test passes over the geometry and still
+    renders, bit for bit, what execute_rgb renders through its (growing)
+    fragment list."""
+    banner("no-alloc entry point: FP64 fragment list overflow, 160^3 at 1920x1080")
+    import torch
+
+    assert api.lib().contour3d_amd_init(0) == 0
+    n = 160
+    f = torch.from_numpy(fields.turbulence((n, n, n), kmax=24, seed=44)).cuda()
+    g = fields.uniform_grid(n)
+    cont = [dict(field=f, grids=(g, g, g), threshold=t, color=c)
+            for t, c in ((-0.8, (250, 60, 30)), (0.0, (40, 220, 90)), (0.8, (60, 120, 255)))]
+    v = fields.example_view(1920, 1080)
+    v["bg"] = (0, 0, 0)
+    # zoomed in twice (halved screen vectors) so that the lattice fills the
+    # image: about 1.5 M covered pixels where the example view covers 0.46 M
+    v["local"] = tuple(tuple(0.5 * x for x in a) for a in v["local"])
+    ref_rgb, ref_dep = api.execute_rgb(v["camera"], v["light"], v["size"], v["center"], v["local"], v["bg"], cont,
+                                       precision=api.FP64, want_depth=True)
+    # every covered pixel holds at least one fragment: more than the list holds
+    assert int((ref_dep > -1e300).sum()) > (1 << 19)
+    rgb = torch.zeros((1080, 1920, 3), dtype=torch.uint8, device="cuda")
+    dep = torch.zeros((1080, 1920), dtype=torch.float64, device="cuda")
+    assert noalloc(v, cont, [], api.FP64, rgb, dep) == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(rgb.cpu().numpy(), ref_rgb)
+    assert same_bits(dep.cpu().numpy(), ref_dep)
