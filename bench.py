@@ -43,6 +43,8 @@ Beside it, in the same JSON line:
   cpu_baseline  the CPU oracle timed on a bounded slab of the same field
 """
 import argparse
+import contextlib
+import gc
 import json
 import os
 import sys
@@ -159,6 +161,22 @@ def mesh_tessellation(api, contours, precision, reps):
     return out, ntri / reps
 
 
+@contextlib.contextmanager
+def _gc_paused():
+    """Timed regions run with Python's cyclic garbage collector paused (after
+    one collection), as timeit runs its loops: a full collection with torch
+    loaded takes about 44 ms (measured with gc.callbacks), and one that
+    landed inside a timed loop was charged to the library's frames."""
+    enabled = gc.isenabled()
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        if enabled:
+            gc.enable()
+
+
 def run_form(api, contours, view, precision, steps, warmup, rgb_dev):
     """`warmup` untimed frames, then `steps` timed frames with the stage
     profiler off (as a caller runs them), then `steps` more with it on for
@@ -173,11 +191,12 @@ def run_form(api, contours, view, precision, steps, warmup, rgb_dev):
     for _ in range(warmup):
         frame()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        frame()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    with _gc_paused():
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            frame()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
     api.profile_reset()
     api.profile_enable(True)
     for _ in range(steps):
@@ -269,11 +288,12 @@ def _timed(fn, reps):
     import torch
 
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        fn()
-    torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / reps
+    with _gc_paused():
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
 
 
 def shared_scene(api, fields, contours, reps):
@@ -300,9 +320,10 @@ def shared_scene(api, fields, contours, reps):
             rep_s.append(_timed(lambda: plain(prec), 1))
         res[f"plain_{name}_fps"] = round(len(views) * reps / sum(rep_s), 2)
         res[f"plain_{name}_rep_ms_per_frame"] = [round(t / len(views) * 1e3, 3) for t in rep_s]
-        # the median round: an occasional one-off delay in a single round
-        # (seen in the first round after the 128^3 C1 frames, outside the
-        # frames' kernels) moves the mean, not the median
+        # the median round beside the mean: a one-off delay in one round
+        # moves the mean, not the median (a full garbage collection of the
+        # Python process, about 44 ms, had landed in single rounds before the
+        # timed loops paused the collector)
         res[f"plain_{name}_fps_median_round"] = round(len(views) / float(np.median(rep_s)), 2)
         # the same loop once more with the stage timers on: the device time
         # of the frames' stages beside the wall time per frame, so that a
@@ -517,18 +538,19 @@ def served_stream(api, fields, nreq, rate, nfield_pool, seed=0, windows=(0.0, 2.
                 srv.wait(srv.submit(image_of(r), api.FP32))
             st0 = srv.stats()
             th = threading.Thread(target=reaper)
-            th.start()
-            t0 = time.perf_counter()
-            for r in reqs:
-                while time.perf_counter() - t0 < r[0]:
-                    time.sleep(0.0002)
-                ticket = srv.submit(image_of(r), api.FP32)
-                with lock:
-                    order.append(ticket)
-                solver_step(r[1])  # at once: the request renders its snapshot
-            done.set()
-            th.join()
-            span = time.perf_counter() - t0
+            with _gc_paused():
+                th.start()
+                t0 = time.perf_counter()
+                for r in reqs:
+                    while time.perf_counter() - t0 < r[0]:
+                        time.sleep(0.0002)
+                    ticket = srv.submit(image_of(r), api.FP32)
+                    with lock:
+                        order.append(ticket)
+                    solver_step(r[1])  # at once: the request renders its snapshot
+                done.set()
+                th.join()
+                span = time.perf_counter() - t0
             st = srv.stats()
         finally:
             srv.stop()
@@ -544,16 +566,17 @@ def served_stream(api, fields, nreq, rate, nfield_pool, seed=0, windows=(0.0, 2.
               "solver_steps": nreq, "snapshot_fields": True, "runs": [serve(w) for w in windows]}
     # the same arrivals through plain single-image calls, one at a time
     plat = []
-    t0 = time.perf_counter()
-    for r in reqs:
-        while time.perf_counter() - t0 < r[0]:
-            time.sleep(0.0002)
-        im = image_of(r)
-        api.execute_rgb(im["camera"], im["light"], im["size"], im["center"], im["local"], im["bg"], im["contours"],
-                        precision=api.FP32, rgb=im["rgb"])
-        plat.append((time.perf_counter() - t0 - r[0]) * 1e3)
-        solver_step(r[1])
-    span = time.perf_counter() - t0
+    with _gc_paused():
+        t0 = time.perf_counter()
+        for r in reqs:
+            while time.perf_counter() - t0 < r[0]:
+                time.sleep(0.0002)
+            im = image_of(r)
+            api.execute_rgb(im["camera"], im["light"], im["size"], im["center"], im["local"], im["bg"],
+                            im["contours"], precision=api.FP32, rgb=im["rgb"])
+            plat.append((time.perf_counter() - t0 - r[0]) * 1e3)
+            solver_step(r[1])
+        span = time.perf_counter() - t0
     plat.sort()
     pct = lambda v, q: v[min(len(v) - 1, int(q * (len(v) - 1) + 0.5))]  # noqa: E731
     served["plain"] = {"p50_ms": round(pct(plat, 0.5), 2), "p99_ms": round(pct(plat, 0.99), 2),
@@ -567,22 +590,24 @@ def served_stream(api, fields, nreq, rate, nfield_pool, seed=0, windows=(0.0, 2.
     with api.Server(max_images=16, hbm_budget=200e9) as srv:
         for rep in range(3):
             st0 = srv.stats()
-            t1 = time.perf_counter()
-            ts = [srv.submit(im, api.FP32) for im in sat_imgs]
-            for t_ in ts:
-                srv.wait(t_)
-            rate_ = nsat / (time.perf_counter() - t1)
+            with _gc_paused():
+                t1 = time.perf_counter()
+                ts = [srv.submit(im, api.FP32) for im in sat_imgs]
+                for t_ in ts:
+                    srv.wait(t_)
+                rate_ = nsat / (time.perf_counter() - t1)
             st1 = srv.stats()
             if rate_ > sat_served:
                 sat_served = rate_
                 sat_batch = (st1["completed"] - st0["completed"]) / max(1, st1["batches"] - st0["batches"])
     sat_plain = 0.0
     for rep in range(3):
-        t1 = time.perf_counter()
-        for im in sat_imgs:
-            api.execute_rgb(im["camera"], im["light"], im["size"], im["center"], im["local"], im["bg"],
-                            im["contours"], precision=api.FP32, rgb=im["rgb"])
-        sat_plain = max(sat_plain, nsat / (time.perf_counter() - t1))
+        with _gc_paused():
+            t1 = time.perf_counter()
+            for im in sat_imgs:
+                api.execute_rgb(im["camera"], im["light"], im["size"], im["center"], im["local"], im["bg"],
+                                im["contours"], precision=api.FP32, rgb=im["rgb"])
+            sat_plain = max(sat_plain, nsat / (time.perf_counter() - t1))
     served["saturated"] = {"requests": nsat, "served_frames_per_s": round(sat_served, 2),
                            "plain_frames_per_s": round(sat_plain, 2), "mean_batch": round(sat_batch, 2)}
     return served
