@@ -19,6 +19,7 @@
 #include <condition_variable>
 #include <deque>
 #include <string>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -72,10 +73,17 @@ bool injected_alloc_failure() {
 
 // grow-only device buffer, or (fixed) a region of the static scratch array
 // that never grows
+// A number no device buffer allocation has had before (DevBuf::gen).
+uint64_t next_buf_gen() {
+  static std::atomic<uint64_t> g{0};
+  return ++g;
+}
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
   bool fixed = false;
+  uint64_t gen = 0;  // the current allocation's number (0: none)
   ~DevBuf() = default;
   // grows to at least `want` bytes, with a quarter of headroom unless
   // `exact` (a growing workload reallocates rarely; each reallocation frees
@@ -94,9 +102,11 @@ struct DevBuf {
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) {
       p = nullptr;
+      gen = 0;
       return e;
     }
     bytes = n;
+    gen = next_buf_gen();
     return hipSuccess;
   }
   void release() {
@@ -104,6 +114,7 @@ struct DevBuf {
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    gen = 0;
   }
   template <typename T>
   T* as() const {
@@ -235,9 +246,20 @@ struct TessLayout {
 // Stream, buffers and host staging of one frame.  The synchronous entry
 // points use the library's main frame; the serving loop alternates two, so
 // that one frame's host work and device work overlap the other's.
+// The bytes last copied from the host to one place of a device buffer
+// (Frame::memo), under the buffer's allocation number then.
+struct UploadMemo {
+  uint64_t gen = 0;
+  std::vector<unsigned char> bytes;
+};
+
 struct Frame {
   hipStream_t st = nullptr;
   hipEvent_t done = nullptr;
+  // per (buffer, byte offset): the last table uploaded there (upload_table);
+  // every buffer takes whole tables at fixed offsets, never overlapping
+  std::map<std::pair<const void*, size_t>, UploadMemo> memo;
+  std::vector<double> h_grids;  // the allocation-free frame's grid region, on the host
   // the allocation-free frame: every buffer a fixed region of the scratch
   // array; device fields and identity lattices only (their grids go to
   // `grids`)
@@ -384,6 +406,7 @@ int carve_scratch(Frame& F, int device) {
     b.p = static_cast<unsigned char*>(base) + off;
     b.bytes = bytes;
     b.fixed = true;
+    b.gen = next_buf_gen();
     off += (bytes + 255) / 256 * 256;
   };
   contour3d_amd_mesh& M = F.mesh;
@@ -858,6 +881,27 @@ bool same_obj(const contour3d_contour_obj_t& a, const contour3d_contour_obj_t& b
 
 // A frame keeps only the cell index of each triangle (the fused raster
 // draws triangles from registers): 1.25x the triangles seen.
+// Copies n bytes of host memory to device buffer b at byte offset off on
+// F's stream, unless the same bytes were the last ones copied there and b
+// has not been reallocated since: a repeated frame then sends only the
+// tables that changed (each small copy from pageable memory costs about
+// 12 us of host time, and the device waits for it).  Kernels may rewrite
+// parts of an uploaded table (k_plan: objects' bases and counts, images'
+// triangle ranges) only where every frame that reads them rewrites them.
+int upload_table(Frame& F, DevBuf& b, size_t off, const void* src, size_t n) {
+  if (n == 0) return 0;
+  UploadMemo& m = F.memo[{static_cast<const void*>(&b), off}];
+  if (b.gen != 0 && m.gen == b.gen && m.bytes.size() == n && std::memcmp(m.bytes.data(), src, n) == 0) return 0;
+  m.gen = 0;
+  C3D_HIP(hipMemcpyAsync(b.as<unsigned char>() + off, src, n, hipMemcpyHostToDevice, F.st));
+  m.bytes.assign(static_cast<const unsigned char*>(src), static_cast<const unsigned char*>(src) + n);
+  m.gen = b.gen;
+  return 0;
+}
+
+// forgets what upload_table last sent to (b, off): something else wrote it
+void forget_table(Frame& F, const DevBuf& b, size_t off) { F.memo.erase({static_cast<const void*>(&b), off}); }
+
 int ensure_cells(contour3d_amd_mesh& M) {
   if (M.cap >= M.total && M.cap != 0 && M.cell.p) return 0;
   const uint64_t cap = M.total + M.total / 4 + 1024;
@@ -895,6 +939,13 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
     return 1;
   }
   size_t grid_used = 0;  // lattice coordinates in the allocation-free frame's grid region
+  // and where each object's grid goes there: (first value, source, values)
+  struct GridPiece {
+    size_t at;
+    const double* src;
+    size_t n;
+  };
+  std::vector<GridPiece> grid_pieces;
   std::vector<FieldDev> fd(n);
   std::vector<int> group(n, -1);
   std::vector<uint32_t> boff(n + 1, 0);
@@ -940,7 +991,7 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
         double* gd = F.grids.as<double>() + grid_used;
         size_t go = 0;
         for (int d = 0; d < 3; d++) {
-          C3D_HIP(hipMemcpyAsync(gd + go, ob.grids[d], ob.glsizes[d] * sizeof(double), hipMemcpyDefault, F.st));
+          grid_pieces.push_back({grid_used + go, ob.grids[d], size_t(ob.glsizes[d])});
           go += ob.glsizes[d];
         }
         grid_used += need;
@@ -1090,9 +1141,24 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
     hm[o].rgb[1] = objs[o]->color.g;
     hm[o].rgb[2] = objs[o]->color.b;
   }
+  if (!grid_pieces.empty()) {
+    // the grid region as one table (its pieces change places with the
+    // objects' lattices), unless a grid lives on the device
+    bool dev = false;
+    for (const GridPiece& g : grid_pieces) dev = dev || is_device_ptr(g.src);
+    if (dev) {
+      forget_table(F, F.grids, 0);
+      for (const GridPiece& g : grid_pieces)
+        C3D_HIP(hipMemcpyAsync(F.grids.as<double>() + g.at, g.src, g.n * sizeof(double), hipMemcpyDefault, F.st));
+    } else {
+      F.h_grids.resize(grid_used);
+      for (const GridPiece& g : grid_pieces) std::memcpy(F.h_grids.data() + g.at, g.src, g.n * sizeof(double));
+      if (upload_table(F, F.grids, 0, F.h_grids.data(), grid_used * sizeof(double))) return 1;
+    }
+  }
   if (n) {
-    C3D_HIP(hipMemcpyAsync(M.meta.p, hm.data(), n * sizeof(ObjMeta), hipMemcpyHostToDevice, F.st));
-    C3D_HIP(hipMemcpyAsync(F.tjobs.p, tj.data(), tj.size() * sizeof(TessJob), hipMemcpyHostToDevice, F.st));
+    if (upload_table(F, M.meta, 0, hm.data(), n * sizeof(ObjMeta))) return 1;
+    if (upload_table(F, F.tjobs, 0, tj.data(), tj.size() * sizeof(TessJob))) return 1;
   }
   F.tl.nbt = nbt;
   F.tl.ntj = ntj;
@@ -1164,6 +1230,7 @@ int tessellate(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<const 
                RasterJob<R>* rjobs, int njobs, FrameHdr& hh) {
   if (tess_prepare<R>(L, F, M, objs)) return 1;
   // a mesh has no line fragments (the plan checks the frame's ids)
+  forget_table(F, F.hdr, offsetof(FrameHdr, nline));
   C3D_HIP(hipMemsetAsync(F.hdr.as<unsigned char>() + offsetof(FrameHdr, nline), 0, sizeof(uint32_t), F.st));
   if (tess_head<R>(L, F, M, rjobs, njobs, ~uint64_t(0))) return 1;
   if (tess_readback<R>(F, M, hh, 0)) return 1;
@@ -1217,8 +1284,7 @@ int raster_prepare(Frame& F, uint64_t npx) {
     C3D_HIP(hipMemcpyAsync(F.frgb.p, LF.rgb.data(), 3 * size_t(nf), hipMemcpyHostToDevice, F.st));
   }
   F.h_nline = nf;
-  C3D_HIP(hipMemcpyAsync(F.hdr.as<unsigned char>() + offsetof(FrameHdr, nline), &F.h_nline, sizeof(uint32_t),
-                         hipMemcpyHostToDevice, F.st));
+  if (upload_table(F, F.hdr, offsetof(FrameHdr, nline), &F.h_nline, sizeof(uint32_t))) return 1;
   (void)fp64;
   if (F.frag_cap == 0) F.frag_cap = kDefaultFragCap;
   C3D_HIP(F.frags.ensure(size_t(F.frag_cap) * (fp64 ? sizeof(Frag64) : sizeof(Frag))));
@@ -1423,8 +1489,7 @@ int upload_pass_images(Frame& F, const std::vector<Image>& imgs) {
   }
   J[nj] = uint32_t(J.size());
   C3D_HIP(F.jimg.ensure(J.size() * sizeof(uint32_t)));
-  C3D_HIP(hipMemcpyAsync(F.jimg.p, J.data(), J.size() * sizeof(uint32_t), hipMemcpyHostToDevice, F.st));
-  return 0;
+  return upload_table(F, F.jimg, 0, J.data(), J.size() * sizeof(uint32_t));
 }
 
 // A frame without a stored mesh: z-buffer clear, the fused tessellation and
@@ -1587,8 +1652,7 @@ int run_frame_t(Lib& L, Frame& F, contour3d_amd_mesh& M, std::vector<Image>& img
   if (frame_lines(F, imgs)) return 1;
   fill_rjobs<R>(F, imgs);
   C3D_HIP(F.rjobs.ensure(imgs.size() * sizeof(RasterJob<R>)));
-  C3D_HIP(hipMemcpyAsync(F.rjobs.p, F.rjobs_host<R>().data(), imgs.size() * sizeof(RasterJob<R>),
-                         hipMemcpyHostToDevice, F.st));
+  if (upload_table(F, F.rjobs, 0, F.rjobs_host<R>().data(), imgs.size() * sizeof(RasterJob<R>))) return 1;
   RasterJob<R>* rjd = F.rjobs.as<RasterJob<R>>();
   const int njobs = int(imgs.size());
   // FP64 ids hold the object index below kLineObj - 1 (tri_id64)
@@ -1703,8 +1767,7 @@ int render_mesh_t(Lib& L, Frame& F, const contour3d_amd_mesh& M, std::vector<Ima
   if (frame_lines(F, imgs)) return 1;
   fill_rjobs<R>(F, imgs);
   C3D_HIP(F.rjobs.ensure(imgs.size() * sizeof(RasterJob<R>)));
-  C3D_HIP(hipMemcpyAsync(F.rjobs.p, F.rjobs_host<R>().data(), imgs.size() * sizeof(RasterJob<R>),
-                         hipMemcpyHostToDevice, F.st));
+  if (upload_table(F, F.rjobs, 0, F.rjobs_host<R>().data(), imgs.size() * sizeof(RasterJob<R>))) return 1;
   RasterJob<R>* rjd = F.rjobs.as<RasterJob<R>>();
   C3D_HIP(F.hdr.ensure(sizeof(FrameHdr)));
   C3D_HIP(F.hstage.ensure(sizeof(FrameHdr)));
