@@ -1300,7 +1300,7 @@ uint32_t line_capacity(const Frame& F) {
 // z-buffer passes, lines and resolve of every image over the mesh M;
 // `lines` sizes the line kernels' grid (at least the frame's line fragments)
 template <typename R>
-int raster_launch(Lib& L, Frame& F, const contour3d_amd_mesh& M, const RasterJob<R>* rjobs, int njobs, uint64_t npx,
+int raster_launch(Lib&, Frame& F, const contour3d_amd_mesh& M, const RasterJob<R>* rjobs, int njobs, uint64_t npx,
                   uint32_t rgrid, uint32_t lines) {
   const bool fp64 = sizeof(R) == 8;
   FrameHdr* hdr = F.hdr.as<FrameHdr>();

@@ -408,7 +408,7 @@ __global__ __launch_bounds__(kThreads) void k_scan_reduce(const uint32_t* __rest
   }
   s[threadIdx.x] = acc;
   __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
+  for (uint32_t w = kThreads / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
     __syncthreads();
   }
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(kThreads) void k_scan_tiles(uint64_t* __restrict__ 
     const uint64_t v = i < ntile ? tile_sum[i] : 0;
     s[threadIdx.x] = v;
     __syncthreads();
-    for (int d = 1; d < kThreads; d <<= 1) {
+    for (uint32_t d = 1; d < kThreads; d <<= 1) {
       const uint64_t y = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
       __syncthreads();
       s[threadIdx.x] += y;
@@ -1747,6 +1747,7 @@ void k_resolve_field(const TessJob* __restrict__ jobs, const uint32_t* __restric
                      const ObjMeta* __restrict__ meta, int nobj, const uint8_t* __restrict__ frag_rgb,
                      const FrameHdr* __restrict__ hdr, const uint64_t* __restrict__ key,
                      const void* __restrict__ idv) {
+  (void)nobj;  // objects are found from the image's range (and, in FP64, the id)
   // a frame whose triangles outgrew the cell store is rendered again: skip it
   if (hdr->overflow & 1u) return;
   const uint64_t ntri = hdr->ntri;
