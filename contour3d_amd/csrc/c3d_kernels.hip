@@ -72,7 +72,9 @@ __device__ __forceinline__ CellLoc locate(const FieldDev& f, uint32_t c) {
 }
 
 __device__ __forceinline__ uint64_t corner_offset(const FieldDev& f, uint32_t c) {
-  return uint64_t(c & 1u) + uint64_t((c >> 1) & 1u) * f.nx + uint64_t(c >> 2) * f.nx * f.ny;
+  // selects rather than 64-bit products: c is a corner number (0-7)
+  const uint64_t nxy = uint64_t(f.nx) * f.ny;
+  return uint64_t(c & 1u) + uint64_t(c & 2u ? f.nx : 0u) + (c & 4u ? nxy : 0ull);
 }
 
 template <typename R>
@@ -1178,13 +1180,10 @@ __device__ __forceinline__ void drain_ring(FusedLds& S, FragOf<TId>* s_frag, uin
     const CellLoc L = locate(f, cc);
     if (interior_cell(f, L)) {
       const uint32_t q = (x >> 18) & 3u;
-      double thr = jobs[jb].ts.thr[0];
-      obj = jobs[jb].ts.obj[0];
-#pragma unroll
-      for (uint32_t z = 1; z < kMaxThr; z++) {
-        thr = q == z ? jobs[jb].ts.thr[z] : thr;
-        obj = q == z ? jobs[jb].ts.obj[z] : obj;
-      }
+      // one load per lane indexed by the threshold number: fewer vector
+      // instructions and spilled registers than selecting among kMaxThr
+      const double thr = jobs[jb].ts.thr[q];
+      obj = jobs[jb].ts.obj[q];
       slot_positions_mem<R>(f, L, thr, x & 0x3ffffu, P);
       if constexpr (sizeof(TId) == 8) tid = tri_id64(obj, cc, (x >> 20) & 15u);
       drawn = true;
@@ -1493,9 +1492,7 @@ __device__ __forceinline__ void emit_ring(FusedLds& S, MeshPos<R>* s_pos, MeshAu
     const uint32_t cc = S.rc[s], x = S.re[s];
     const CellLoc L = locate(f, cc);
     const uint32_t q = (x >> 18) & 3u;
-    double thr = jobs[jb].ts.thr[0];
-#pragma unroll
-    for (uint32_t z = 1; z < kMaxThr; z++) thr = q == z ? jobs[jb].ts.thr[z] : thr;
+    const double thr = jobs[jb].ts.thr[q];
     V3<R> P[3];
     slot_positions_mem<R>(f, L, thr, x & 0x3ffffu, P);
     const V3<R> w = smoothing_weight<R>(P);
