@@ -1082,6 +1082,10 @@ constexpr uint32_t kFragLdsFused = 512;  // LDS fragment slots of a workgroup (a
 constexpr uint32_t kFragFlushAt = 256;   // flushed to the frame's list from this fill on
 constexpr uint32_t kRing = 512;          // a power of two, at least 2 * kThreads
 
+// a queued triangle names its threshold in 2 bits (ring item bits 18-19),
+// and the drains index ThrSet::thr and ::obj with it
+static_assert(kMaxThr == 4, "the ring's threshold field is 2 bits wide");
+
 struct FusedLds {
   uint8_t code[kMaxThr][kThreads];
   uint16_t pref[kMaxThr][kThreads];
