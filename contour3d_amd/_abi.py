@@ -33,7 +33,7 @@ class ContourObj(C.Structure):
     _fields_ = [
         ("pencil", C.c_int),
         ("glsizes", C.c_size_t * 3),
-        ("grids", C.POINTER(C.c_double) * 3),
+        ("grids", C.c_void_p * 3),  # const double* grids[3], set as addresses
         ("converter", C.c_void_p),
         ("threshold", C.c_double),
         ("color", Color),
@@ -104,6 +104,3 @@ def color(c):
     return Color(int(c[0]), int(c[1]), int(c[2]))
 
 
-def dptr(addr):
-    """A raw address as a double* for the grids[] members."""
-    return C.cast(C.c_void_p(addr), C.POINTER(C.c_double))

@@ -105,8 +105,15 @@ def lib():
     return _lib
 
 
+_IDENTITY = None
+
+
 def identity_address():
-    return C.cast(lib().contour3d_amd_identity, C.c_void_p).value
+    """Address of contour3d_amd_identity (looked up once per process)."""
+    global _IDENTITY
+    if _IDENTITY is None:
+        _IDENTITY = C.cast(lib().contour3d_amd_identity, C.c_void_p).value
+    return _IDENTITY
 
 
 def _addr(a):
@@ -117,7 +124,7 @@ def _addr(a):
 
 
 def _shape(a):
-    return tuple(int(x) for x in a.shape)
+    return tuple(a.shape)  # ints for numpy arrays and torch tensors alike
 
 
 class _Keep:
@@ -141,11 +148,25 @@ def _unit_grid(m):
     return g
 
 
+def _grid_address(g, keep, seen):
+    """Address of grid g as contiguous float64; `seen` maps id(g) to it for
+    the grids of one call (contours usually share their grid arrays, and
+    `keep` holds every array for the call's duration, so no id is reused)."""
+    a = seen.get(id(g))
+    if a is None:
+        h = np.ascontiguousarray(g, dtype=np.float64)
+        keep.items += [g, h]
+        a = h.__array_interface__["data"][0]
+        seen[id(g)] = a
+    return a
+
+
 def contour_objs(contours, keep):
     """ctypes array of contour3d_contour_obj_t from dicts with keys field
     (nz, ny, nx), threshold, color and optional grids / converter."""
     ident = identity_address()
     objs = (_abi.ContourObj * max(1, len(contours)))()
+    seen = {}
     for n, c in enumerate(contours):
         f = c["field"]
         if isinstance(f, np.ndarray):
@@ -156,17 +177,14 @@ def contour_objs(contours, keep):
         grids = c.get("grids")
         if grids is None:
             grids = tuple(_unit_grid(m) for m in (nx, ny, nz))
-        grids = [np.ascontiguousarray(g, dtype=np.float64) for g in grids]
-        keep.items += [f] + grids
+        keep.items.append(f)
         conv = c.get("converter", ident)
         if not isinstance(conv, int):
             keep.items.append(conv)
             conv = C.cast(conv, C.c_void_p).value
         o = objs[n]
-        o.pencil = 0
-        o.glsizes[:] = [nx, ny, nz]
-        for d in range(3):
-            o.grids[d] = _abi.dptr(grids[d].ctypes.data)
+        o.glsizes = (nx, ny, nz)
+        o.grids = tuple(_grid_address(g, keep, seen) for g in grids)
         o.converter = conv
         o.threshold = float(c["threshold"])
         o.color = _abi.color(c["color"])

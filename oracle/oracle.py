@@ -135,7 +135,7 @@ def execute(camera, light, size, center, local, bg, contours, lines=(), fname=No
         o.pencil = 0
         o.glsizes[:] = [nx, ny, nz]
         for d in range(3):
-            o.grids[d] = _abi.dptr(grids[d].ctypes.data)
+            o.grids[d] = grids[d].ctypes.data
         o.converter = conv
         o.threshold = float(c["threshold"])
         o.color = _abi.color(c["color"])
