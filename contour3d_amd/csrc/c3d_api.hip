@@ -552,9 +552,18 @@ bool is_device_ptr(const void* p) {
   return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
+// FNV-1a over 8-byte words (then the remaining bytes): a filter for the
+// geometry cache, whose hits are confirmed by comparing the values
 uint64_t fnv(const void* data, size_t bytes, uint64_t h) {
   const unsigned char* c = static_cast<const unsigned char*>(data);
-  for (size_t q = 0; q < bytes; q++) {
+  size_t q = 0;
+  for (; q + 8 <= bytes; q += 8) {
+    uint64_t w;
+    std::memcpy(&w, c + q, 8);
+    h ^= w;
+    h *= 1099511628211ull;
+  }
+  for (; q < bytes; q++) {
     h ^= c[q];
     h *= 1099511628211ull;
   }
@@ -951,9 +960,25 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
   std::vector<uint32_t> boff(n + 1, 0);
   std::vector<const double*> upl_src;
   std::vector<void*> upl_dst;
+  // per object: whether its array is device memory, and its lattice; an
+  // object with the array, or the converter, sizes and grid pointers, of an
+  // earlier object of this call takes that object's (the frame's objects
+  // usually share one field and lattice)
+  std::vector<char> on_dev(n, 0);
+  std::vector<Geom*> geo(n, nullptr);
   for (size_t o = 0; o < n; o++) {
     const contour3d_contour_obj_t& ob = *objs[o];
     if (validate_obj(ob, o)) return 1;
+    int same_array = -1, same_lattice = -1;
+    for (size_t q = 0; q < o; q++) {
+      const contour3d_contour_obj_t& oq = *objs[q];
+      if (same_array < 0 && oq.array == ob.array) same_array = int(q);
+      if (same_lattice < 0 && oq.converter == ob.converter &&
+          std::memcmp(oq.glsizes, ob.glsizes, sizeof(ob.glsizes)) == 0 &&
+          std::memcmp(oq.grids, ob.grids, sizeof(ob.grids)) == 0)
+        same_lattice = int(q);
+    }
+    on_dev[o] = same_array >= 0 ? on_dev[same_array] : char(is_device_ptr(ob.array));
     FieldDev& f = fd[o];
     f.nx = uint32_t(ob.glsizes[0]);
     f.ny = uint32_t(ob.glsizes[1]);
@@ -970,7 +995,7 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
     if (F.noalloc) {
       // the allocation-free frame reads device fields in place and the
       // identity map of grids copied into its scratch region
-      if (!is_device_ptr(ob.array) || ob.converter != contour3d_amd_identity) {
+      if (!on_dev[o] || ob.converter != contour3d_amd_identity) {
         log_error("contour object %zu: the allocation-free call takes device fields and the "
                   "contour3d_amd_identity converter", o);
         return 1;
@@ -1001,7 +1026,7 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
       f.gy = f.gx + f.nx;
       f.gz = f.gy + f.ny;
       f.cx = f.cy = f.cz = nullptr;
-    } else if (is_device_ptr(ob.array)) {
+    } else if (on_dev[o]) {
       f.f = ob.array;
     } else {
       const size_t bytes = size_t(f.nx) * f.ny * f.nz * sizeof(double);
@@ -1020,8 +1045,9 @@ int tess_prepare(Lib& L, Frame& F, contour3d_amd_mesh& M, const std::vector<cons
       f.f = static_cast<const double*>(dst);
     }
     if (!F.noalloc) {
-      Geom* g = nullptr;
-      if (get_geometry(L, ob, &g)) return 1;
+      Geom* g = same_lattice >= 0 ? geo[same_lattice] : nullptr;
+      if (!g && get_geometry(L, ob, &g)) return 1;
+      geo[o] = g;
       g->busy++;
       F.geoms_used.push_back(g);
       f.gx = g->grids.as<double>();
